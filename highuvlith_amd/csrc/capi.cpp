@@ -1,0 +1,537 @@
+// C ABI of include/huv.h: host engine entry points (FP64) and the device
+// context / kernel-set / batched SOCS image entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/huv.h"
+#include "capi_internal.hpp"
+#include "hip/socs_kernels.hpp"
+#include "host/engine.hpp"
+
+namespace {
+thread_local std::string g_err;
+}
+
+namespace huvi {
+
+huv_status fail(huv_status code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+huv_status hip_fail(hipError_t e, const char* what) {
+  return fail(HUV_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* Workspace::get(int slot, size_t bytes, hipError_t& err) {
+  err = hipSuccess;
+  if (bytes == 0) bytes = 256;
+  if (size[slot] >= bytes) return ptr[slot];
+  if (ptr[slot]) {
+    err = hipFree(ptr[slot]);
+    ptr[slot] = nullptr;
+    size[slot] = 0;
+    if (err != hipSuccess) return nullptr;
+  }
+  err = hipMalloc(&ptr[slot], bytes);
+  if (err != hipSuccess) {
+    ptr[slot] = nullptr;
+    return nullptr;
+  }
+  size[slot] = bytes;
+  return ptr[slot];
+}
+
+void Workspace::release() {
+  for (int i = 0; i < kSlots; ++i) {
+    if (ptr[i]) hipFree(ptr[i]);
+    ptr[i] = nullptr;
+    size[i] = 0;
+  }
+}
+
+const float2* Ctx::twiddles(int N, hipError_t& err) {
+  err = hipSuccess;
+  auto it = tw.find(N);
+  if (it != tw.end()) return it->second;
+  std::vector<float2> h((size_t)N);
+  for (int k = 0; k < N; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)N;
+    h[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  float2* d = nullptr;
+  err = hipMalloc(&d, sizeof(float2) * N);
+  if (err != hipSuccess) return nullptr;
+  err = hipMemcpy(d, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    hipFree(d);
+    return nullptr;
+  }
+  tw[N] = d;
+  return d;
+}
+
+const float* Ctx::sinc_table(int n, int B, bool pixel_transfer, hipError_t& err) {
+  err = hipSuccess;
+  const long key = ((long)n << 20) | ((long)B << 1) | (pixel_transfer ? 1 : 0);
+  auto it = sinc.find(key);
+  if (it != sinc.end()) return it->second;
+  const int W = 2 * B + 1;
+  std::vector<float> h((size_t)W);
+  for (int k = -B; k <= B; ++k) {
+    double v = 1.0;
+    if (pixel_transfer) {
+      const double u = M_PI * (double)k / (double)n;
+      v = std::fabs(u) < 1e-6 ? 1.0 - u * u / 6.0 : std::sin(u) / u;
+    }
+    h[k + B] = (float)v;
+  }
+  float* d = nullptr;
+  err = hipMalloc(&d, sizeof(float) * W);
+  if (err != hipSuccess) return nullptr;
+  err = hipMemcpy(d, h.data(), sizeof(float) * W, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    hipFree(d);
+    return nullptr;
+  }
+  sinc[key] = d;
+  return d;
+}
+
+Ctx::~Ctx() {
+  hipSetDevice(device);
+  if (own_stream) hipStreamSynchronize(own_stream);
+  ws.release();
+  for (auto& kv : tw) hipFree(kv.second);
+  for (auto& kv : sinc) hipFree(kv.second);
+  for (auto& kv : plane_tables) hipFree(kv.second);
+  if (own_stream) hipStreamDestroy(own_stream);
+}
+
+// Band spectrum stage for nclips real masks on the n grid into Mq (layout
+// [clip][q][kx+B][ky+B], twisted per x class of coarse grid m).
+huv_status band_stage(Ctx* c, int nclips, int n, int B, int m, const float* masks, float2* Mq,
+                      bool pixel_transfer) {
+  hipError_t e;
+  const float2* tw_n = c->twiddles(n, e);
+  if (!tw_n) return hip_fail(e, "twiddle table");
+  const float2* tw_m = c->twiddles(m, e);
+  if (!tw_m) return hip_fail(e, "twiddle table");
+  const float* st = c->sinc_table(n, B, pixel_transfer, e);
+  if (!st) return hip_fail(e, "sinc table");
+  const size_t tmp_bytes = (size_t)nclips * (2 * B + 1) * (n / 2) * sizeof(float2);
+  float2* tmp = (float2*)c->ws.get(Workspace::TMP_ROWS, tmp_bytes, e);
+  if (!tmp) return hip_fail(e, "workspace");
+  e = huvk::launch_rows_band(n, B, nclips, masks, tw_n, tmp, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "rows_band launch");
+  e = huvk::launch_cols_band(n, B, m, nclips, tmp, tw_n, tw_m, st, Mq, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "cols_band launch");
+  return HUV_OK;
+}
+
+}  // namespace huvi
+
+using namespace huvi;
+
+// ===========================================================================
+// Host engine
+// ===========================================================================
+struct huv_host_kernels {
+  huv::KernelSet ks;
+};
+
+static huv::Optics to_optics(const huv_optics* o) {
+  huv::Optics r;
+  r.kind = o->kind;
+  r.na = o->na;
+  r.immersion_index = o->immersion_index;
+  r.flare = o->flare;
+  r.central_obscuration = o->central_obscuration;
+  r.transmission = o->transmission;
+  r.paraxial_defocus = o->paraxial_defocus;
+  return r;
+}
+
+static huv::Illumination to_ill(const huv_illumination* i) {
+  huv::Illumination r;
+  r.kind = i->kind;
+  for (int k = 0; k < 6; ++k) r.p[k] = i->p[k];
+  return r;
+}
+
+extern "C" {
+
+const char* huv_last_error(void) { return g_err.c_str(); }
+int huv_abi_version(void) { return 100; }
+
+double huv_illumination_at(const huv_illumination* ill, double sx, double sy) {
+  if (!ill) return 0.0;
+  return huv::illumination_at(to_ill(ill), sx, sy);
+}
+
+huv_status huv_sample_source(const huv_illumination* ill, int points_per_axis, double df_sigma,
+                             double* xyw, int64_t capacity, int64_t* count) {
+  if (!ill || !count) return fail(HUV_INVALID_PARAMETER, "null argument");
+  try {
+    const huv::Illumination s = to_ill(ill);
+    std::vector<huv::SourcePoint> pts;
+    std::string err;
+    const int st = huv::sample_intensity(
+        [&](double x, double y) { return huv::illumination_at(s, x, y); }, points_per_axis,
+        df_sigma, pts, err);
+    if (st != huv::OK) return fail(st, err);
+    *count = (int64_t)pts.size();
+    if (xyw)
+      for (int64_t i = 0; i < std::min<int64_t>(capacity, *count); ++i) {
+        xyw[3 * i] = pts[i].sx;
+        xyw[3 * i + 1] = pts[i].sy;
+        xyw[3 * i + 2] = pts[i].weight;
+      }
+    return HUV_OK;
+  } catch (const std::exception& ex) {
+    return fail(HUV_INTERNAL_ERROR, ex.what());
+  }
+}
+
+huv_status huv_pupil(const huv_optics* optics, double fx, double fy, double z, double lambda,
+                     double* re, double* im) {
+  if (!optics || !re || !im) return fail(HUV_INVALID_PARAMETER, "null argument");
+  const huv::cd p = huv::pupil_function(to_optics(optics), fx, fy, z, lambda);
+  *re = p.real();
+  *im = p.imag();
+  return HUV_OK;
+}
+
+int huv_select_kernel_count(const double* values, int64_t nvals, double trace,
+                            int64_t max_kernels, double fraction) {
+  return huv::select_kernel_count(values, nvals, trace, max_kernels, fraction);
+}
+
+huv_status huv_kernels_build(const huv_optics* optics, const double* xyw, int64_t npoints, int n,
+                             double pixel_nm, double defocus_nm, double wavelength_nm,
+                             int max_kernels, double energy_fraction, int method,
+                             huv_host_kernels** out) {
+  if (!optics || !xyw || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (max_kernels < 1) return fail(HUV_INVALID_PARAMETER, "max_kernels: must be at least 1");
+  if (!(energy_fraction > 0.0 && energy_fraction <= 1.0))
+    return fail(HUV_INVALID_PARAMETER, "kernel_energy_fraction: must be in (0, 1]");
+  if (n < 2 || !(std::isfinite(pixel_nm) && pixel_nm > 0.0))
+    return fail(HUV_INVALID_PARAMETER, "grid: needs size >= 2 and a positive finite pixel_nm");
+  if (!(std::isfinite(wavelength_nm) && wavelength_nm > 0.0))
+    return fail(HUV_INVALID_PARAMETER, "wavelength_nm: must be positive and finite");
+  const double medium = optics->kind == 0 ? optics->immersion_index : 1.0;
+  if (!(std::isfinite(medium) && medium >= 1.0 && optics->na > 0.0 && optics->na < medium))
+    return fail(HUV_INVALID_PARAMETER,
+                "numerical_aperture: must be positive and below the image-space index");
+  try {
+    std::vector<huv::SourcePoint> pts((size_t)npoints);
+    for (int64_t i = 0; i < npoints; ++i) pts[i] = {xyw[3 * i], xyw[3 * i + 1], xyw[3 * i + 2]};
+    huv::BuildSettings bs;
+    bs.max_kernels = max_kernels;
+    bs.kernel_energy_fraction = energy_fraction;
+    bs.method = method;
+    auto h = std::make_unique<huv_host_kernels>();
+    std::string err;
+    const int st = huv::build_kernel_set(to_optics(optics), pts, n, pixel_nm, defocus_nm,
+                                         wavelength_nm, bs, h->ks, err);
+    if (st != huv::OK) return fail(st, err);
+    *out = h.release();
+    return HUV_OK;
+  } catch (const std::exception& ex) {
+    return fail(HUV_INTERNAL_ERROR, ex.what());
+  }
+}
+
+huv_status huv_kernels_info_get(const huv_host_kernels* k, huv_kernels_info* info) {
+  if (!k || !info) return fail(HUV_INVALID_PARAMETER, "null argument");
+  const huv::KernelSet& s = k->ks;
+  info->n = s.n;
+  info->pixel_nm = s.pixel_nm;
+  info->num_kernels = s.num_kernels();
+  info->num_support = s.num_support();
+  info->num_rows = (int64_t)s.nonzero_rows.size();
+  info->method = s.method;
+  info->num_columns = s.num_columns;
+  info->support_exceeds_nyquist = s.support_exceeds_nyquist ? 1 : 0;
+  info->captured_energy_fraction = s.captured_energy_fraction;
+  info->clear_field = s.clear_field;
+  info->pupil_peak = s.pupil_peak;
+  info->normalization_divisor = s.normalization_divisor();
+  info->defocus_nm = s.defocus_nm;
+  info->wavelength_nm = s.wavelength_nm;
+  return HUV_OK;
+}
+
+huv_status huv_kernels_copy(const huv_host_kernels* k, int64_t* support, int64_t* rows,
+                            double* eigenvalues, double* values) {
+  if (!k) return fail(HUV_INVALID_PARAMETER, "null argument");
+  const huv::KernelSet& s = k->ks;
+  if (support) std::copy(s.support.begin(), s.support.end(), support);
+  if (rows) std::copy(s.nonzero_rows.begin(), s.nonzero_rows.end(), rows);
+  if (eigenvalues) std::copy(s.eigenvalues.begin(), s.eigenvalues.end(), eigenvalues);
+  if (values)
+    for (size_t i = 0; i < s.values.size(); ++i) {
+      values[2 * i] = s.values[i].real();
+      values[2 * i + 1] = s.values[i].imag();
+    }
+  return HUV_OK;
+}
+
+void huv_kernels_free(huv_host_kernels* k) { delete k; }
+
+huv_status huv_rasterize_rects(int n, double pixel_nm, int nrects, const double* rects,
+                               double bg_re, double bg_im, double* out) {
+  if (!out || (nrects > 0 && !rects) || n < 1 || !(pixel_nm > 0.0))
+    return fail(HUV_INVALID_PARAMETER, "invalid rasterize arguments");
+  std::vector<huv::RectFeature> fs((size_t)nrects);
+  for (int i = 0; i < nrects; ++i) {
+    const double* r = rects + 6 * i;
+    fs[i] = {r[0], r[1], r[2], r[3], huv::cd(r[4], r[5])};
+  }
+  std::vector<huv::cd> img;
+  huv::rasterize_rects(fs, huv::cd(bg_re, bg_im), n, pixel_nm, img);
+  for (size_t i = 0; i < img.size(); ++i) {
+    out[2 * i] = img[i].real();
+    out[2 * i + 1] = img[i].imag();
+  }
+  return HUV_OK;
+}
+
+// ===========================================================================
+// Device backend
+// ===========================================================================
+huv_status huv_ctx_create(int device, huv_ctx** out) {
+  if (!out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(HUV_DEVICE_ERROR, "no HIP device available (the backend needs an MI355X GPU)");
+  if (device < 0 || device >= count) return fail(HUV_INVALID_PARAMETER, "device out of range");
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(HUV_UNSUPPORTED, std::string("device is ") + prop.gcnArchName +
+                                     "; this backend is built for gfx950 (MI355X)");
+  auto c = std::make_unique<huv_ctx>();
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  *out = c.release();
+  return HUV_OK;
+}
+
+void huv_ctx_destroy(huv_ctx* c) { delete c; }
+
+huv_status huv_ctx_set_stream(huv_ctx* c, void* s) {
+  if (!c) return fail(HUV_INVALID_PARAMETER, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->ext_stream = (hipStream_t)s;
+  return HUV_OK;
+}
+
+void* huv_ctx_stream(huv_ctx* c) { return c ? (void*)c->stream() : nullptr; }
+
+huv_status huv_ctx_synchronize(huv_ctx* c) {
+  if (!c) return fail(HUV_INVALID_PARAMETER, "null context");
+  hipSetDevice(c->device);
+  const hipError_t e = hipStreamSynchronize(c->stream());
+  return e == hipSuccess ? HUV_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+huv_status huv_kset_upload(huv_ctx* c, int n, int K, int64_t S, const int64_t* support,
+                           const double* eig, const double* values, double flare, huv_kset** out) {
+  if (!c || !support || !eig || !values || !out)
+    return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (n < 2 || (n & (n - 1)) != 0) return fail(HUV_INVALID_PARAMETER, "grid size must be a power of two");
+  if (K < 1 || S < 1) return fail(HUV_INVALID_PARAMETER, "empty kernel set");
+  if (!(flare >= 0.0 && flare < 1.0)) return fail(HUV_INVALID_PARAMETER, "flare must be in [0, 1)");
+  int B = 0;
+  for (int64_t t = 0; t < S; ++t) {
+    const int64_t idx = support[t];
+    if (idx < 0 || idx >= (int64_t)n * n) return fail(HUV_INVALID_PARAMETER, "support index out of range");
+    const int64_t i = idx / n, j = idx % n;
+    const int64_t ky = i < n / 2 ? i : i - n, kx = j < n / 2 ? j : j - n;
+    B = std::max<int>(B, (int)std::max(std::llabs(ky), std::llabs(kx)));
+  }
+  const int m = huvk::choose_coarse(n, B);
+  if (m == 0)
+    return fail(HUV_UNSUPPORTED, "kernel band " + std::to_string(B) + " on a " + std::to_string(n) +
+                                     " grid needs a coarse grid above 256 (or above n)");
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  auto ks = std::make_unique<huv_kset>();
+  ks->ctx = c;
+  ks->n = n;
+  ks->K = K;
+  ks->S = S;
+  ks->B = B;
+  ks->m = m;
+  ks->flare = flare;
+  const int W = 2 * B + 1;
+  std::vector<float2> box((size_t)K * W * W, make_float2(0.f, 0.f));
+  for (int k = 0; k < K; ++k)
+    for (int64_t t = 0; t < S; ++t) {
+      const int64_t i = support[t] / n, j = support[t] % n;
+      const int64_t ky = i < n / 2 ? i : i - n, kx = j < n / 2 ? j : j - n;
+      box[((size_t)k * W + (kx + B)) * W + (ky + B)] =
+          make_float2((float)values[2 * (k * S + t)], (float)values[2 * (k * S + t) + 1]);
+    }
+  // Device eigenvalues carry the 1/n^4 of the two unnormalised coarse
+  // transforms (|IFFT|^2 with the 1/n^2 inverse scale).
+  const double n4 = std::pow((double)n, 4.0);
+  std::vector<float> lam((size_t)K);
+  ks->lam_host.assign(eig, eig + K);
+  for (int k = 0; k < K; ++k) lam[k] = (float)(eig[k] / n4);
+  ks->support.assign(support, support + S);
+  ks->values.assign(values, values + 2 * K * S);
+  hipError_t e = hipMalloc(&ks->kern, box.size() * sizeof(float2));
+  if (e != hipSuccess) return hip_fail(e, "kernel-set allocation");
+  e = hipMalloc(&ks->lam, lam.size() * sizeof(float));
+  if (e != hipSuccess) return hip_fail(e, "kernel-set allocation");
+  e = hipMemcpy(ks->kern, box.data(), box.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "kernel-set upload");
+  e = hipMemcpy(ks->lam, lam.data(), lam.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "kernel-set upload");
+  *out = ks.release();
+  return HUV_OK;
+}
+
+void huv_kset_destroy(huv_kset* ks) { delete ks; }
+
+huv_status huv_kset_shape(const huv_kset* ks, int* band, int* coarse) {
+  if (!ks) return fail(HUV_INVALID_PARAMETER, "null kernel set");
+  if (band) *band = ks->B;
+  if (coarse) *coarse = ks->m;
+  return HUV_OK;
+}
+
+huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, int nplanes,
+                           huv_kset* const* planes, float* out, const huv_socs_opts* opts) {
+  if (!c || !masks || !planes || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (nclips < 0 || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0 and nplanes >= 1");
+  if (nclips == 0) return HUV_OK;
+  const int B = planes[0]->B, m = planes[0]->m;
+  const double flare = planes[0]->flare;
+  for (int p = 0; p < nplanes; ++p) {
+    const huv_kset* ks = planes[p];
+    if (!ks || ks->ctx != c) return fail(HUV_INVALID_PARAMETER, "kernel set from another context");
+    if (ks->n != n)
+      return fail(HUV_DIMENSION_MISMATCH, "mask grid " + std::to_string(n) +
+                                              " does not match the kernel-set grid " + std::to_string(ks->n));
+    if (ks->B != B || ks->flare != flare)
+      return fail(HUV_INVALID_PARAMETER, "all planes of one call must share band and flare");
+  }
+  const bool pt = opts && opts->pixel_transfer;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  hipError_t e;
+  const int Q = huvk::coarse_classes(m), W = 2 * B + 1;
+  const int nimg = nclips * nplanes;
+  float2* Mq = (float2*)c->ws.get(Workspace::BAND, (size_t)nclips * Q * W * W * sizeof(float2), e);
+  if (!Mq) return hip_fail(e, "workspace");
+  huv_status st = band_stage(c, nclips, n, B, m, masks, Mq, pt);
+  if (st != HUV_OK) return st;
+  // Per-plane pointer tables (K, kernel pointers, eigenvalue pointers),
+  // uploaded once per distinct plane list and kept on the device.
+  std::vector<const void*> key(planes, planes + nplanes);
+  const size_t offk = 0, offp = ((size_t)nplanes * sizeof(int) + 63) / 64 * 64;
+  const size_t offl = offp + (size_t)nplanes * sizeof(void*);
+  const size_t tbl = offl + (size_t)nplanes * sizeof(void*);
+  void* dtab = nullptr;
+  auto it = c->plane_tables.find(key);
+  if (it != c->plane_tables.end()) {
+    dtab = it->second;
+  } else {
+    std::vector<char> host(tbl, 0);
+    for (int p = 0; p < nplanes; ++p) {
+      ((int*)(host.data() + offk))[p] = planes[p]->K;
+      ((const float2**)(host.data() + offp))[p] = planes[p]->kern;
+      ((const float**)(host.data() + offl))[p] = planes[p]->lam;
+    }
+    e = hipMalloc(&dtab, tbl);
+    if (e != hipSuccess) return hip_fail(e, "plane table");
+    e = hipMemcpy(dtab, host.data(), tbl, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(dtab);
+      return hip_fail(e, "plane table upload");
+    }
+    c->plane_tables[key] = dtab;
+  }
+  const int* dK = (const int*)((char*)dtab + offk);
+  const float2* const* dkern = (const float2* const*)((char*)dtab + offp);
+  const float* const* dlam = (const float* const*)((char*)dtab + offl);
+
+  float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)nimg * m * m * sizeof(float), e);
+  if (!coarse) return hip_fail(e, "workspace");
+  e = huvk::launch_socs_fields(m, B, nclips, nplanes, dK, dkern, dlam, Mq, coarse, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
+  const float2* tw_m = c->twiddles(m, e);
+  if (!tw_m) return hip_fail(e, "twiddle table");
+  const float2* tw_n = c->twiddles(n, e);
+  if (!tw_n) return hip_fail(e, "twiddle table");
+  const int G = 2 * B;
+  float2* tmp1 = (float2*)c->ws.get(Workspace::TMP_COARSE, (size_t)nimg * (2 * G + 1) * (m / 2) * sizeof(float2), e);
+  if (!tmp1) return hip_fail(e, "workspace");
+  float2* Ib = (float2*)c->ws.get(Workspace::IBAND, (size_t)nimg * (G + 1) * (2 * G + 1) * sizeof(float2), e);
+  if (!Ib) return hip_fail(e, "workspace");
+  e = huvk::launch_coarse_rows(m, B, nimg, coarse, tw_m, tmp1, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "coarse_rows launch");
+  e = huvk::launch_coarse_cols(m, B, nimg, (float)flare, tmp1, tw_m, Ib, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "coarse_cols launch");
+  e = huvk::launch_band_image(n, B, nimg, Ib, tw_n, out, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "band_image launch");
+  return HUV_OK;
+}
+
+huv_status huv_socs_images_host(huv_ctx* c, int nclips, int n, const float* masks, int nplanes,
+                                huv_kset* const* planes, float* out, const huv_socs_opts* opts) {
+  if (!c || !masks || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  hipSetDevice(c->device);
+  const size_t in_b = (size_t)nclips * n * n * sizeof(float);
+  const size_t out_b = (size_t)nclips * nplanes * n * n * sizeof(float);
+  float *dm = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&dm, in_b);
+  if (e != hipSuccess) return hip_fail(e, "input allocation");
+  e = hipMalloc(&dout, out_b);
+  if (e != hipSuccess) {
+    hipFree(dm);
+    return hip_fail(e, "output allocation");
+  }
+  huv_status st = HUV_OK;
+  e = hipMemcpy(dm, masks, in_b, hipMemcpyHostToDevice);
+  if (e != hipSuccess) st = hip_fail(e, "input copy");
+  if (st == HUV_OK) st = huv_socs_images(c, nclips, n, dm, nplanes, planes, dout, opts);
+  if (st == HUV_OK) {
+    e = hipStreamSynchronize(c->stream());
+    if (e == hipSuccess) e = hipMemcpy(out, dout, out_b, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) st = hip_fail(e, "output copy");
+  }
+  hipFree(dm);
+  hipFree(dout);
+  return st;
+}
+
+huv_status huv_band_spectrum(huv_ctx* c, int nclips, int n, int band, const float* masks,
+                             float* out, int pixel_transfer) {
+  if (!c || !masks || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (n < 256 || n > 2048 || (n & (n - 1)) != 0)
+    return fail(HUV_UNSUPPORTED, "band spectrum supports n in {256, 512, 1024, 2048}");
+  if (band < 0 || 2 * band + 1 > n) return fail(HUV_INVALID_PARAMETER, "band out of range");
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  // m = 32 keeps a single x class (no twist): layout [kx+B][ky+B].
+  return band_stage(c, nclips, n, band, 32, masks, (float2*)out, pixel_transfer != 0);
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// Internal state behind the opaque handles of include/huv.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/huv.h"
+
+namespace huvi {
+
+huv_status fail(huv_status code, const std::string& msg);
+huv_status hip_fail(hipError_t e, const char* what);
+
+// Device scratch owned by a context, grown on demand and reused across calls.
+struct Workspace {
+  enum Slot { TMP_ROWS = 0, BAND, COARSE, TMP_COARSE, IBAND, ILT0, ILT1, ILT2, ILT3, ILT4, ILT5,
+              ILT6, ILT7, kSlots };
+  void* ptr[kSlots] = {};
+  size_t size[kSlots] = {};
+  void* get(int slot, size_t bytes, hipError_t& err);
+  void release();
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t ext_stream = nullptr;
+  std::mutex mu;
+  Workspace ws;
+  std::map<int, float2*> tw;
+  std::map<long, float*> sinc;
+  // Device copies of per-plane pointer tables, keyed by the plane list.
+  std::map<std::vector<const void*>, void*> plane_tables;
+  hipStream_t stream() const { return ext_stream ? ext_stream : own_stream; }
+  const float2* twiddles(int N, hipError_t& err);
+  const float* sinc_table(int n, int B, bool pixel_transfer, hipError_t& err);
+  ~Ctx();
+};
+
+huv_status band_stage(Ctx* c, int nclips, int n, int B, int m, const float* masks, float2* Mq,
+                      bool pixel_transfer);
+
+}  // namespace huvi
+
+struct huv_ctx : huvi::Ctx {};
+
+struct huv_kset {
+  huv_ctx* ctx = nullptr;
+  int n = 0, K = 0, B = 0, m = 0;
+  int64_t S = 0;
+  double flare = 0.0;
+  float2* kern = nullptr;  // K x W x W, [k][kx+B][ky+B]
+  float* lam = nullptr;    // K, eigenvalue / n^4
+  std::vector<double> lam_host;
+  std::vector<int64_t> support;
+  std::vector<double> values;
+  ~huv_kset() {
+    if (kern) hipFree(kern);
+    if (lam) hipFree(lam);
+  }
+};

@@ -1,0 +1,188 @@
+/*
+ * huv.h — C ABI of the MI355X backend for the Hopkins/SOCS aerial-image path
+ * and the ILT adjoint of highuvlith-core.
+ *
+ * Two halves:
+ *   - host engine (FP64, CPU): source sampling, SOCS kernel-set construction,
+ *     mask rasterisation.  Replaces the CPU-side construction the reference
+ *     performs once per optical setting.
+ *   - device backend (HIP, gfx950): batched band-limited SOCS images and the
+ *     ILT cost / adjoint gradient.  Replaces the per-image hot loops.
+ *
+ * Conventions (crates/highuvlith-core/src/math/fft2d.rs:11-13,
+ * aerial.rs:977-983): row-major (ny, nx) grids; FFT index k maps to the
+ * signed frequency k (k < n/2) or k - n; complex values are interleaved
+ * (re, im) pairs, the layout of num_complex::Complex<f64> / <f32>.
+ *
+ * Every entry point returns a huv_status; none panics or throws across the
+ * ABI.  huv_last_error() returns a thread-local message for the last failure
+ * on the calling thread.  A context serialises its own calls; distinct
+ * contexts may be used from distinct threads.  Results are deterministic: no
+ * floating-point atomics, fixed summation orders.
+ */
+#ifndef HUV_H
+#define HUV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes mirror LithographyError (crates/highuvlith-core/src/error.rs:3-61). */
+typedef int huv_status;
+#define HUV_OK 0
+#define HUV_INVALID_PARAMETER 1
+#define HUV_DIMENSION_MISMATCH 2
+#define HUV_NUMERICAL_ERROR 3
+#define HUV_INTERNAL_ERROR 4
+#define HUV_DEVICE_ERROR 5
+#define HUV_NO_DIFFRACTION_ORDERS 6
+#define HUV_PUPIL_SAMPLING_TOO_DENSE 7
+#define HUV_UNSUPPORTED 8
+
+const char* huv_last_error(void);
+/* ABI version of this header (major * 100 + minor). */
+int huv_abi_version(void);
+
+/* ======================================================================== */
+/* Host engine (FP64)                                                        */
+/* ======================================================================== */
+
+/* Optics: kind 0 = refractive / immersion projection optics
+ * (optics/mod.rs:201-360), kind 1 = EUV mirror optics (optics/euv.rs:49-165). */
+typedef struct {
+  int kind;
+  double na;
+  double immersion_index;     /* kind 0 */
+  double flare;               /* uniform flare fraction */
+  double central_obscuration; /* kind 1, fraction of NA */
+  double transmission;        /* kind 1, intensity */
+  int paraxial_defocus;
+} huv_optics;
+
+/* Pupil fills (source.rs:1178-1248):
+ *   0 conventional   p0 = sigma
+ *   1 annular        p0 = sigma_inner, p1 = sigma_outer
+ *   2 quadrupole     p0 = sigma_center, p1 = sigma_radius, p2 = opening_deg
+ *   3 dipole         p0 = sigma_center, p1 = sigma_radius, p2 = orientation_deg
+ *   4 gaussian       p0 = sigma
+ *   5 sector poles   p0 = sigma_inner, p1 = sigma_outer, p2 = opening_deg,
+ *                    p3 = first pole angle (deg), p4 = number of poles
+ *                    (x-dipole: first 0, 2 poles; quasar: first 45, 4 poles) */
+typedef struct {
+  int kind;
+  double p[6];
+} huv_illumination;
+
+/* Pupil fill intensity at (sx, sy) in sigma units. */
+double huv_illumination_at(const huv_illumination* ill, double sx, double sy);
+
+/* Adaptive (points_per_axis <= 0) or n x n midpoint source sampling
+ * (aerial.rs:372-947, replaces `sample_source` aerial.rs:941).  Writes up to
+ * `capacity` (sx, sy, weight) triples to xyw and the total count to *count
+ * (call with capacity 0 to size the buffer). */
+huv_status huv_sample_source(const huv_illumination* ill, int points_per_axis, double df_sigma,
+                             double* xyw, int64_t capacity, int64_t* count);
+
+/* Pupil function P(fx, fy; z, lambda) (optics/mod.rs:310-355, euv.rs:129-165). */
+huv_status huv_pupil(const huv_optics* optics, double fx, double fy, double defocus_nm,
+                     double wavelength_nm, double* re, double* im);
+
+/* Kernel count selection (aerial.rs:992-1029). */
+int huv_select_kernel_count(const double* values, int64_t nvals, double trace,
+                            int64_t max_kernels, double fraction);
+
+/* One SOCS kernel set (replaces KernelBuilder::build, aerial.rs:1041-1317).
+ * method: -1 auto, 0 dense TCC, 1 dense Gram, 2 randomized. */
+typedef struct huv_host_kernels huv_host_kernels;
+typedef struct {
+  int n;
+  double pixel_nm;
+  int num_kernels;
+  int64_t num_support;
+  int64_t num_rows;
+  int method;
+  int64_t num_columns;
+  int support_exceeds_nyquist;
+  double captured_energy_fraction;
+  double clear_field;
+  double pupil_peak;
+  double normalization_divisor; /* clear field, or 1 for dark-field sets */
+  double defocus_nm;
+  double wavelength_nm;
+} huv_kernels_info;
+
+huv_status huv_kernels_build(const huv_optics* optics, const double* xyw, int64_t npoints, int n,
+                             double pixel_nm, double defocus_nm, double wavelength_nm,
+                             int max_kernels, double energy_fraction, int method,
+                             huv_host_kernels** out);
+huv_status huv_kernels_info_get(const huv_host_kernels* ks, huv_kernels_info* info);
+/* Copy out (any pointer may be NULL): support (S flat indices i*n+j),
+ * nonzero rows, absolute eigenvalues (K), values (K x S interleaved complex). */
+huv_status huv_kernels_copy(const huv_host_kernels* ks, int64_t* support, int64_t* rows,
+                            double* eigenvalues, double* values);
+void huv_kernels_free(huv_host_kernels* ks);
+
+/* Rasterise painted rectangles (later ones win) over a background with exact
+ * area coverage (mask.rs:590, 1365-1571).  rects: nrects x (x, y, w, h, re, im);
+ * out: n x n interleaved complex. */
+huv_status huv_rasterize_rects(int n, double pixel_nm, int nrects, const double* rects,
+                               double bg_re, double bg_im, double* out);
+
+/* ======================================================================== */
+/* Device backend (HIP)                                                      */
+/* ======================================================================== */
+
+typedef struct huv_ctx huv_ctx;
+typedef struct huv_kset huv_kset;
+
+/* Context on HIP device `device` with its own stream. */
+huv_status huv_ctx_create(int device, huv_ctx** out);
+void huv_ctx_destroy(huv_ctx* ctx);
+/* Use an external hipStream_t (e.g. a framework's current stream); NULL
+ * restores the context's own stream. */
+huv_status huv_ctx_set_stream(huv_ctx* ctx, void* hip_stream);
+void* huv_ctx_stream(huv_ctx* ctx);
+huv_status huv_ctx_synchronize(huv_ctx* ctx);
+
+/* Upload one kernel set (the sparse SparseKernelSet of aerial.rs:955-973):
+ * support (S flat indices on the n grid), eigenvalues ALREADY divided by the
+ * normalisation divisor (the KernelSet convention, aerial.rs:1912-1942),
+ * values K x S interleaved complex, and the optics' flare fraction. */
+huv_status huv_kset_upload(huv_ctx* ctx, int n, int num_kernels, int64_t num_support,
+                           const int64_t* support, const double* eigenvalues,
+                           const double* values, double flare, huv_kset** out);
+void huv_kset_destroy(huv_kset* ks);
+/* Band half-width B and coarse grid m chosen for this set. */
+huv_status huv_kset_shape(const huv_kset* ks, int* band, int* coarse);
+
+typedef struct {
+  /* Multiply the mask spectrum by sinc(pi kx/n) sinc(pi ky/n) (mask.rs:804-815):
+   * with a pixel-aligned raster this reproduces AerialImageEngine::compute(mask). */
+  int pixel_transfer;
+} huv_socs_opts;
+
+/* Batched aerial images (replaces socs_intensity + normalize + apply_flare,
+ * aerial.rs:1682-1759, for compute_from_transmittance / compute_through_focus).
+ * masks: DEVICE pointer, nclips x n x n real transmittance (FP32).
+ * planes: nplanes kernel sets sharing n, band and flare (focus planes).
+ * out: DEVICE pointer, nclips x nplanes x n x n FP32, image (c, p) at
+ * out + (c * nplanes + p) * n * n.  Asynchronous on the context stream. */
+huv_status huv_socs_images(huv_ctx* ctx, int nclips, int n, const float* masks, int nplanes,
+                           huv_kset* const* planes, float* out, const huv_socs_opts* opts);
+/* Same with host buffers (copies in, computes, copies out, synchronises). */
+huv_status huv_socs_images_host(huv_ctx* ctx, int nclips, int n, const float* masks,
+                                int nplanes, huv_kset* const* planes, float* out,
+                                const huv_socs_opts* opts);
+/* Band spectrum stage alone (device pointers): M[ky][kx] = FFT_n(mask) for
+ * |kx|, |ky| <= band; out: nclips x (2B+1) x (2B+1) interleaved complex FP32
+ * with layout [kx+B][ky+B] (ky fastest). */
+huv_status huv_band_spectrum(huv_ctx* ctx, int nclips, int n, int band, const float* masks,
+                             float* out, int pixel_transfer);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HUV_H */

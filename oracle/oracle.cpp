@@ -1,0 +1,932 @@
+// ORACLE — test infrastructure only.  An FP64 CPU restatement of the
+// reference's Hopkins/SOCS image path and ILT adjoint, used by tests/,
+// __graft_entry__.smoke() and the CPU-baseline leg of bench.py as the
+// checker.  Nothing in the product (highuvlith_amd/, include/) may link,
+// load or call it.
+//
+// What is restated (crates/highuvlith-core/src/...):
+//   orc_fft2 / orc_ifft2_pruned  math/fft2d.rs:96-170 (unnormalised forward,
+//                                1/N inverse, row pass limited to listed rows)
+//   orc_socs_intensity           aerial.rs:1682-1721 (support gather, <=16
+//                                contiguous kernel groups, per-kernel scatter +
+//                                row-pruned inverse FFT + lambda|E|^2, groups
+//                                summed in index order)
+//   orc_normalize_flare          aerial.rs:1727-1748
+//   orc_mask_spectrum            mask.rs:616-657, 910-1072, 1365-1502 (exact
+//                                Fourier-series spectrum of rectangles, gratings
+//                                and rectangle arrays; overlaps resolved into
+//                                disjoint cells)
+//   orc_abbe_image               tests/imaging_validation.rs:167-198 (direct Abbe
+//                                sum over source points, pupil at every grid
+//                                frequency, no truncation or eigen-solve)
+//   orc_ilt_*                    ilt.rs:389-951, opc.rs:432-488 (mask
+//                                parametrisation, forward fields, resist/image
+//                                fidelity, TV + binarisation, exact adjoint,
+//                                CG PR+ with Armijo backtracking)
+// The FFT is this file's own radix-2 transform (rustfft 6.4.1 is not
+// available); it is pinned by a naive DFT and numpy in the tests.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using cd = std::complex<double>;
+static const double PI = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// FFT (radix-2, FP64), 2-D with row pass + blocked transpose + column pass
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Plan1D {
+  int n = 0, logn = 0;
+  std::vector<int> rev;
+  std::vector<cd> tw;  // exp(-2 pi i k / n), k < n/2
+};
+
+std::mutex g_plan_mu;
+std::map<int, std::shared_ptr<Plan1D>> g_plans;
+
+std::shared_ptr<Plan1D> plan_for(int n) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto it = g_plans.find(n);
+  if (it != g_plans.end()) return it->second;
+  auto p = std::make_shared<Plan1D>();
+  p->n = n;
+  while ((1 << p->logn) < n) ++p->logn;
+  p->rev.resize(n);
+  for (int i = 0; i < n; ++i) {
+    int r = 0;
+    for (int b = 0; b < p->logn; ++b)
+      if (i & (1 << b)) r |= 1 << (p->logn - 1 - b);
+    p->rev[i] = r;
+  }
+  p->tw.resize(std::max(1, n / 2));
+  for (int k = 0; k < n / 2; ++k) p->tw[k] = std::polar(1.0, -2.0 * PI * k / n);
+  g_plans[n] = p;
+  return p;
+}
+
+// In-place unnormalised 1-D DFT; inverse uses conjugate twiddles (no scale).
+void fft1d(cd* x, const Plan1D& p, bool inverse) {
+  const int n = p.n;
+  for (int i = 0; i < n; ++i)
+    if (i < p.rev[i]) std::swap(x[i], x[p.rev[i]]);
+  for (int len = 2; len <= n; len <<= 1) {
+    const int half = len >> 1, stride = n / len;
+    for (int i = 0; i < n; i += len)
+      for (int j = 0; j < half; ++j) {
+        cd w = p.tw[j * stride];
+        if (inverse) w = std::conj(w);
+        const cd u = x[i + j], v = x[i + j + half] * w;
+        x[i + j] = u + v;
+        x[i + j + half] = u - v;
+      }
+  }
+}
+
+void transpose_into(const cd* src, cd* dst, int rows, int cols) {
+  const int T = 32;
+  for (int ib = 0; ib < rows; ib += T)
+    for (int jb = 0; jb < cols; jb += T)
+      for (int i = ib; i < std::min(ib + T, rows); ++i)
+        for (int j = jb; j < std::min(jb + T, cols); ++j) dst[(size_t)j * rows + i] = src[(size_t)i * cols + j];
+}
+
+// Unnormalised 2-D transform of a row-major ny x nx array; rows (if given)
+// limits the row pass to the listed rows (all others must be zero).
+void transform2d(cd* data, int ny, int nx, bool inverse, const int64_t* rows, int64_t nrows,
+                 std::vector<cd>& tmp) {
+  auto px = plan_for(nx), py = plan_for(ny);
+  if (!rows) {
+    for (int r = 0; r < ny; ++r) fft1d(data + (size_t)r * nx, *px, inverse);
+  } else {
+    for (int64_t k = 0; k < nrows; ++k) fft1d(data + (size_t)rows[k] * nx, *px, inverse);
+  }
+  tmp.resize((size_t)nx * ny);
+  transpose_into(data, tmp.data(), ny, nx);
+  for (int c = 0; c < nx; ++c) fft1d(tmp.data() + (size_t)c * ny, *py, inverse);
+  transpose_into(tmp.data(), data, nx, ny);
+}
+
+}  // namespace
+
+extern "C" {
+
+// data: ny*nx interleaved complex; inverse carries 1/(nx*ny).
+void orc_fft2(double* data, int ny, int nx, int inverse) {
+  std::vector<cd> tmp;
+  cd* d = reinterpret_cast<cd*>(data);
+  transform2d(d, ny, nx, inverse != 0, nullptr, 0, tmp);
+  if (inverse) {
+    const double s = 1.0 / ((double)nx * ny);
+    for (size_t i = 0; i < (size_t)nx * ny; ++i) d[i] *= s;
+  }
+}
+
+void orc_ifft2_pruned(double* data, int n, const int64_t* rows, int64_t nrows) {
+  std::vector<cd> tmp;
+  cd* d = reinterpret_cast<cd*>(data);
+  transform2d(d, n, n, true, rows, nrows, tmp);
+  const double s = 1.0 / ((double)n * n);
+  for (size_t i = 0; i < (size_t)n * n; ++i) d[i] *= s;
+}
+
+// Naive O(N^4) 2-D DFT (test pin for the FFT).
+void orc_dft2_naive(const double* in, double* out, int n, int inverse) {
+  const cd* x = reinterpret_cast<const cd*>(in);
+  cd* y = reinterpret_cast<cd*>(out);
+  const double sgn = inverse ? 1.0 : -1.0;
+  for (int ky = 0; ky < n; ++ky)
+    for (int kx = 0; kx < n; ++kx) {
+      cd acc(0, 0);
+      for (int iy = 0; iy < n; ++iy)
+        for (int ix = 0; ix < n; ++ix) {
+          const long ph = ((long)ky * iy + (long)kx * ix) % n;
+          acc += x[iy * n + ix] * std::polar(1.0, sgn * 2.0 * PI * ph / n);
+        }
+      y[ky * n + kx] = inverse ? acc / ((double)n * n) : acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SOCS intensity in the reference's structure (aerial.rs:1682-1721)
+// ---------------------------------------------------------------------------
+// values: K x S interleaved complex; spectrum: n x n interleaved complex.
+// threads > 0 runs the kernel groups on that many OpenMP threads.
+int orc_socs_intensity(int n, int K, int64_t S, const int64_t* support, const int64_t* nonzero_rows,
+                       int64_t nrows, const double* eig, const double* values,
+                       const double* spectrum, double* out, int threads) {
+  const cd* spec = reinterpret_cast<const cd*>(spectrum);
+  const cd* vals = reinterpret_cast<const cd*>(values);
+  std::vector<cd> m_s((size_t)S);
+  for (int64_t t = 0; t < S; ++t) m_s[t] = spec[support[t]];
+  const int64_t nk = K;
+  const int64_t bytes_per_group = (int64_t)n * n * (8 + 16);
+  const int64_t max_groups = std::min<int64_t>(16, std::max<int64_t>(1, (256LL << 20) / std::max<int64_t>(bytes_per_group, 1)));
+  const int64_t per_group = std::max<int64_t>(1, (nk + std::min(std::max<int64_t>(nk, 1), max_groups) - 1) /
+                                                     std::min(std::max<int64_t>(nk, 1), max_groups));
+  const int64_t groups = (nk + per_group - 1) / per_group;
+  const size_t npix = (size_t)n * n;
+  std::vector<std::vector<double>> partials((size_t)groups);
+#ifdef _OPENMP
+  const int nt = threads > 0 ? threads : 1;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+#endif
+  for (int64_t g = 0; g < groups; ++g) {
+    std::vector<double> acc(npix, 0.0);
+    std::vector<cd> buf(npix), tmp;
+    for (int64_t k = g * per_group; k < std::min((g + 1) * per_group, nk); ++k) {
+      std::fill(buf.begin(), buf.end(), cd(0, 0));
+      for (int64_t t = 0; t < S; ++t) buf[support[t]] = vals[k * S + t] * m_s[t];
+      transform2d(buf.data(), n, n, true, nonzero_rows, nrows, tmp);
+      const double scale = 1.0 / (double)npix;
+      const double lambda = eig[k];
+      for (size_t i = 0; i < npix; ++i) acc[i] += lambda * std::norm(buf[i] * scale);
+    }
+    partials[g] = std::move(acc);
+  }
+  std::fill(out, out + npix, 0.0);
+  for (int64_t g = 0; g < groups; ++g)
+    for (size_t i = 0; i < npix; ++i) out[i] += partials[g][i];
+  (void)threads;
+  return 0;
+}
+
+// Divide by the normalisation divisor, then I <- (1 - phi) I + phi <I>.
+void orc_normalize_flare(double* img, int64_t npix, double divisor, double flare) {
+  if (divisor != 1.0)
+    for (int64_t i = 0; i < npix; ++i) img[i] /= divisor;
+  if (flare > 0.0) {
+    double mean = 0.0;
+    for (int64_t i = 0; i < npix; ++i) mean += img[i];
+    mean /= (double)npix;
+    for (int64_t i = 0; i < npix; ++i) img[i] = img[i] * (1.0 - flare) + flare * mean;
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Exact mask spectrum (mask.rs)
+// ---------------------------------------------------------------------------
+namespace {
+
+const double COMMENSURATE_RTOL = 1e-9;
+const double MAX_GRATING_PERIODS = (double)(1ULL << 20);
+
+enum AxisKind { AX_FINITE, AX_FULL, AX_GRATING };
+struct Axis {
+  std::vector<std::pair<double, double>> iv;
+  AxisKind kind = AX_FINITE;
+  double width = 0, pitch = 0, offset = 0;
+  int64_t periods = 0;
+};
+struct Field {
+  int n;
+  double pixel, half;
+  double len() const { return 2 * half; }
+  double eps() const { return 1e-9 * len(); }
+};
+struct Resolved {
+  cd value;
+  Axis x, y;
+  double bbox[4];
+};
+
+double sinc(double u) { return std::fabs(u) < 1e-6 ? 1.0 - u * u / 6.0 : std::sin(u) / u; }
+int64_t signed_order(int64_t k, int64_t n) { return k < n / 2 ? k : k - n; }
+
+Axis full_axis(const Field& f) {
+  Axis a;
+  a.iv = {{-f.half, f.half}};
+  a.kind = AX_FULL;
+  return a;
+}
+
+bool axis_from_interval(double a, double b, const Field& f, Axis& out) {
+  const double lo = std::max(a, -f.half), hi = std::min(b, f.half);
+  if (hi - lo <= 0.0) return false;
+  const double eps = f.eps();
+  if (lo <= -f.half + eps && hi >= f.half - eps) {
+    out = full_axis(f);
+    return true;
+  }
+  out = Axis();
+  out.iv = {{lo, hi}};
+  return true;
+}
+
+bool grating_axis(double width, double pitch, double offset, const Field& f, Axis& out,
+                  double& fraction) {
+  if (!(std::isfinite(width) && std::isfinite(pitch) && std::isfinite(offset)) || width <= 0.0)
+    return false;
+  if (pitch <= 0.0) return false;
+  if (width >= pitch) {
+    out = full_axis(f);
+    fraction = 1.0;
+    return true;
+  }
+  const double l = f.len();
+  const double periods = l / pitch;
+  if (periods > MAX_GRATING_PERIODS) {
+    out = full_axis(f);
+    fraction = width / pitch;
+    return true;
+  }
+  const double m = std::round(periods);
+  const bool comm = m >= 1.0 && std::fabs(periods - m) <= COMMENSURATE_RTOL * periods;
+  const double half = f.half;
+  const int64_t k_lo = (int64_t)std::floor((-half - offset - width / 2.0) / pitch);
+  const int64_t k_hi = (int64_t)std::ceil((half - offset + width / 2.0) / pitch);
+  Axis a;
+  for (int64_t k = k_lo; k <= k_hi; ++k) {
+    const double c = offset + (double)k * pitch;
+    const double lo = std::max(c - width / 2.0, -half), hi = std::min(c + width / 2.0, half);
+    if (hi > lo) a.iv.push_back({lo, hi});
+  }
+  if (a.iv.empty()) return false;
+  if (comm) {
+    a.kind = AX_GRATING;
+    a.width = width;
+    a.pitch = pitch;
+    a.offset = offset;
+    a.periods = (int64_t)m;
+  }
+  out = a;
+  fraction = 1.0;
+  return true;
+}
+
+Resolved separable(cd v, const Axis& x, const Axis& y) {
+  Resolved r;
+  r.value = v;
+  r.x = x;
+  r.y = y;
+  r.bbox[0] = x.iv.front().first;
+  r.bbox[1] = x.iv.back().second;
+  r.bbox[2] = y.iv.front().first;
+  r.bbox[3] = y.iv.back().second;
+  return r;
+}
+
+bool intervals_overlap(const std::vector<std::pair<double, double>>& a,
+                       const std::vector<std::pair<double, double>>& b, double eps) {
+  size_t i = 0, j = 0;
+  while (i < a.size() && j < b.size()) {
+    const double lo = std::max(a[i].first, b[j].first), hi = std::min(a[i].second, b[j].second);
+    if (hi - lo > eps) return true;
+    if (a[i].second < b[j].second) ++i; else ++j;
+  }
+  return false;
+}
+
+bool any_overlap(const std::vector<Resolved>& res, const Field& f) {
+  const double eps = f.eps();
+  for (size_t i = 0; i < res.size(); ++i)
+    for (size_t j = 0; j < i; ++j) {
+      const Resolved &a = res[i], &b = res[j];
+      const bool hit = std::min(a.bbox[1], b.bbox[1]) - std::max(a.bbox[0], b.bbox[0]) > eps &&
+                       std::min(a.bbox[3], b.bbox[3]) - std::max(a.bbox[2], b.bbox[2]) > eps;
+      if (!hit) continue;
+      if (intervals_overlap(a.x.iv, b.x.iv, eps) && intervals_overlap(a.y.iv, b.y.iv, eps)) return true;
+    }
+  return false;
+}
+
+struct Term {
+  Axis y;
+  std::vector<std::pair<cd, Axis>> xs;
+};
+
+void dedup_edges(std::vector<double>& e, double eps) {
+  std::sort(e.begin(), e.end());
+  std::vector<double> o;
+  for (double v : e)
+    if (o.empty() || std::fabs(v - o.back()) > eps) o.push_back(v);
+  e.swap(o);
+}
+size_t edge_index(const std::vector<double>& e, double v, double eps) {
+  return (size_t)(std::lower_bound(e.begin(), e.end(), v - eps) - e.begin());
+}
+
+bool compress(const std::vector<Resolved>& res, cd bg, const Field& f, std::vector<Term>& terms) {
+  const double eps = 1e-12 * f.len();
+  std::vector<double> xe{-f.half, f.half}, ye{-f.half, f.half};
+  for (auto& r : res) {
+    for (auto& p : r.x.iv) { xe.push_back(p.first); xe.push_back(p.second); }
+    for (auto& p : r.y.iv) { ye.push_back(p.first); ye.push_back(p.second); }
+  }
+  dedup_edges(xe, eps);
+  dedup_edges(ye, eps);
+  const size_t nx = xe.size() - 1, ny = ye.size() - 1;
+  if (nx == 0 || ny == 0 || nx * ny > (size_t(1) << 20)) return false;
+  std::vector<cd> cells(nx * ny, bg);
+  for (auto& r : res)
+    for (auto& py : r.y.iv) {
+      const size_t i0 = edge_index(ye, py.first, eps), i1 = edge_index(ye, py.second, eps);
+      for (auto& pxv : r.x.iv) {
+        const size_t j0 = edge_index(xe, pxv.first, eps), j1 = edge_index(xe, pxv.second, eps);
+        for (size_t i = i0; i < std::min(i1, ny); ++i)
+          for (size_t j = j0; j < std::min(j1, nx); ++j) cells[i * nx + j] = r.value;
+      }
+    }
+  terms.clear();
+  size_t i = 0;
+  while (i < ny) {
+    size_t k = i + 1;
+    while (k < ny && std::equal(cells.begin() + k * nx, cells.begin() + (k + 1) * nx, cells.begin() + i * nx)) ++k;
+    Term t;
+    size_t j = 0;
+    while (j < nx) {
+      const cd v = cells[i * nx + j];
+      size_t e = j + 1;
+      while (e < nx && cells[i * nx + e] == v) ++e;
+      if (v != bg) {
+        Axis a;
+        a.iv = {{xe[j], xe[e]}};
+        t.xs.push_back({v, a});
+      }
+      j = e;
+    }
+    if (!t.xs.empty()) {
+      t.y = Axis();
+      t.y.iv = {{ye[i], ye[k]}};
+      terms.push_back(t);
+    }
+    i = k;
+  }
+  return true;
+}
+
+std::vector<cd> axis_coefficients(const Axis& a, const Field& f) {
+  const int64_t n = f.n;
+  const double nf = (double)n, l = f.len();
+  std::vector<cd> out((size_t)n, cd(0, 0));
+  if (a.kind == AX_FULL) {
+    out[0] = cd(nf, 0.0);
+  } else if (a.kind == AX_GRATING) {
+    for (int64_t k = 0; k < n; ++k) {
+      const int64_t m = signed_order(k, n);
+      if (m % a.periods != 0) continue;
+      const double q = (double)(m / a.periods);
+      const double amp = a.width / a.pitch * sinc(PI * q * a.width / a.pitch);
+      const double phase = -2.0 * PI * q * a.offset / a.pitch - PI * (double)m * (nf - 1.0) / nf;
+      out[k] = std::polar(nf * amp, phase);
+    }
+  } else {
+    for (int64_t k = 0; k < n; ++k) {
+      const double m = (double)signed_order(k, n);
+      cd acc(0, 0);
+      for (auto& p : a.iv) {
+        const double w = p.second - p.first;
+        const double amp = w / l * sinc(PI * m * w / l);
+        acc += std::polar(amp, -PI * m * (p.first + p.second) / l);
+      }
+      out[k] = acc * std::polar(nf, -PI * m * (nf - 1.0) / nf);
+    }
+  }
+  return out;
+}
+
+void add_term_spectrum(cd* spec, const Term& t, cd bg, const Field& f) {
+  const int n = f.n;
+  std::vector<cd> xc((size_t)n, cd(0, 0));
+  for (auto& xa : t.xs) {
+    const cd w = xa.first - bg;
+    if (w == cd(0, 0)) continue;
+    auto c = axis_coefficients(xa.second, f);
+    for (int k = 0; k < n; ++k) xc[k] += w * c[k];
+  }
+  auto yc = axis_coefficients(t.y, f);
+  for (int i = 0; i < n; ++i) {
+    if (yc[i] == cd(0, 0)) continue;
+    for (int j = 0; j < n; ++j)
+      if (xc[j] != cd(0, 0)) spec[(size_t)i * n + j] += yc[i] * xc[j];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// feats: nfeat records of 8 doubles: [kind, a, b, c, d, e, f, g]
+//   kind 0 Rect      a=x b=y c=w d=h
+//   kind 1 LineSpace a=cd b=pitch c=orientation(0 vertical, 1 horizontal) d=offset
+//   kind 2 RectArray a=w b=h c=pitch_x d=pitch_y e=offset_x f=offset_y
+// absorber: complex amplitude of the absorber (binary: 0).
+// Returns 0 on success, 1 if overlaps could not be resolved (painter fallback
+// not restated).
+int orc_mask_spectrum(int n, double pixel, int nfeat, const double* feats, int dark_field,
+                      double absorber_re, double absorber_im, double* out) {
+  Field f{n, pixel, n * pixel / 2.0};
+  const cd absorber(absorber_re, absorber_im);
+  const cd bg = dark_field ? absorber : cd(1.0, 0.0);
+  const cd fg = dark_field ? cd(1.0, 0.0) : absorber;
+  std::vector<Resolved> res;
+  for (int i = 0; i < nfeat; ++i) {
+    const double* r = feats + 8 * i;
+    const int kind = (int)r[0];
+    if (kind == 0) {
+      const double x = r[1], y = r[2], w = r[3], h = r[4];
+      if (!(w > 0 && h > 0)) continue;
+      Axis ax, ay;
+      if (!axis_from_interval(x - w / 2, x + w / 2, f, ax)) continue;
+      if (!axis_from_interval(y - h / 2, y + h / 2, f, ay)) continue;
+      res.push_back(separable(fg, ax, ay));
+    } else if (kind == 1) {
+      Axis g;
+      double frac;
+      if (!grating_axis(r[1], r[2], r[4], f, g, frac)) continue;
+      const cd v = bg + (fg - bg) * frac;
+      if ((int)r[3] == 0) res.push_back(separable(v, g, full_axis(f)));
+      else res.push_back(separable(v, full_axis(f), g));
+    } else if (kind == 2) {
+      Axis ax, ay;
+      double fx, fy;
+      if (!grating_axis(r[1], r[3], r[5], f, ax, fx)) continue;
+      if (!grating_axis(r[2], r[4], r[6], f, ay, fy)) continue;
+      res.push_back(separable(bg + (fg - bg) * (fx * fy), ax, ay));
+    }
+  }
+  std::vector<Term> terms;
+  const bool overlap = res.size() > 1024 || any_overlap(res, f);
+  if (!overlap) {
+    for (auto& r : res) {
+      Term t;
+      t.y = r.y;
+      t.xs.push_back({r.value, r.x});
+      terms.push_back(t);
+    }
+  } else if (!compress(res, bg, f, terms)) {
+    return 1;
+  }
+  cd* spec = reinterpret_cast<cd*>(out);
+  std::fill(spec, spec + (size_t)n * n, cd(0, 0));
+  spec[0] = bg * ((double)n * n);
+  for (auto& t : terms) add_term_spectrum(spec, t, bg, f);
+  return 0;
+}
+
+// Multiply a spectrum in FFT layout by sinc(pi mx/N) sinc(pi my/N)
+// (mask.rs:804-815).
+void orc_pixel_transfer(double* data, int n) {
+  cd* s = reinterpret_cast<cd*>(data);
+  std::vector<double> t((size_t)n);
+  for (int k = 0; k < n; ++k) t[k] = sinc(PI * (double)signed_order(k, n) / n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s[(size_t)i * n + j] *= t[j] * t[i];
+}
+
+// ---------------------------------------------------------------------------
+// Abbe brute force (tests/imaging_validation.rs:167-198) with the oracle's own
+// pupil (optics/mod.rs:310-355, euv.rs:129-165).
+// ---------------------------------------------------------------------------
+static cd orc_pupil(int kind, double na, double nmed, double obsc, double trans, double fx,
+                    double fy, double z, double lambda) {
+  const double rho2 = fx * fx + fy * fy;
+  if (rho2 > 1.0) return cd(0, 0);
+  const double rho = std::sqrt(rho2);
+  const double medium = kind == 1 ? 1.0 : nmed;
+  if (kind == 1 && rho < obsc) return cd(0, 0);
+  double phase = 0.0;
+  if (z != 0.0) {
+    const double s = na * rho / medium, s2 = s * s;
+    const double ct = std::sqrt(std::max(1.0 - s2, 0.0));
+    phase = 2.0 * PI * medium * z * s2 / ((1.0 + ct) * lambda);
+  }
+  const double amp = kind == 1 ? std::sqrt(std::max(trans, 0.0)) : 1.0;
+  return std::polar(amp, phase);
+}
+
+void orc_abbe_image(int n, double pixel, const double* spectrum, const double* points, int64_t npts,
+                    int optics_kind, double na, double nmed, double obsc, double trans, double z,
+                    double lambda, double* out) {
+  const cd* spec = reinterpret_cast<const cd*>(spectrum);
+  const double df = 1.0 / (n * pixel), fc = na / lambda;
+  const size_t npix = (size_t)n * n;
+  std::fill(out, out + npix, 0.0);
+  std::vector<cd> field(npix), tmp;
+  for (int64_t s = 0; s < npts; ++s) {
+    const double sx = points[3 * s], sy = points[3 * s + 1], w = points[3 * s + 2];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double fx = (double)signed_order(j, n) * df / fc + sx;
+        const double fy = (double)signed_order(i, n) * df / fc + sy;
+        field[(size_t)i * n + j] = orc_pupil(optics_kind, na, nmed, obsc, trans, fx, fy, z, lambda) * spec[(size_t)i * n + j];
+      }
+    transform2d(field.data(), n, n, true, nullptr, 0, tmp);
+    const double sc = 1.0 / (double)npix;
+    for (size_t i = 0; i < npix; ++i) out[i] += w * std::norm(field[i] * sc);
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ILT (ilt.rs:389-951 with opc.rs:432-488)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct IltCond {
+  double dose, weight;
+  std::vector<double> lam;        // K eigenvalues (already / I_clear)
+  std::vector<cd> kernels;        // K x n^2 dense
+};
+
+struct IltProblem {
+  int n = 0;
+  std::vector<double> target;
+  int cost_kind = 0;  // 0 image fidelity, 1 resist contour
+  double threshold = 0.25, steepness = 50.0;
+  std::vector<IltCond> conds;
+  double flare = 0, beta = 4, tv_weight = 0, tv_eps = 0.01, bin_weight = 0;
+  cd absorber{0, 0};
+  std::vector<double> filter;  // empty = none
+  int proxy = 0;
+
+  size_t npix() const { return (size_t)n * n; }
+
+  std::vector<double> filtered(const std::vector<double>& x) const {
+    if (filter.empty()) return x;
+    std::vector<cd> s(npix()), tmp;
+    for (size_t i = 0; i < npix(); ++i) s[i] = x[i];
+    transform2d(s.data(), n, n, false, nullptr, 0, tmp);
+    for (size_t i = 0; i < npix(); ++i) s[i] *= filter[i];
+    transform2d(s.data(), n, n, true, nullptr, 0, tmp);
+    std::vector<double> r(npix());
+    const double sc = 1.0 / (double)npix();
+    for (size_t i = 0; i < npix(); ++i) r[i] = s[i].real() * sc;
+    return r;
+  }
+  std::vector<double> mask(const std::vector<double>& theta) const {
+    auto f = filtered(theta);
+    for (auto& v : f) v = 1.0 / (1.0 + std::exp(-beta * v));
+    return f;
+  }
+  std::vector<cd> spectrum(const std::vector<double>& m) const {
+    std::vector<cd> t(npix()), tmp;
+    const cd oma = cd(1, 0) - absorber;
+    for (size_t i = 0; i < npix(); ++i) t[i] = absorber + oma * m[i];
+    transform2d(t.data(), n, n, false, nullptr, 0, tmp);
+    return t;
+  }
+  std::vector<std::vector<cd>> fields(const IltCond& c, const std::vector<cd>& spec) const {
+    const size_t K = c.lam.size();
+    std::vector<std::vector<cd>> f(K);
+    for (size_t k = 0; k < K; ++k) {
+      std::vector<cd> p(npix()), tmp;
+      const cd* ker = c.kernels.data() + k * npix();
+      for (size_t i = 0; i < npix(); ++i) p[i] = ker[i] * spec[i];
+      transform2d(p.data(), n, n, true, nullptr, 0, tmp);
+      const double sc = 1.0 / (double)npix();
+      for (auto& v : p) v *= sc;
+      f[k] = std::move(p);
+    }
+    return f;
+  }
+  std::vector<double> intensity(const IltCond& c, const std::vector<std::vector<cd>>& f) const {
+    std::vector<double> img(npix(), 0.0);
+    for (size_t k = 0; k < f.size(); ++k)
+      for (size_t i = 0; i < npix(); ++i) img[i] += c.lam[k] * std::norm(f[k][i]);
+    if (flare > 0.0) {
+      double mean = 0;
+      for (double v : img) mean += v;
+      mean /= (double)npix();
+      for (auto& v : img) v = v * (1.0 - flare) + flare * mean;
+    }
+    return img;
+  }
+  double data_term(const std::vector<double>& img, const IltCond& c, std::vector<double>* g) const {
+    const double inv_n = 1.0 / (double)npix();
+    double cost = 0.0;
+    if (g) g->assign(npix(), 0.0);
+    for (size_t i = 0; i < npix(); ++i) {
+      const double z = target[i];
+      if (cost_kind == 0) {
+        const double r = c.dose * img[i] - z;
+        cost += r * r;
+        if (g) (*g)[i] = 2.0 * r * c.dose * inv_n;
+      } else {
+        const double rr = 1.0 / (1.0 + std::exp(-steepness * (c.dose * img[i] - threshold)));
+        const double r = rr - z;
+        cost += r * r;
+        if (g) (*g)[i] = 2.0 * r * steepness * rr * (1.0 - rr) * c.dose * inv_n;
+      }
+    }
+    return cost * inv_n;
+  }
+  double regularization(const std::vector<double>& m, std::vector<double>& grad) const {
+    const int ny = n, nx = n;
+    const double inv_n = 1.0 / (double)npix();
+    grad.assign(npix(), 0.0);
+    double cost = 0.0;
+    if (tv_weight > 0.0) {
+      std::vector<double> dx(npix()), dy(npix()), s(npix());
+      for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+          const double gx = m[(size_t)i * nx + (j + 1) % nx] - m[(size_t)i * nx + j];
+          const double gy = m[(size_t)((i + 1) % ny) * nx + j] - m[(size_t)i * nx + j];
+          const double si = std::sqrt(gx * gx + gy * gy + tv_eps * tv_eps);
+          dx[(size_t)i * nx + j] = gx;
+          dy[(size_t)i * nx + j] = gy;
+          s[(size_t)i * nx + j] = si;
+          cost += tv_weight * (si - tv_eps) * inv_n;
+        }
+      for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+          const size_t c0 = (size_t)i * nx + j;
+          const size_t cl = (size_t)i * nx + (j + nx - 1) % nx;
+          const size_t cu = (size_t)((i + ny - 1) % ny) * nx + j;
+          const double g = -(dx[c0] + dy[c0]) / s[c0] + dx[cl] / s[cl] + dy[cu] / s[cu];
+          grad[c0] += tv_weight * g * inv_n;
+        }
+    }
+    if (bin_weight > 0.0)
+      for (size_t i = 0; i < npix(); ++i) {
+        cost += bin_weight * 4.0 * m[i] * (1.0 - m[i]) * inv_n;
+        grad[i] += bin_weight * 4.0 * (1.0 - 2.0 * m[i]) * inv_n;
+      }
+    return cost;
+  }
+  double cost(const std::vector<double>& theta) const {
+    auto m = mask(theta);
+    auto spec = spectrum(m);
+    double total = 0.0;
+    for (auto& c : conds) {
+      auto f = fields(c, spec);
+      auto img = intensity(c, f);
+      total += c.weight * data_term(img, c, nullptr);
+    }
+    std::vector<double> rg;
+    return total + regularization(m, rg);
+  }
+  double cost_and_gradient(const std::vector<double>& theta, std::vector<double>& grad_out) const {
+    auto m = mask(theta);
+    auto spec = spectrum(m);
+    const double npx = (double)npix();
+    const cd conj_c = std::conj(cd(1, 0) - absorber);
+    double total = 0.0;
+    std::vector<double> dcdm(npix(), 0.0);
+    for (auto& c : conds) {
+      auto f = fields(c, spec);
+      auto img = intensity(c, f);
+      std::vector<double> g;
+      total += c.weight * data_term(img, c, &g);
+      double gsum = 0.0;
+      for (double v : g) gsum += v;
+      std::vector<double> g0(npix());
+      for (size_t i = 0; i < npix(); ++i) g0[i] = (1.0 - flare) * g[i] + flare * gsum / npx;
+      if (proxy) {
+        for (size_t i = 0; i < npix(); ++i) dcdm[i] += c.weight * g0[i];
+        continue;
+      }
+      std::vector<cd> v(npix()), tmp;
+      for (size_t k = 0; k < f.size(); ++k) {
+        for (size_t i = 0; i < npix(); ++i) v[i] = f[k][i] * g0[i];
+        transform2d(v.data(), n, n, false, nullptr, 0, tmp);
+        const cd* ker = c.kernels.data() + k * npix();
+        for (size_t i = 0; i < npix(); ++i) v[i] *= std::conj(ker[i]);
+        transform2d(v.data(), n, n, true, nullptr, 0, tmp);
+        const double sc = c.lam[k] / npx;
+        const double w2 = 2.0 * c.weight;
+        for (size_t i = 0; i < npix(); ++i) dcdm[i] += w2 * (conj_c * (v[i] * sc)).real();
+      }
+    }
+    std::vector<double> rg;
+    total += regularization(m, rg);
+    for (size_t i = 0; i < npix(); ++i) {
+      dcdm[i] += rg[i];
+      dcdm[i] *= beta * m[i] * (1.0 - m[i]);
+    }
+    grad_out = filtered(dcdm);
+    return total;
+  }
+};
+
+double dotv(const std::vector<double>& a, const std::vector<double>& b) {
+  double s = 0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Create an ILT problem.  conds: nconds records of (dose, weight, K); lam:
+// concatenated eigenvalues; kernels: concatenated K x n^2 complex dense kernels.
+void* orc_ilt_create(int n, const double* target, int cost_kind, double threshold,
+                     double steepness, int nconds, const double* cond_rec, const double* lam,
+                     const double* kernels, double flare, double beta, double tv_weight,
+                     double tv_eps, double bin_weight, double absorber_re, double absorber_im,
+                     double min_feature_nm, double pixel_nm, int proxy) {
+  auto* p = new IltProblem();
+  p->n = n;
+  p->target.assign(target, target + (size_t)n * n);
+  p->cost_kind = cost_kind;
+  p->threshold = threshold;
+  p->steepness = steepness;
+  p->flare = flare;
+  p->beta = beta;
+  p->tv_weight = tv_weight;
+  p->tv_eps = tv_eps;
+  p->bin_weight = bin_weight;
+  p->absorber = cd(absorber_re, absorber_im);
+  p->proxy = proxy;
+  size_t lo = 0, ko = 0;
+  for (int c = 0; c < nconds; ++c) {
+    IltCond cc;
+    cc.dose = cond_rec[3 * c];
+    cc.weight = cond_rec[3 * c + 1];
+    const int K = (int)cond_rec[3 * c + 2];
+    cc.lam.assign(lam + lo, lam + lo + K);
+    const cd* kk = reinterpret_cast<const cd*>(kernels) + ko;
+    cc.kernels.assign(kk, kk + (size_t)K * n * n);
+    lo += K;
+    ko += (size_t)K * n * n;
+    p->conds.push_back(std::move(cc));
+  }
+  if (min_feature_nm > 0.0) {
+    const double s = min_feature_nm / 1.349;
+    const double df = 1.0 / (n * pixel_nm);
+    p->filter.resize((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double fx = (double)signed_order(j, n) * df, fy = (double)signed_order(i, n) * df;
+        p->filter[(size_t)i * n + j] = std::exp(-2.0 * PI * PI * s * s * (fx * fx + fy * fy));
+      }
+  }
+  return p;
+}
+
+void orc_ilt_destroy(void* h) { delete static_cast<IltProblem*>(h); }
+
+void orc_ilt_set_bin_weight(void* h, double w) { static_cast<IltProblem*>(h)->bin_weight = w; }
+
+double orc_ilt_cost(void* h, const double* theta) {
+  auto* p = static_cast<IltProblem*>(h);
+  std::vector<double> t(theta, theta + p->npix());
+  return p->cost(t);
+}
+
+double orc_ilt_cost_grad(void* h, const double* theta, double* grad) {
+  auto* p = static_cast<IltProblem*>(h);
+  std::vector<double> t(theta, theta + p->npix()), g;
+  const double c = p->cost_and_gradient(t, g);
+  std::copy(g.begin(), g.end(), grad);
+  return c;
+}
+
+void orc_ilt_mask(void* h, const double* theta, double* m_out) {
+  auto* p = static_cast<IltProblem*>(h);
+  std::vector<double> t(theta, theta + p->npix());
+  auto m = p->mask(t);
+  std::copy(m.begin(), m.end(), m_out);
+}
+
+// Aerial image (flare included, not dose-scaled) of mask m at condition c.
+void orc_ilt_image(void* h, const double* m, int cond, double* out) {
+  auto* p = static_cast<IltProblem*>(h);
+  std::vector<double> mm(m, m + p->npix());
+  auto spec = p->spectrum(mm);
+  auto f = p->fields(p->conds[cond], spec);
+  auto img = p->intensity(p->conds[cond], f);
+  std::copy(img.begin(), img.end(), out);
+}
+
+// One CG/SD stage with Armijo backtracking (ilt.rs:755-866).  theta is
+// updated in place; cost_hist receives up to max_iter+1 costs.  Returns the
+// termination kind (0 converged, 1 max iterations, 2 line search failed) and
+// writes the number of iterations taken.
+int orc_ilt_run_stage(void* h, double* theta_io, int max_iterations, double learning_rate,
+                      double convergence_tol, int use_cg, double* cost_hist, int* iters_out) {
+  auto* p = static_cast<IltProblem*>(h);
+  const size_t N = p->npix();
+  std::vector<double> theta(theta_io, theta_io + N), grad;
+  double cost = p->cost_and_gradient(theta, grad);
+  int hist = 0;
+  cost_hist[hist++] = cost;
+  double last_gnorm = std::sqrt(dotv(grad, grad));
+  const bool cg = use_cg && !p->proxy;
+  std::vector<double> dir(N);
+  for (size_t i = 0; i < N; ++i) dir[i] = -grad[i];
+  double step = learning_rate;
+  int small = 0, it = 0, term = 1;
+  for (; it < max_iterations;) {
+    bool have = false;
+    double c_acc = 0, s_used = 0;
+    std::vector<double> theta_acc;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      double dmax = 0;
+      for (double v : dir) dmax = std::max(dmax, std::fabs(v));
+      if (dmax == 0.0) break;
+      std::vector<double> unit(N);
+      for (size_t i = 0; i < N; ++i) unit[i] = dir[i] / dmax;
+      const double slope = dotv(grad, unit);
+      if (slope >= 0.0 && !p->proxy) {
+        for (size_t i = 0; i < N; ++i) dir[i] = -grad[i];
+        continue;
+      }
+      double s = step;
+      for (int b = 0; b < 25; ++b) {
+        std::vector<double> trial(N);
+        for (size_t i = 0; i < N; ++i) trial[i] = theta[i] + s * unit[i];
+        const double ct = p->cost(trial);
+        const bool ok = p->proxy ? ct < cost : ct <= cost + 1e-4 * s * slope;
+        if (ok && std::isfinite(ct)) {
+          have = true;
+          c_acc = ct;
+          theta_acc = std::move(trial);
+          s_used = s;
+          break;
+        }
+        s *= 0.5;
+      }
+      if (have || attempt == 1 || !cg) break;
+      for (size_t i = 0; i < N; ++i) dir[i] = -grad[i];
+    }
+    (void)c_acc;
+    if (!have) {
+      term = last_gnorm > 0.0 ? 2 : 0;
+      break;
+    }
+    theta = std::move(theta_acc);
+    std::vector<double> gnew;
+    const double cnew = p->cost_and_gradient(theta, gnew);
+    const double prev = cost;
+    cost = cnew;
+    ++it;
+    cost_hist[hist++] = cost;
+    last_gnorm = std::sqrt(dotv(gnew, gnew));
+    if (cg) {
+      const double gg = dotv(grad, grad);
+      const double b = gg > 0.0 ? std::max(0.0, (dotv(gnew, gnew) - dotv(gnew, grad)) / gg) : 0.0;
+      for (size_t i = 0; i < N; ++i) dir[i] = -gnew[i] + b * dir[i];
+    } else {
+      for (size_t i = 0; i < N; ++i) dir[i] = -gnew[i];
+    }
+    grad = std::move(gnew);
+    step = std::min(s_used * 1.5, 10.0 * learning_rate);
+    const double rel = prev > 0.0 ? (prev - cost) / prev : 0.0;
+    if (rel < convergence_tol) ++small; else small = 0;
+    if (small >= 3 || cost == 0.0) {
+      term = 0;
+      break;
+    }
+  }
+  if (it >= max_iterations && term == 1) term = 1;
+  std::copy(theta.begin(), theta.end(), theta_io);
+  *iters_out = it;
+  return term;
+}
+
+}  // extern "C"
