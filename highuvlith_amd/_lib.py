@@ -111,7 +111,10 @@ SIGNATURES = {
     "huv_ctx_destroy": (None, [P]),
     "huv_ctx_set_stream": (C.c_int, [P, P]),
     "huv_ctx_stream": (P, [P]),
+    "huv_ctx_own_stream": (P, [P]),
     "huv_ctx_synchronize": (C.c_int, [P]),
+    "huv_ctx_set_profiling": (C.c_int, [P, C.c_int]),
+    "huv_ctx_stage_times": (C.c_int, [P, PD, PI64, C.c_int, C.c_int]),
     "huv_kset_upload": (C.c_int, [P, C.c_int, C.c_int, C.c_int64, PI64, PD, PD, C.c_double,
                                   C.POINTER(P)]),
     "huv_kset_destroy": (None, [P]),
@@ -135,6 +138,13 @@ def load():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m highuvlith_amd.build` "
             "(the GPU path has no fallback)")
+    # torch ships its own libamdhip64.so.7 (same SONAME as /opt/rocm's).  Load
+    # it first so the process holds exactly one HIP runtime, shared by torch's
+    # allocator/streams and this library's kernels.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         if not hasattr(lib, name):

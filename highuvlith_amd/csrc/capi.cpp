@@ -106,9 +106,40 @@ const float* Ctx::sinc_table(int n, int B, bool pixel_transfer, hipError_t& err)
   return d;
 }
 
+void Ctx::stage_begin(int stage, hipEvent_t* a) {
+  (void)stage;
+  *a = nullptr;
+  if (!profiling) return;
+  if (hipEventCreate(a) != hipSuccess) { *a = nullptr; return; }
+  hipEventRecord(*a, stream());
+}
+
+void Ctx::stage_end(int stage, hipEvent_t a) {
+  if (!profiling || !a) return;
+  hipEvent_t b;
+  if (hipEventCreate(&b) != hipSuccess) return;
+  hipEventRecord(b, stream());
+  pending.push_back({stage, a, b});
+}
+
+void Ctx::collect_stage_times() {
+  for (auto& p : pending) {
+    hipEventSynchronize(p.b);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      stage_ms[p.stage] += ms;
+      stage_calls[p.stage] += 1;
+    }
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  pending.clear();
+}
+
 Ctx::~Ctx() {
   hipSetDevice(device);
   if (own_stream) hipStreamSynchronize(own_stream);
+  collect_stage_times();
   ws.release();
   for (auto& kv : tw) hipFree(kv.second);
   for (auto& kv : sinc) hipFree(kv.second);
@@ -130,10 +161,15 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int m, const float* mask
   const size_t tmp_bytes = (size_t)nclips * (2 * B + 1) * (n / 2) * sizeof(float2);
   float2* tmp = (float2*)c->ws.get(Workspace::TMP_ROWS, tmp_bytes, e);
   if (!tmp) return hip_fail(e, "workspace");
+  hipEvent_t ev;
+  c->stage_begin(ST_ROWS_BAND, &ev);
   e = huvk::launch_rows_band(n, B, nclips, masks, tw_n, tmp, c->stream());
   if (e != hipSuccess) return hip_fail(e, "rows_band launch");
+  c->stage_end(ST_ROWS_BAND, ev);
+  c->stage_begin(ST_COLS_BAND, &ev);
   e = huvk::launch_cols_band(n, B, m, nclips, tmp, tw_n, tw_m, st, Mq, c->stream());
   if (e != hipSuccess) return hip_fail(e, "cols_band launch");
+  c->stage_end(ST_COLS_BAND, ev);
   return HUV_OK;
 }
 
@@ -325,8 +361,10 @@ huv_status huv_ctx_create(int device, huv_ctx** out) {
                                      "; this backend is built for gfx950 (MI355X)");
   auto c = std::make_unique<huv_ctx>();
   c->device = device;
-  e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  // A blocking stream: it orders with work a caller puts on the null stream.
+  e = hipStreamCreate(&c->own_stream);
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  c->active = c->own_stream;
   *out = c.release();
   return HUV_OK;
 }
@@ -336,11 +374,37 @@ void huv_ctx_destroy(huv_ctx* c) { delete c; }
 huv_status huv_ctx_set_stream(huv_ctx* c, void* s) {
   if (!c) return fail(HUV_INVALID_PARAMETER, "null context");
   std::lock_guard<std::mutex> lk(c->mu);
-  c->ext_stream = (hipStream_t)s;
+  c->active = (hipStream_t)s;
   return HUV_OK;
 }
 
 void* huv_ctx_stream(huv_ctx* c) { return c ? (void*)c->stream() : nullptr; }
+
+void* huv_ctx_own_stream(huv_ctx* c) { return c ? (void*)c->own_stream : nullptr; }
+
+huv_status huv_ctx_set_profiling(huv_ctx* c, int on) {
+  if (!c) return fail(HUV_INVALID_PARAMETER, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->profiling = on != 0;
+  return HUV_OK;
+}
+
+huv_status huv_ctx_stage_times(huv_ctx* c, double* ms, int64_t* calls, int nstages, int reset) {
+  if (!c) return fail(HUV_INVALID_PARAMETER, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  c->collect_stage_times();
+  for (int i = 0; i < nstages && i < ST_COUNT; ++i) {
+    if (ms) ms[i] = c->stage_ms[i];
+    if (calls) calls[i] = c->stage_calls[i];
+  }
+  if (reset)
+    for (int i = 0; i < ST_COUNT; ++i) {
+      c->stage_ms[i] = 0.0;
+      c->stage_calls[i] = 0;
+    }
+  return HUV_OK;
+}
 
 huv_status huv_ctx_synchronize(huv_ctx* c) {
   if (!c) return fail(HUV_INVALID_PARAMETER, "null context");
@@ -474,8 +538,11 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
 
   float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)nimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");
+  hipEvent_t ev;
+  c->stage_begin(ST_FIELDS, &ev);
   e = huvk::launch_socs_fields(m, B, nclips, nplanes, dK, dkern, dlam, Mq, coarse, c->stream());
   if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
+  c->stage_end(ST_FIELDS, ev);
   const float2* tw_m = c->twiddles(m, e);
   if (!tw_m) return hip_fail(e, "twiddle table");
   const float2* tw_n = c->twiddles(n, e);
@@ -485,12 +552,18 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
   if (!tmp1) return hip_fail(e, "workspace");
   float2* Ib = (float2*)c->ws.get(Workspace::IBAND, (size_t)nimg * (G + 1) * (2 * G + 1) * sizeof(float2), e);
   if (!Ib) return hip_fail(e, "workspace");
+  c->stage_begin(ST_COARSE_ROWS, &ev);
   e = huvk::launch_coarse_rows(m, B, nimg, coarse, tw_m, tmp1, c->stream());
   if (e != hipSuccess) return hip_fail(e, "coarse_rows launch");
+  c->stage_end(ST_COARSE_ROWS, ev);
+  c->stage_begin(ST_COARSE_COLS, &ev);
   e = huvk::launch_coarse_cols(m, B, nimg, (float)flare, tmp1, tw_m, Ib, c->stream());
   if (e != hipSuccess) return hip_fail(e, "coarse_cols launch");
+  c->stage_end(ST_COARSE_COLS, ev);
+  c->stage_begin(ST_BAND_IMAGE, &ev);
   e = huvk::launch_band_image(n, B, nimg, Ib, tw_n, out, c->stream());
   if (e != hipSuccess) return hip_fail(e, "band_image launch");
+  c->stage_end(ST_BAND_IMAGE, ev);
   return HUV_OK;
 }
 

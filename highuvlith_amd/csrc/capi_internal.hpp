@@ -29,18 +29,30 @@ struct Workspace {
 struct Ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
-  hipStream_t ext_stream = nullptr;
+  hipStream_t active = nullptr;  // stream all work is issued on (may be the null stream)
   std::mutex mu;
   Workspace ws;
   std::map<int, float2*> tw;
   std::map<long, float*> sinc;
   // Device copies of per-plane pointer tables, keyed by the plane list.
   std::map<std::vector<const void*>, void*> plane_tables;
-  hipStream_t stream() const { return ext_stream ? ext_stream : own_stream; }
+  // Optional per-stage timing with HIP events on the active stream.
+  bool profiling = false;
+  struct StageEvents { int stage; hipEvent_t a, b; };
+  std::vector<StageEvents> pending;
+  double stage_ms[16] = {};
+  int64_t stage_calls[16] = {};
+  void stage_begin(int stage, hipEvent_t* a);
+  void stage_end(int stage, hipEvent_t a);
+  void collect_stage_times();
+  hipStream_t stream() const { return active; }
   const float2* twiddles(int N, hipError_t& err);
   const float* sinc_table(int n, int B, bool pixel_transfer, hipError_t& err);
   ~Ctx();
 };
+
+enum Stage { ST_ROWS_BAND = 0, ST_COLS_BAND, ST_FIELDS, ST_COARSE_ROWS, ST_COARSE_COLS,
+             ST_BAND_IMAGE, ST_ILT_POINTWISE, ST_ILT_ADJOINT, ST_ILT_OTHER, ST_COUNT };
 
 huv_status band_stage(Ctx* c, int nclips, int n, int B, int m, const float* masks, float2* Mq,
                       bool pixel_transfer);

@@ -313,6 +313,20 @@ class DeviceContext:
     def synchronize(self):
         check(self.lib.huv_ctx_synchronize(self.handle))
 
+    STAGES = ("rows_band", "cols_band", "socs_fields", "coarse_rows", "coarse_cols",
+              "band_image", "ilt_pointwise", "ilt_adjoint", "ilt_other")
+
+    def set_profiling(self, on=True):
+        check(self.lib.huv_ctx_set_profiling(self.handle, int(bool(on))))
+
+    def stage_times(self, reset=True):
+        """{stage: (total_ms, launches)} from HIP events on the context stream."""
+        ms = np.zeros(len(self.STAGES))
+        calls = np.zeros(len(self.STAGES), dtype=np.int64)
+        check(self.lib.huv_ctx_stage_times(self.handle, dptr(ms), i64ptr(calls), len(ms),
+                                           int(bool(reset))))
+        return {s: (float(ms[i]), int(calls[i])) for i, s in enumerate(self.STAGES)}
+
     def upload(self, ks, flare):
         """Upload a SparseKernelSet (eigenvalues normalised) -> DeviceKernelSet."""
         eig = np.ascontiguousarray(ks.eigenvalues / ks.normalization_divisor)

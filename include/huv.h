@@ -140,11 +140,20 @@ typedef struct huv_kset huv_kset;
 /* Context on HIP device `device` with its own stream. */
 huv_status huv_ctx_create(int device, huv_ctx** out);
 void huv_ctx_destroy(huv_ctx* ctx);
-/* Use an external hipStream_t (e.g. a framework's current stream); NULL
- * restores the context's own stream. */
+/* Issue all further work on hip_stream (e.g. a framework's current stream;
+ * NULL is the null stream).  huv_ctx_own_stream returns the context's own
+ * (blocking) stream, the initial choice. */
 huv_status huv_ctx_set_stream(huv_ctx* ctx, void* hip_stream);
 void* huv_ctx_stream(huv_ctx* ctx);
+void* huv_ctx_own_stream(huv_ctx* ctx);
 huv_status huv_ctx_synchronize(huv_ctx* ctx);
+/* Per-stage timing with HIP events recorded on the active stream around each
+ * kernel launch (off by default).  Stages: 0 rows_band, 1 cols_band,
+ * 2 socs_fields, 3 coarse_rows, 4 coarse_cols, 5 band_image, 6 ilt_pointwise,
+ * 7 ilt_adjoint, 8 ilt_other.  huv_ctx_stage_times synchronises, returns the
+ * accumulated milliseconds and launch counts, and optionally resets them. */
+huv_status huv_ctx_set_profiling(huv_ctx* ctx, int on);
+huv_status huv_ctx_stage_times(huv_ctx* ctx, double* ms, int64_t* calls, int nstages, int reset);
 
 /* Upload one kernel set (the sparse SparseKernelSet of aerial.rs:955-973):
  * support (S flat indices on the n grid), eigenvalues ALREADY divided by the

@@ -50,6 +50,7 @@ def load():
         "orc_ilt_destroy": (None, [C.c_void_p]),
         "orc_ilt_set_bin_weight": (None, [C.c_void_p, C.c_double]),
         "orc_ilt_cost": (C.c_double, [C.c_void_p, PD]),
+        "orc_ilt_regularization": (C.c_double, [C.c_void_p, PD, PD]),
         "orc_ilt_cost_grad": (C.c_double, [C.c_void_p, PD, PD]),
         "orc_ilt_mask": (None, [C.c_void_p, PD, PD]),
         "orc_ilt_image": (None, [C.c_void_p, PD, C.c_int, PD]),
@@ -196,6 +197,12 @@ class IltProblem:
         t = np.ascontiguousarray(theta, dtype=np.float64)
         g = np.zeros_like(t)
         c = load().orc_ilt_cost_grad(self._h, _d(t), _d(g))
+        return c, g
+
+    def regularization(self, m):
+        mm = np.ascontiguousarray(m, dtype=np.float64)
+        g = np.zeros_like(mm)
+        c = load().orc_ilt_regularization(self._h, _d(mm), _d(g))
         return c, g
 
     def mask(self, theta):

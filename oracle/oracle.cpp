@@ -812,6 +812,15 @@ void orc_ilt_destroy(void* h) { delete static_cast<IltProblem*>(h); }
 
 void orc_ilt_set_bin_weight(void* h, double w) { static_cast<IltProblem*>(h)->bin_weight = w; }
 
+// Regularisation cost and its gradient with respect to m (ilt.rs:606-645).
+double orc_ilt_regularization(void* h, const double* m, double* grad) {
+  auto* p = static_cast<IltProblem*>(h);
+  std::vector<double> mm(m, m + p->npix()), g;
+  const double c = p->regularization(mm, g);
+  std::copy(g.begin(), g.end(), grad);
+  return c;
+}
+
 double orc_ilt_cost(void* h, const double* theta) {
   auto* p = static_cast<IltProblem*>(h);
   std::vector<double> t(theta, theta + p->npix());
