@@ -124,6 +124,12 @@ SIGNATURES = {
     "huv_socs_images_host": (C.c_int, [P, C.c_int, C.c_int, PF, C.c_int, C.POINTER(P), PF,
                                        C.POINTER(huv_socs_opts)]),
     "huv_band_spectrum": (C.c_int, [P, C.c_int, C.c_int, C.c_int, PF, PF, C.c_int]),
+    "huv_ilt_create": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(P), PD, PD,
+                                 C.POINTER(huv_ilt_config), C.POINTER(P)]),
+    "huv_ilt_destroy": (None, [P]),
+    "huv_ilt_cost_grad": (C.c_int, [P, C.c_int, PF, PF, PD, PF]),
+    "huv_ilt_optimize": (C.c_int, [P, C.c_int, PF, PF, C.c_int, C.c_double, C.c_double, C.c_int,
+                                   PD, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 _lib = None

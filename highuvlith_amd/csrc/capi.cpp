@@ -36,6 +36,9 @@ void* Workspace::get(int slot, size_t bytes, hipError_t& err) {
   if (bytes == 0) bytes = 256;
   if (size[slot] >= bytes) return ptr[slot];
   if (ptr[slot]) {
+    // Work already queued may still use the old buffer.
+    err = hipDeviceSynchronize();
+    if (err != hipSuccess) return nullptr;
     err = hipFree(ptr[slot]);
     ptr[slot] = nullptr;
     size[slot] = 0;
@@ -480,13 +483,14 @@ huv_status huv_kset_shape(const huv_kset* ks, int* band, int* coarse) {
   return HUV_OK;
 }
 
-huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, int nplanes,
-                           huv_kset* const* planes, float* out, const huv_socs_opts* opts) {
-  if (!c || !masks || !planes || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
-  if (nclips < 0 || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0 and nplanes >= 1");
-  if (nclips == 0) return HUV_OK;
-  const int B = planes[0]->B, m = planes[0]->m;
-  const double flare = planes[0]->flare;
+}  // extern "C"
+
+namespace huvi {
+
+huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes) {
+  if (!planes || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need at least one kernel set");
+  const int B = planes[0] ? planes[0]->B : 0;
+  const double flare = planes[0] ? planes[0]->flare : 0.0;
   for (int p = 0; p < nplanes; ++p) {
     const huv_kset* ks = planes[p];
     if (!ks || ks->ctx != c) return fail(HUV_INVALID_PARAMETER, "kernel set from another context");
@@ -496,18 +500,13 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
     if (ks->B != B || ks->flare != flare)
       return fail(HUV_INVALID_PARAMETER, "all planes of one call must share band and flare");
   }
-  const bool pt = opts && opts->pixel_transfer;
-  std::lock_guard<std::mutex> lk(c->mu);
-  hipSetDevice(c->device);
-  hipError_t e;
-  const int Q = huvk::coarse_classes(m), W = 2 * B + 1;
-  const int nimg = nclips * nplanes;
-  float2* Mq = (float2*)c->ws.get(Workspace::BAND, (size_t)nclips * Q * W * W * sizeof(float2), e);
-  if (!Mq) return hip_fail(e, "workspace");
-  huv_status st = band_stage(c, nclips, n, B, m, masks, Mq, pt);
-  if (st != HUV_OK) return st;
-  // Per-plane pointer tables (K, kernel pointers, eigenvalue pointers),
-  // uploaded once per distinct plane list and kept on the device.
+  return HUV_OK;
+}
+
+// Device copy of the per-plane (K, kernel pointer, eigenvalue pointer)
+// tables, uploaded once per distinct plane list.
+huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
+                       const float2* const** dkern, const float* const** dlam) {
   std::vector<const void*> key(planes, planes + nplanes);
   const size_t offk = 0, offp = ((size_t)nplanes * sizeof(int) + 63) / 64 * 64;
   const size_t offl = offp + (size_t)nplanes * sizeof(void*);
@@ -523,7 +522,7 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
       ((const float2**)(host.data() + offp))[p] = planes[p]->kern;
       ((const float**)(host.data() + offl))[p] = planes[p]->lam;
     }
-    e = hipMalloc(&dtab, tbl);
+    hipError_t e = hipMalloc(&dtab, tbl);
     if (e != hipSuccess) return hip_fail(e, "plane table");
     e = hipMemcpy(dtab, host.data(), tbl, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -532,10 +531,31 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
     }
     c->plane_tables[key] = dtab;
   }
-  const int* dK = (const int*)((char*)dtab + offk);
-  const float2* const* dkern = (const float2* const*)((char*)dtab + offp);
-  const float* const* dlam = (const float* const*)((char*)dtab + offl);
+  *dK = (const int*)((char*)dtab + offk);
+  *dkern = (const float2* const*)((char*)dtab + offp);
+  *dlam = (const float* const*)((char*)dtab + offl);
+  return HUV_OK;
+}
 
+// Forward images of real masks: band spectrum -> fused fields -> epilogue.
+// Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
+huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
+                        huv_kset* const* planes, float* out, bool pt, float2** Mq_out) {
+  const int B = planes[0]->B, m = planes[0]->m;
+  const double flare = planes[0]->flare;
+  hipError_t e;
+  const int Q = huvk::coarse_classes(m), W = 2 * B + 1;
+  const int nimg = nclips * nplanes;
+  float2* Mq = (float2*)c->ws.get(Workspace::BAND, (size_t)nclips * Q * W * W * sizeof(float2), e);
+  if (!Mq) return hip_fail(e, "workspace");
+  if (Mq_out) *Mq_out = Mq;
+  huv_status st = band_stage(c, nclips, n, B, m, masks, Mq, pt);
+  if (st != HUV_OK) return st;
+  const int* dK;
+  const float2* const* dkern;
+  const float* const* dlam;
+  st = plane_table(c, nplanes, planes, &dK, &dkern, &dlam);
+  if (st != HUV_OK) return st;
   float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)nimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");
   hipEvent_t ev;
@@ -561,10 +581,26 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
   if (e != hipSuccess) return hip_fail(e, "coarse_cols launch");
   c->stage_end(ST_COARSE_COLS, ev);
   c->stage_begin(ST_BAND_IMAGE, &ev);
-  e = huvk::launch_band_image(n, B, nimg, Ib, tw_n, out, c->stream());
+  e = huvk::launch_band_image(n, G, nimg, Ib, tw_n, out, c->stream());
   if (e != hipSuccess) return hip_fail(e, "band_image launch");
   c->stage_end(ST_BAND_IMAGE, ev);
   return HUV_OK;
+}
+
+}  // namespace huvi
+
+extern "C" {
+
+huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, int nplanes,
+                           huv_kset* const* planes, float* out, const huv_socs_opts* opts) {
+  if (!c || !masks || !planes || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (nclips < 0 || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0 and nplanes >= 1");
+  huv_status st = validate_planes(c, n, nplanes, planes);
+  if (st != HUV_OK) return st;
+  if (nclips == 0) return HUV_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return socs_forward(c, nclips, n, masks, nplanes, planes, out, opts && opts->pixel_transfer, nullptr);
 }
 
 huv_status huv_socs_images_host(huv_ctx* c, int nclips, int n, const float* masks, int nplanes,

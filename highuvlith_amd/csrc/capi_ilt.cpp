@@ -1,0 +1,430 @@
+// C ABI of the GPU ILT (include/huv.h): batched cost / exact adjoint gradient
+// (IltProblem::cost_and_gradient_with, ilt.rs:667-738) and the CG / Armijo
+// driver (run_stage, ilt.rs:755-866) with per-target state on the host and
+// all vector work on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/huv.h"
+#include "capi_internal.hpp"
+#include "hip/ilt_kernels.hpp"
+#include "hip/socs_kernels.hpp"
+
+using namespace huvi;
+
+struct huv_ilt {
+  huv_ctx* ctx = nullptr;
+  int n = 0, ncond = 0, B = 0, m = 0;
+  std::vector<huv_kset*> conds;
+  huv_ilt_config cfg{};
+  double flare = 0.0;
+  float* dose_d = nullptr;
+  float* weight_d = nullptr;
+  ~huv_ilt() {
+    if (dose_d) hipFree(dose_d);
+    if (weight_d) hipFree(weight_d);
+  }
+};
+
+namespace {
+
+constexpr int kBpi = 64;  // reduction blocks per image
+
+struct IltBuffers {
+  float *m, *t, *image, *G, *dcdm, *reg_grad;
+  float2 *Gb, *Rout, *H;
+  double *fid_part, *reg_part, *cost_d;
+};
+
+huv_status ilt_buffers(Ctx* c, const huv_ilt* p, int T, IltBuffers& b) {
+  hipError_t e;
+  const size_t N = (size_t)p->n * p->n;
+  const int W = 2 * p->B + 1, W2 = 4 * p->B + 1;
+  const size_t TC = (size_t)T * p->ncond;
+  char* a = (char*)c->ws.get(Workspace::ILT0, 2 * (size_t)T * N * sizeof(float), e);
+  if (!a) return hip_fail(e, "ILT workspace");
+  b.m = (float*)a;
+  b.t = b.m + (size_t)T * N;
+  b.image = (float*)c->ws.get(Workspace::ILT1, TC * N * sizeof(float), e);
+  if (!b.image) return hip_fail(e, "ILT workspace");
+  b.G = (float*)c->ws.get(Workspace::ILT2, TC * N * sizeof(float), e);
+  if (!b.G) return hip_fail(e, "ILT workspace");
+  a = (char*)c->ws.get(Workspace::ILT3, 2 * (size_t)T * N * sizeof(float), e);
+  if (!a) return hip_fail(e, "ILT workspace");
+  b.dcdm = (float*)a;
+  b.reg_grad = b.dcdm + (size_t)T * N;
+  const size_t nGb = TC * W2 * W2, nR = TC * W * W, nH = (size_t)T * (p->B + 1) * W;
+  a = (char*)c->ws.get(Workspace::ILT4, (nGb + nR + nH) * sizeof(float2), e);
+  if (!a) return hip_fail(e, "ILT workspace");
+  b.Gb = (float2*)a;
+  b.Rout = b.Gb + nGb;
+  b.H = b.Rout + nR;
+  a = (char*)c->ws.get(Workspace::ILT5, (TC * kBpi + (size_t)T * kBpi + T) * sizeof(double), e);
+  if (!a) return hip_fail(e, "ILT workspace");
+  b.fid_part = (double*)a;
+  b.reg_part = b.fid_part + TC * kBpi;
+  b.cost_d = b.reg_part + (size_t)T * kBpi;
+  return HUV_OK;
+}
+
+// Cost (into b.cost_d, device) and optionally the theta-gradient.
+huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const float* theta, float* grad,
+                    IltBuffers& b) {
+  Ctx* c = p->ctx;
+  hipStream_t s = c->stream();
+  hipError_t e;
+  const int n = p->n, C = p->ncond, B = p->B, m = p->m;
+  const int64_t N = (int64_t)n * n;
+  const float a = (float)p->cfg.absorber_re;
+  hipEvent_t ev;
+  c->stage_begin(ST_ILT_POINTWISE, &ev);
+  e = huvk::launch_ilt_mask(theta, b.m, b.t, (int64_t)T * N, (float)p->cfg.beta, a, s);
+  if (e != hipSuccess) return hip_fail(e, "ilt mask launch");
+  c->stage_end(ST_ILT_POINTWISE, ev);
+  float2* Mq = nullptr;
+  huv_status st = socs_forward(c, T, n, b.t, C, p->conds.data(), b.image, false, &Mq);
+  if (st != HUV_OK) return st;
+  c->stage_begin(ST_ILT_POINTWISE, &ev);
+  e = huvk::launch_ilt_pointwise(b.image, targets, T * C, C, N, kBpi, p->cfg.cost_kind,
+                                 (float)p->cfg.threshold, (float)p->cfg.steepness, p->dose_d,
+                                 grad ? b.G : nullptr, b.fid_part, s);
+  if (e != hipSuccess) return hip_fail(e, "ilt pointwise launch");
+  const bool reg = p->cfg.tv_weight > 0.0 || p->cfg.bin_weight > 0.0;
+  if (reg) {
+    e = huvk::launch_ilt_reg(b.m, n, T, kBpi, (float)p->cfg.tv_weight, (float)p->cfg.tv_epsilon,
+                             (float)p->cfg.bin_weight, b.reg_grad, b.reg_part, s);
+    if (e != hipSuccess) return hip_fail(e, "ilt reg launch");
+  }
+  e = huvk::launch_ilt_cost_reduce(b.fid_part, reg ? b.reg_part : nullptr, T, C, kBpi, kBpi,
+                                   p->weight_d, 1.0 / (double)N, b.cost_d, s);
+  if (e != hipSuccess) return hip_fail(e, "ilt cost reduce launch");
+  c->stage_end(ST_ILT_POINTWISE, ev);
+  if (!grad) return HUV_OK;
+  // |g| <= 2B band of F(G) for every (target, condition) image.
+  st = band_stage(c, T * C, n, 2 * B, 64, b.G, b.Gb, false);
+  if (st != HUV_OK) return st;
+  const int* dK;
+  const float2* const* dkern;
+  const float* const* dlam;
+  st = plane_table(c, C, p->conds.data(), &dK, &dkern, &dlam);
+  if (st != HUV_OK) return st;
+  c->stage_begin(ST_ILT_ADJOINT, &ev);
+  const double n2 = (double)n * n;
+  const float g0_scale = 1.0f / ((float)m * (float)m);
+  const float lam_scale = (float)n2;  // device lambda carries 1/n^4
+  e = huvk::launch_ilt_adjoint(m, T * C, Mq, b.Gb, B, C, dK, dkern, dlam, lam_scale, g0_scale,
+                               (float)p->flare, b.Rout, s);
+  if (e != hipSuccess) return hip_fail(e, "ilt adjoint launch");
+  c->stage_end(ST_ILT_ADJOINT, ev);
+  c->stage_begin(ST_ILT_OTHER, &ev);
+  e = huvk::launch_ilt_hband(b.Rout, B, T, C, p->weight_d, (float)((1.0 - p->cfg.absorber_re) / n2),
+                             b.H, s);
+  if (e != hipSuccess) return hip_fail(e, "ilt hband launch");
+  const float2* tw_n = c->twiddles(n, e);
+  if (!tw_n) return hip_fail(e, "twiddle table");
+  e = huvk::launch_band_image(n, B, T, b.H, tw_n, b.dcdm, s);
+  if (e != hipSuccess) return hip_fail(e, "ilt band image launch");
+  e = huvk::launch_ilt_chain(b.dcdm, reg ? b.reg_grad : nullptr, b.m, (float)p->cfg.beta,
+                             (int64_t)T * N, grad, s);
+  if (e != hipSuccess) return hip_fail(e, "ilt chain launch");
+  c->stage_end(ST_ILT_OTHER, ev);
+  return HUV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+huv_status huv_ilt_create(huv_ctx* c, int n, int ncond, huv_kset* const* conds, const double* dose,
+                          const double* weight, const huv_ilt_config* cfg, huv_ilt** out) {
+  if (!c || !conds || !dose || !weight || !cfg || !out)
+    return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (ncond < 1) return fail(HUV_INVALID_PARAMETER, "conditions: at least one process condition is required");
+  huv_status st = validate_planes(c, n, ncond, conds);
+  if (st != HUV_OK) return st;
+  double wsum = 0.0;
+  for (int i = 0; i < ncond; ++i) {
+    if (!(std::isfinite(dose[i]) && dose[i] > 0.0)) return fail(HUV_INVALID_PARAMETER, "dose: must be positive and finite");
+    if (!(std::isfinite(weight[i]) && weight[i] >= 0.0)) return fail(HUV_INVALID_PARAMETER, "weight: must be non-negative and finite");
+    wsum += weight[i];
+  }
+  if (wsum <= 0.0) return fail(HUV_INVALID_PARAMETER, "weight: condition weights must not all be zero");
+  if (!(cfg->beta > 0.0 && std::isfinite(cfg->beta))) return fail(HUV_INVALID_PARAMETER, "sigmoid_steepness: must be positive and finite");
+  if (!(cfg->tv_epsilon > 0.0)) return fail(HUV_INVALID_PARAMETER, "tv_epsilon: must be positive and finite");
+  if (!(cfg->tv_weight >= 0.0 && cfg->bin_weight >= 0.0)) return fail(HUV_INVALID_PARAMETER, "regularization weights must be non-negative");
+  if (cfg->cost_kind == 1 && !(cfg->threshold > 0.0 && cfg->steepness > 0.0))
+    return fail(HUV_INVALID_PARAMETER, "threshold / steepness: must be positive and finite");
+  if (cfg->absorber_im != 0.0)
+    return fail(HUV_UNSUPPORTED, "complex absorber amplitudes are not supported on the GPU path (real only)");
+  const int B = conds[0]->B, m = conds[0]->m;
+  if (m > 64 || B > 15)
+    return fail(HUV_UNSUPPORTED, "GPU ILT adjoint supports coarse grids up to 64 (band <= 15)");
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  auto p = std::make_unique<huv_ilt>();
+  p->ctx = c;
+  p->n = n;
+  p->ncond = ncond;
+  p->B = B;
+  p->m = m;
+  p->conds.assign(conds, conds + ncond);
+  p->cfg = *cfg;
+  p->flare = conds[0]->flare;
+  std::vector<float> d(dose, dose + ncond), w(weight, weight + ncond);
+  hipError_t e = hipMalloc(&p->dose_d, ncond * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&p->weight_d, ncond * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(p->dose_d, d.data(), ncond * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->weight_d, w.data(), ncond * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "ILT condition tables");
+  *out = p.release();
+  return HUV_OK;
+}
+
+void huv_ilt_destroy(huv_ilt* p) { delete p; }
+
+huv_status huv_ilt_cost_grad(huv_ilt* p, int T, const float* targets, const float* theta,
+                             double* cost, float* grad) {
+  if (!p || !targets || !theta || !cost) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (T < 1) return fail(HUV_INVALID_PARAMETER, "need at least one target");
+  Ctx* c = p->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  IltBuffers b;
+  huv_status st = ilt_buffers(c, p, T, b);
+  if (st != HUV_OK) return st;
+  st = ilt_eval(p, T, targets, theta, grad, b);
+  if (st != HUV_OK) return st;
+  hipError_t e = hipMemcpyAsync(cost, b.cost_d, T * sizeof(double), hipMemcpyDeviceToHost, c->stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream());
+  return e == hipSuccess ? HUV_OK : hip_fail(e, "cost copy");
+}
+
+huv_status huv_ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta,
+                            int max_iterations, double learning_rate, double tol, int use_cg,
+                            double* cost_hist, int* iterations, int* termination) {
+  if (!p || !targets || !theta || !cost_hist || !iterations || !termination)
+    return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (T < 1 || max_iterations < 0) return fail(HUV_INVALID_PARAMETER, "bad target count or budget");
+  if (!(learning_rate > 0.0)) return fail(HUV_INVALID_PARAMETER, "learning_rate: must be positive and finite");
+  Ctx* c = p->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream();
+  IltBuffers b;
+  huv_status st = ilt_buffers(c, p, T, b);
+  if (st != HUV_OK) return st;
+  const int64_t N = (int64_t)p->n * p->n;
+  const size_t vb = (size_t)T * N * sizeof(float);
+  hipError_t e;
+  char* vec = (char*)c->ws.get(Workspace::ILT6, 4 * vb, e);
+  if (!vec) return hip_fail(e, "ILT optimizer workspace");
+  float* grad = (float*)vec;
+  float* gnew = (float*)(vec + vb);
+  float* dir = (float*)(vec + 2 * vb);
+  float* trial = (float*)(vec + 3 * vb);
+  const int nblk = 64;
+  char* sc = (char*)c->ws.get(Workspace::ILT7, ((size_t)T * nblk + 4 * (size_t)T) * sizeof(double) + T * sizeof(int), e);
+  if (!sc) return hip_fail(e, "ILT optimizer workspace");
+  double* part = (double*)sc;
+  double* red = part + (size_t)T * nblk;
+  double* coefA = red + T;
+  double* coefB = coefA + T;
+  int* sel = (int*)(coefB + T);
+
+  auto fail_e = [&](const char* what) { return hip_fail(e, what); };
+  std::vector<double> h_red(T), h_a(T), h_b(T);
+  std::vector<int> h_sel(T);
+  auto reduce = [&](const float* x, const float* y, int mode) -> bool {
+    e = huvk::launch_vec_reduce(x, y, N, T, nblk, mode, part, red, s);
+    if (e != hipSuccess) return false;
+    e = hipMemcpyAsync(h_red.data(), red, T * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return false;
+    e = hipStreamSynchronize(s);
+    return e == hipSuccess;
+  };
+  auto upload_coefs = [&]() -> bool {
+    e = hipMemcpyAsync(coefA, h_a.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(coefB, h_b.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
+    return e == hipSuccess;
+  };
+  auto eval = [&](const float* th, float* g, std::vector<double>& out) -> huv_status {
+    huv_status r = ilt_eval(p, T, targets, th, g, b);
+    if (r != HUV_OK) return r;
+    e = hipMemcpyAsync(out.data(), b.cost_d, T * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? HUV_OK : hip_fail(e, "cost copy");
+  };
+
+  const int H = max_iterations + 1;
+  std::vector<double> cost(T), ctrial(T), cnew(T), step(T, learning_rate), slope(T), dmax(T),
+      s_try(T), s_used(T), gnorm(T);
+  std::vector<int> active(T, 1), small(T, 0), it(T, 0), term(T, 1), attempt(T, 0), searching(T, 0),
+      accepted(T, 0);
+  for (int t = 0; t < T; ++t)
+    for (int k = 0; k < H; ++k) cost_hist[(size_t)t * H + k] = NAN;
+  st = eval(theta, grad, cost);
+  if (st != HUV_OK) return st;
+  for (int t = 0; t < T; ++t) cost_hist[(size_t)t * H] = cost[t];
+  if (!reduce(grad, grad, 0)) return fail_e("dot");
+  for (int t = 0; t < T; ++t) gnorm[t] = std::sqrt(h_red[t]);
+  for (int t = 0; t < T; ++t) { h_a[t] = -1.0; h_b[t] = 0.0; }
+  if (!upload_coefs()) return fail_e("coef upload");
+  e = huvk::launch_vec_dir(grad, coefA, coefB, N, T, dir, s);
+  if (e != hipSuccess) return fail_e("dir");
+  const bool cg = use_cg != 0;
+
+  for (int iter = 0; iter < max_iterations; ++iter) {
+    bool any = false;
+    for (int t = 0; t < T; ++t) any = any || active[t];
+    if (!any) break;
+    for (int t = 0; t < T; ++t) { attempt[t] = 0; accepted[t] = 0; searching[t] = 0; }
+    // Up to two attempts (CG direction, then steepest descent).
+    for (int att = 0; att < 2; ++att) {
+      if (!reduce(dir, nullptr, 1)) return fail_e("maxabs");
+      for (int t = 0; t < T; ++t) dmax[t] = h_red[t];
+      if (!reduce(grad, dir, 0)) return fail_e("dot");
+      bool restart = false;
+      for (int t = 0; t < T; ++t) {
+        h_a[t] = 0.0;
+        h_b[t] = 1.0;
+        if (!active[t] || accepted[t] || attempt[t] != att) continue;
+        if (dmax[t] == 0.0) { searching[t] = 0; continue; }
+        slope[t] = h_red[t] / dmax[t];
+        if (slope[t] >= 0.0) {
+          // not a descent direction: restart along -grad (consumes this attempt)
+          h_a[t] = -1.0;
+          h_b[t] = 0.0;
+          attempt[t] = att + 1;
+          restart = true;
+          continue;
+        }
+        searching[t] = 1;
+        s_try[t] = step[t];
+      }
+      if (restart) {
+        if (!upload_coefs()) return fail_e("coef upload");
+        e = huvk::launch_vec_dir(grad, coefA, coefB, N, T, dir, s);
+        if (e != hipSuccess) return fail_e("dir");
+      }
+      // Batched Armijo backtracking for the searching targets.
+      for (int bt = 0; bt < 25; ++bt) {
+        bool any_s = false;
+        for (int t = 0; t < T; ++t) {
+          h_a[t] = searching[t] ? s_try[t] / dmax[t] : 0.0;
+          any_s = any_s || searching[t];
+        }
+        if (!any_s) break;
+        e = hipMemcpyAsync(coefA, h_a.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return fail_e("coef upload");
+        e = huvk::launch_vec_axpy(theta, dir, coefA, N, T, trial, s);
+        if (e != hipSuccess) return fail_e("axpy");
+        st = eval(trial, nullptr, ctrial);
+        if (st != HUV_OK) return st;
+        for (int t = 0; t < T; ++t) h_sel[t] = 0;
+        bool take = false;
+        for (int t = 0; t < T; ++t) {
+          if (!searching[t]) continue;
+          if (ctrial[t] <= cost[t] + 1e-4 * s_try[t] * slope[t] && std::isfinite(ctrial[t])) {
+            searching[t] = 0;
+            accepted[t] = 1;
+            s_used[t] = s_try[t];
+            h_sel[t] = 1;
+            take = true;
+          } else {
+            s_try[t] *= 0.5;
+          }
+        }
+        if (take) {
+          e = hipMemcpyAsync(sel, h_sel.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
+          if (e != hipSuccess) return fail_e("sel upload");
+          e = huvk::launch_vec_select(trial, sel, N, T, theta, s);
+          if (e != hipSuccess) return fail_e("select");
+        }
+      }
+      // Targets whose search failed: retry once along -grad with CG, else stop.
+      bool retry = false;
+      for (int t = 0; t < T; ++t) {
+        h_a[t] = 0.0;
+        h_b[t] = 1.0;
+        if (!active[t] || accepted[t]) continue;
+        if (searching[t]) searching[t] = 0;  // exhausted backtracks
+        if (attempt[t] == att && att == 0 && cg && dmax[t] != 0.0) {
+          attempt[t] = 1;
+          h_a[t] = -1.0;
+          h_b[t] = 0.0;
+          retry = true;
+        }
+      }
+      if (att == 0 && retry) {
+        if (!upload_coefs()) return fail_e("coef upload");
+        e = huvk::launch_vec_dir(grad, coefA, coefB, N, T, dir, s);
+        if (e != hipSuccess) return fail_e("dir");
+      }
+    }
+    bool any_acc = false;
+    for (int t = 0; t < T; ++t) {
+      if (!active[t]) continue;
+      if (!accepted[t]) {
+        active[t] = 0;
+        term[t] = gnorm[t] > 0.0 ? 2 : 0;
+      } else {
+        any_acc = true;
+      }
+    }
+    if (!any_acc) break;
+    st = eval(theta, gnew, cnew);
+    if (st != HUV_OK) return st;
+    // CG update: beta = max(0, (g'.g' - g'.g) / g.g)
+    std::vector<double> gg(T), gng(T), gngn(T);
+    if (!reduce(grad, grad, 0)) return fail_e("dot");
+    gg = h_red;
+    if (!reduce(gnew, grad, 0)) return fail_e("dot");
+    gng = h_red;
+    if (!reduce(gnew, gnew, 0)) return fail_e("dot");
+    gngn = h_red;
+    for (int t = 0; t < T; ++t) {
+      h_sel[t] = 0;
+      h_a[t] = 0.0;
+      h_b[t] = 1.0;
+      if (!active[t] || !accepted[t]) continue;
+      const double prev = cost[t];
+      cost[t] = cnew[t];
+      it[t] += 1;
+      cost_hist[(size_t)t * H + it[t]] = cost[t];
+      gnorm[t] = std::sqrt(gngn[t]);
+      const double beta = cg ? (gg[t] > 0.0 ? std::max(0.0, (gngn[t] - gng[t]) / gg[t]) : 0.0) : 0.0;
+      h_a[t] = -1.0;
+      h_b[t] = beta;
+      h_sel[t] = 1;
+      step[t] = std::min(s_used[t] * 1.5, 10.0 * learning_rate);
+      const double rel = prev > 0.0 ? (prev - cost[t]) / prev : 0.0;
+      small[t] = rel < tol ? small[t] + 1 : 0;
+      if (small[t] >= 3 || cost[t] == 0.0) {
+        active[t] = 0;
+        term[t] = 0;
+      }
+    }
+    if (!upload_coefs()) return fail_e("coef upload");
+    e = huvk::launch_vec_dir(gnew, coefA, coefB, N, T, dir, s);
+    if (e != hipSuccess) return fail_e("dir");
+    e = hipMemcpyAsync(sel, h_sel.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return fail_e("sel upload");
+    e = huvk::launch_vec_select(gnew, sel, N, T, grad, s);
+    if (e != hipSuccess) return fail_e("select");
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail_e("synchronize");
+  for (int t = 0; t < T; ++t) {
+    iterations[t] = it[t];
+    termination[t] = term[t];
+  }
+  return HUV_OK;
+}
+
+}  // extern "C"

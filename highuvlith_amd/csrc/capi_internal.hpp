@@ -57,6 +57,12 @@ enum Stage { ST_ROWS_BAND = 0, ST_COLS_BAND, ST_FIELDS, ST_COARSE_ROWS, ST_COARS
 huv_status band_stage(Ctx* c, int nclips, int n, int B, int m, const float* masks, float2* Mq,
                       bool pixel_transfer);
 
+huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes);
+huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
+                       const float2* const** dkern, const float* const** dlam);
+huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
+                        huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out);
+
 }  // namespace huvi
 
 struct huv_ctx : huvi::Ctx {};

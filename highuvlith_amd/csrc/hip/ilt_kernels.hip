@@ -1,0 +1,510 @@
+// HIP kernels of the ILT cost and exact adjoint gradient (gfx950).
+//
+// Reference semantics (crates/highuvlith-core/src/ilt.rs:28-45, 538-738):
+//   m = sigma(beta theta), t = a + (1 - a) m           (a real)
+//   I = flare(sum_k lambda_k |F^-1(K_k . F t)|^2), R = sigma(alpha (d I - t_r)) or d I
+//   C = sum_c w_c (1/N) sum (R - Z)^2 + TV + binarisation
+//   G = dC/dI, G0 = (1 - phi) G + (phi/N) sum G
+//   dC/dm = 2 Re[(1 - a) sum_k lambda_k F^-1(K_k^* . F(G0 . A_k))] + reg;  dC/dtheta = dC/dm beta m (1 - m)
+//
+// Band-limited adjoint: F(G0 . A_k) is only needed on the kernel support
+// |f| <= B, which only involves the |g| <= 2B band of G0; with G0 replaced by
+// that band-limited G0_b the product G0_b A_k has per-axis degree 3B, so on
+// a coarse grid m >= 4B + 1 its DFT is alias-free on |f| <= B.  The fused
+// adjoint kernel therefore recomputes each A_k on the coarse grid, multiplies
+// by G0_b sampled there, transforms back to the |f| <= B band and
+// accumulates lambda_k conj(K_k) times it; only the band leaves the chip.
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+#include "ilt_kernels.hpp"
+
+namespace huvk {
+
+// m = sigma(beta theta), t = a + (1 - a) m.
+__global__ void k_ilt_mask(const float* __restrict__ theta, float* __restrict__ m,
+                           float* __restrict__ t, int64_t total, float beta, float a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float mm = 1.0f / (1.0f + expf(-beta * theta[i]));
+    m[i] = mm;
+    t[i] = a + (1.0f - a) * mm;
+  }
+}
+
+// Block-wise partial sums in FP64, fixed order within the block.
+template <int THREADS>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Fidelity term of image img = target * ncond + cond: cost partials per block
+// and G = dC/dI (unweighted by the condition weight).
+constexpr int PW_THREADS = 256;
+__global__ __launch_bounds__(PW_THREADS) void k_ilt_pointwise(
+    const float* __restrict__ image, const float* __restrict__ target, int ncond, int64_t npix,
+    int blocks_per_img, int cost_kind, float threshold, float steepness,
+    const float* __restrict__ dose, float* __restrict__ G, double* __restrict__ partial) {
+  __shared__ double sh[PW_THREADS];
+  const int img = blockIdx.y;
+  const int tgt = img / ncond, c = img % ncond;
+  const float d = dose[c];
+  const float inv_n = 1.0f / (float)npix;
+  const float* I = image + (size_t)img * npix;
+  const float* Z = target + (size_t)tgt * npix;
+  float* g = G ? G + (size_t)img * npix : nullptr;
+  double acc = 0.0;
+  const int64_t chunk = (npix + blocks_per_img - 1) / blocks_per_img;
+  const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += PW_THREADS) {
+    const float di = d * I[i];
+    float r, gi;
+    if (cost_kind == 0) {
+      r = di - Z[i];
+      gi = 2.0f * r * d * inv_n;
+    } else {
+      const float rr = 1.0f / (1.0f + expf(-steepness * (di - threshold)));
+      r = rr - Z[i];
+      gi = 2.0f * r * steepness * rr * (1.0f - rr) * d * inv_n;
+    }
+    acc += (double)r * (double)r;
+    if (g) g[i] = gi;
+  }
+  const double s = block_sum<PW_THREADS>(acc, sh);
+  if (threadIdx.x == 0) partial[(size_t)img * blocks_per_img + blockIdx.x] = s;
+}
+
+// Regularisers on m (ilt.rs:606-645): cost partials per block and gradient.
+__global__ __launch_bounds__(PW_THREADS) void k_ilt_reg(
+    const float* __restrict__ m, int n, int blocks_per_img, float tv_w, float tv_eps, float bin_w,
+    float* __restrict__ reg_grad, double* __restrict__ partial) {
+  __shared__ double sh[PW_THREADS];
+  const int tgt = blockIdx.y;
+  const int64_t npix = (int64_t)n * n;
+  const float* mm = m + (size_t)tgt * npix;
+  float* rg = reg_grad + (size_t)tgt * npix;
+  const float inv_n = 1.0f / (float)npix;
+  double acc = 0.0;
+  const int64_t chunk = (npix + blocks_per_img - 1) / blocks_per_img;
+  const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
+  auto sval = [&](int i, int j, float& gx, float& gy) {
+    const float v = mm[(size_t)i * n + j];
+    gx = mm[(size_t)i * n + (j + 1) % n] - v;
+    gy = mm[(size_t)((i + 1) % n) * n + j] - v;
+    return sqrtf(gx * gx + gy * gy + tv_eps * tv_eps);
+  };
+  for (int64_t p = lo + threadIdx.x; p < hi; p += PW_THREADS) {
+    const int i = (int)(p / n), j = (int)(p % n);
+    float g = 0.0f;
+    if (tv_w > 0.0f) {
+      float gx, gy, lx, ly, ux, uy;
+      const float s0 = sval(i, j, gx, gy);
+      const float sl = sval(i, (j + n - 1) % n, lx, ly);
+      const float su = sval((i + n - 1) % n, j, ux, uy);
+      acc += (double)(tv_w * (s0 - tv_eps) * inv_n);
+      g += tv_w * (-(gx + gy) / s0 + lx / sl + uy / su) * inv_n;
+    }
+    if (bin_w > 0.0f) {
+      const float v = mm[p];
+      acc += (double)(bin_w * 4.0f * v * (1.0f - v) * inv_n);
+      g += bin_w * 4.0f * (1.0f - 2.0f * v) * inv_n;
+    }
+    rg[p] = g;
+  }
+  const double s = block_sum<PW_THREADS>(acc, sh);
+  if (threadIdx.x == 0) partial[(size_t)tgt * blocks_per_img + blockIdx.x] = s;
+}
+
+// cost[t] = sum_c w_c (1/N) sum_blocks fid[t, c] + sum_blocks reg[t]  (fixed order).
+__global__ void k_ilt_cost_reduce(const double* __restrict__ fid, const double* __restrict__ reg,
+                                  int ntargets, int ncond, int bpi, int reg_bpi,
+                                  const float* __restrict__ weight, double inv_n,
+                                  double* __restrict__ cost) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntargets) return;
+  double total = 0.0;
+  for (int c = 0; c < ncond; ++c) {
+    double s = 0.0;
+    const double* p = fid + ((size_t)t * ncond + c) * bpi;
+    for (int b = 0; b < bpi; ++b) s += p[b];
+    total += (double)weight[c] * s * inv_n;
+  }
+  if (reg) {
+    const double* p = reg + (size_t)t * reg_bpi;
+    for (int b = 0; b < reg_bpi; ++b) total += p[b];
+  }
+  cost[t] = total;
+}
+
+// ---------------------------------------------------------------------------
+// Fused adjoint, one work-group per (target, condition), coarse grid M <= 64
+// (a single x class).  Gb: |g| <= 2B band of F(G) on the n grid, layout
+// [gx + 2B][gy + 2B].  Rout: accumulated band [fx + B][fy + B] of
+// sum_k lambda_k conj(K_k) F_n(G0_b . A_k) / n^2 (complex).
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(M) void k_ilt_adjoint(
+    const float2* __restrict__ Mq, const float2* __restrict__ Gb, int B, int ncond,
+    const int* __restrict__ cond_K, const float2* const* __restrict__ cond_kern,
+    const float* const* __restrict__ cond_lam, float lam_scale, float g0_scale, float flare,
+    float2* __restrict__ Rout) {
+  constexpr int TP = M + 1;
+  extern __shared__ unsigned char smem_raw[];
+  c32* T = reinterpret_cast<c32*>(smem_raw);             // (4B+1) x TP
+  const int W = 2 * B + 1, G = 2 * B, W2 = 4 * B + 1;
+  c32* P1 = T + (size_t)W2 * TP;                         // W x TP
+  float* G0 = reinterpret_cast<float*>(P1 + (size_t)W * TP);  // M x TP
+  const int img = blockIdx.x;
+  const int tgt = img / ncond, c = img % ncond;
+  const int tid = threadIdx.x;
+  const float2* mq = Mq + (size_t)tgt * W * W;
+  const float2* gb = Gb + (size_t)img * W2 * W2;
+
+  // ---- G0_b on the coarse grid: x pass over the 4B+1 band rows gy ----
+  for (int iy = tid; iy < W2; iy += M) {
+    c32 v[M];
+#pragma unroll
+    for (int h = 0; h < M; ++h) {
+      v[h] = c32{0.f, 0.f};
+      // gx = h (h <= 2B) or h - M (>= -2B)
+      if (h <= G) {
+        const float2 a = gb[(size_t)(h + G) * W2 + iy];
+        v[h] = c32{a.x, a.y};
+      }
+      if (h - M >= -G) {
+        const float2 a = gb[(size_t)(h - M + G) * W2 + iy];
+        v[h] = cadd(v[h], c32{a.x, a.y});
+      }
+    }
+    // flare adjoint: non-DC terms scale by (1 - phi)
+    const int gy = iy - G;
+#pragma unroll
+    for (int h = 0; h < M; ++h)
+      if (!(h == 0 && gy == 0)) v[h] = cscale(v[h], 1.0f - flare);
+    fft_reg<+1, M>(v);
+#pragma unroll
+    for (int j = 0; j < M; ++j) T[iy * TP + j] = v[j];
+  }
+  __syncthreads();
+  {
+    const int j = tid;
+    c32 d[M];
+#pragma unroll
+    for (int h = 0; h < M; ++h) {
+      d[h] = c32{0.f, 0.f};
+      if (h <= G) d[h] = T[(h + G) * TP + j];
+      if (h - M >= -G) d[h] = cadd(d[h], T[(h - M + G) * TP + j]);
+    }
+    fft_reg<+1, M>(d);
+#pragma unroll
+    for (int y = 0; y < M; ++y) G0[y * TP + j] = d[y].re * g0_scale;
+  }
+  __syncthreads();
+
+  // ---- per kernel: field, product, back to the |f| <= B band ----
+  const int K = cond_K[c];
+  const float2* kern = cond_kern[c];
+  const float* lam = cond_lam[c];
+  c32 R[2 * 15 + 1];  // accumulator row fy = tid - B (B <= 15)
+#pragma unroll
+  for (int i = 0; i < 31; ++i) R[i] = c32{0.f, 0.f};
+  for (int k = 0; k < K; ++k) {
+    // x pass (rows iy in [0, 2B]): E row = IFFT_M(K . M)
+    for (int iy = tid; iy < W; iy += M) {
+      const float2* kk = kern + (size_t)k * W * W + iy;
+      const float2* mm = mq + iy;
+      c32 v[M];
+#pragma unroll
+      for (int h = 0; h < M; ++h) {
+        v[h] = c32{0.f, 0.f};
+        if (h <= B) {
+          const float2 a = kk[(size_t)(h + B) * W], b = mm[(size_t)(h + B) * W];
+          v[h] = cmul(c32{a.x, a.y}, c32{b.x, b.y});
+        }
+        if (h - M >= -B) {
+          const float2 a = kk[(size_t)(h - M + B) * W], b = mm[(size_t)(h - M + B) * W];
+          v[h] = cadd(v[h], cmul(c32{a.x, a.y}, c32{b.x, b.y}));
+        }
+      }
+      fft_reg<+1, M>(v);
+#pragma unroll
+      for (int j = 0; j < M; ++j) T[iy * TP + j] = v[j];
+    }
+    __syncthreads();
+    // y pass (column j): E column, times G0_b, forward FFT_M, keep |fy| <= B
+    {
+      const int j = tid;
+      c32 d[M];
+#pragma unroll
+      for (int h = 0; h < M; ++h) {
+        d[h] = c32{0.f, 0.f};
+        if (h <= B) d[h] = T[(h + B) * TP + j];
+        if (h - M >= -B) d[h] = cadd(d[h], T[(h - M + B) * TP + j]);
+      }
+      fft_reg<+1, M>(d);
+#pragma unroll
+      for (int y = 0; y < M; ++y) d[y] = cscale(d[y], G0[y * TP + j]);
+      fft_reg<-1, M>(d);
+#pragma unroll
+      for (int h = 0; h < M; ++h) {
+        if (h <= B) P1[(h + B) * TP + j] = d[h];
+        if (h - M >= -B) P1[(h - M + B) * TP + j] = d[h];
+      }
+    }
+    __syncthreads();
+    // x stage (row fy): forward FFT_M over x, keep |fx| <= B, accumulate
+    if (tid < W) {
+      c32 v[M];
+#pragma unroll
+      for (int j = 0; j < M; ++j) v[j] = P1[tid * TP + j];
+      fft_reg<-1, M>(v);
+      const float l = lam[k] * lam_scale;
+      const float2* kk = kern + (size_t)k * W * W + tid;  // K[k][fx + B][fy + B], fy = tid - B
+#pragma unroll
+      for (int i = 0; i < 31; ++i) {
+        const int fx = i - 15;
+        if (fx >= -B && fx <= B) {
+          const c32 val = v[(fx + M) % M];
+          const float2 kv = kk[(size_t)(fx + B) * W];
+          R[i] = cadd(R[i], cscale(cconjmul(c32{kv.x, kv.y}, val), l));
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < W) {
+    float2* ro = Rout + (size_t)img * W * W + tid;
+#pragma unroll
+    for (int i = 0; i < 31; ++i) {
+      const int fx = i - 15;
+      if (fx >= -B && fx <= B) ro[(size_t)(fx + B) * W] = make_float2(R[i].re, R[i].im);
+    }
+  }
+}
+
+// H[fx][fy] (fx in [0, B], fy in [-B, B]) = sum_c w_c (1-a) (R_c[f] + conj R_c[-f]) / n^2,
+// layout [fx][fy + B] for band_image with band half-width B.
+__global__ void k_ilt_hband(const float2* __restrict__ Rin, int B, int ncond,
+                            const float* __restrict__ weight, float scale,
+                            float2* __restrict__ H) {
+  const int tgt = blockIdx.y;
+  const int W = 2 * B + 1;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (B + 1) * W) return;
+  const int fx = idx / W, fy = idx % W - B;
+  c32 acc{0.f, 0.f};
+  for (int c = 0; c < ncond; ++c) {
+    const float2* r = Rin + ((size_t)tgt * ncond + c) * W * W;
+    const float2 p = r[(size_t)(fx + B) * W + (fy + B)];
+    const float2 q = r[(size_t)(B - fx) * W + (B - fy)];
+    acc = cadd(acc, cscale(c32{p.x + q.x, p.y - q.y}, weight[c]));
+  }
+  H[(size_t)tgt * (B + 1) * W + idx] = make_float2(acc.re * scale, acc.im * scale);
+}
+
+// grad = (dcdm + reg) * beta m (1 - m).
+__global__ void k_ilt_chain(const float* __restrict__ dcdm, const float* __restrict__ reg,
+                            const float* __restrict__ m, float beta, int64_t total,
+                            float* __restrict__ grad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float mm = m[i];
+    const float r = reg ? reg[i] : 0.0f;
+    grad[i] = (dcdm[i] + r) * beta * mm * (1.0f - mm);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched vector operations of the CG / Armijo loop (ilt.rs:755-866).
+// Per-target FP64 partial sums, reduced in a fixed order.
+// ---------------------------------------------------------------------------
+// out[t * nblk + b] = partial of sum_i x_i y_i  (mode 0) or max |x_i| (mode 1)
+__global__ __launch_bounds__(PW_THREADS) void k_vec_reduce(const float* __restrict__ x,
+                                                           const float* __restrict__ y,
+                                                           int64_t npix, int nblk, int mode,
+                                                           double* __restrict__ out) {
+  __shared__ double sh[PW_THREADS];
+  const int t = blockIdx.y;
+  const float* xx = x + (size_t)t * npix;
+  const float* yy = y ? y + (size_t)t * npix : nullptr;
+  const int64_t chunk = (npix + nblk - 1) / nblk;
+  const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
+  double acc = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += PW_THREADS) {
+    if (mode == 0) acc += (double)xx[i] * (double)yy[i];
+    else acc = fmax(acc, (double)fabsf(xx[i]));
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = PW_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      sh[threadIdx.x] = mode == 0 ? sh[threadIdx.x] + sh[threadIdx.x + s]
+                                  : fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(size_t)t * nblk + blockIdx.x] = sh[0];
+}
+
+__global__ void k_vec_finish(const double* __restrict__ part, int ntargets, int nblk, int mode,
+                             double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntargets) return;
+  double a = 0.0;
+  for (int b = 0; b < nblk; ++b) a = mode == 0 ? a + part[(size_t)t * nblk + b] : fmax(a, part[(size_t)t * nblk + b]);
+  out[t] = a;
+}
+
+// y = x + coef[t] * d   (per-target coefficient; coef 0 copies x)
+__global__ void k_vec_axpy(const float* __restrict__ x, const float* __restrict__ d,
+                           const double* __restrict__ coef, int64_t npix, int ntargets,
+                           float* __restrict__ y) {
+  const int64_t total = npix * ntargets;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(i / npix);
+    y[i] = (float)((double)x[i] + coef[t] * (double)d[i]);
+  }
+}
+
+// d = a[t] * g + b[t] * d
+__global__ void k_vec_dir(const float* __restrict__ g, const double* __restrict__ a,
+                          const double* __restrict__ b, int64_t npix, int ntargets,
+                          float* __restrict__ d) {
+  const int64_t total = npix * ntargets;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(i / npix);
+    d[i] = (float)(a[t] * (double)g[i] + b[t] * (double)d[i]);
+  }
+}
+
+// dst[t] = src[t] where sel[t] != 0
+__global__ void k_vec_select(const float* __restrict__ src, const int* __restrict__ sel,
+                             int64_t npix, int ntargets, float* __restrict__ dst) {
+  const int64_t total = npix * ntargets;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (sel[i / npix]) dst[i] = src[i];
+  }
+}
+
+// ===========================================================================
+// Launchers
+// ===========================================================================
+static int grid_for(int64_t total, int threads) {
+  const int64_t b = (total + threads - 1) / threads;
+  return (int)std::min<int64_t>(b, 1 << 16);
+}
+
+hipError_t launch_ilt_mask(const float* theta, float* m, float* t, int64_t total, float beta,
+                           float a, hipStream_t s) {
+  hipLaunchKernelGGL(k_ilt_mask, dim3(grid_for(total, 256)), dim3(256), 0, s, theta, m, t, total,
+                     beta, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_pointwise(const float* image, const float* target, int nimg, int ncond,
+                                int64_t npix, int bpi, int cost_kind, float threshold,
+                                float steepness, const float* dose, float* G, double* partial,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_ilt_pointwise, dim3(bpi, nimg), dim3(PW_THREADS), 0, s, image, target,
+                     ncond, npix, bpi, cost_kind, threshold, steepness, dose, G, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_reg(const float* m, int n, int ntargets, int bpi, float tv_w, float tv_eps,
+                          float bin_w, float* reg_grad, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_ilt_reg, dim3(bpi, ntargets), dim3(PW_THREADS), 0, s, m, n, bpi, tv_w,
+                     tv_eps, bin_w, reg_grad, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_cost_reduce(const double* fid, const double* reg, int ntargets, int ncond,
+                                  int bpi, int reg_bpi, const float* weight, double inv_n,
+                                  double* cost, hipStream_t s) {
+  hipLaunchKernelGGL(k_ilt_cost_reduce, dim3((ntargets + 63) / 64), dim3(64), 0, s, fid, reg,
+                     ntargets, ncond, bpi, reg_bpi, weight, inv_n, cost);
+  return hipGetLastError();
+}
+
+template <int M>
+static hipError_t adjoint_m(int nimg, const float2* Mq, const float2* Gb, int B, int ncond,
+                            const int* cond_K, const float2* const* cond_kern,
+                            const float* const* cond_lam, float lam_scale, float g0_scale,
+                            float flare, float2* Rout, hipStream_t s) {
+  const int W = 2 * B + 1, W2 = 4 * B + 1;
+  const size_t shmem = (size_t)(W2 + W) * (M + 1) * sizeof(c32) + (size_t)M * (M + 1) * sizeof(float);
+  auto kern = k_ilt_adjoint<M>;
+  hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  hipLaunchKernelGGL(kern, dim3(nimg), dim3(M), shmem, s, Mq, Gb, B, ncond, cond_K, cond_kern,
+                     cond_lam, lam_scale, g0_scale, flare, Rout);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_adjoint(int m, int nimg, const float2* Mq, const float2* Gb, int B,
+                              int ncond, const int* cond_K, const float2* const* cond_kern,
+                              const float* const* cond_lam, float lam_scale, float g0_scale,
+                              float flare, float2* Rout, hipStream_t s) {
+  if (B > 15) return hipErrorInvalidValue;
+  switch (m) {
+    case 32: return adjoint_m<32>(nimg, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, Rout, s);
+    case 64: return adjoint_m<64>(nimg, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, Rout, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, const float* weight,
+                            float scale, float2* H, hipStream_t s) {
+  const int cnt = (B + 1) * (2 * B + 1);
+  hipLaunchKernelGGL(k_ilt_hband, dim3((cnt + 127) / 128, ntargets), dim3(128), 0, s, Rin, B,
+                     ncond, weight, scale, H);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m, float beta,
+                            int64_t total, float* grad, hipStream_t s) {
+  hipLaunchKernelGGL(k_ilt_chain, dim3(grid_for(total, 256)), dim3(256), 0, s, dcdm, reg, m, beta,
+                     total, grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int ntargets, int nblk,
+                             int mode, double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_vec_reduce, dim3(nblk, ntargets), dim3(PW_THREADS), 0, s, x, y, npix, nblk,
+                     mode, part);
+  hipLaunchKernelGGL(k_vec_finish, dim3((ntargets + 63) / 64), dim3(64), 0, s, part, ntargets,
+                     nblk, mode, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_axpy(const float* x, const float* d, const double* coef, int64_t npix,
+                           int ntargets, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_vec_axpy, dim3(grid_for(npix * ntargets, 256)), dim3(256), 0, s, x, d, coef,
+                     npix, ntargets, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int64_t npix,
+                          int ntargets, float* d, hipStream_t s) {
+  hipLaunchKernelGGL(k_vec_dir, dim3(grid_for(npix * ntargets, 256)), dim3(256), 0, s, g, a, b,
+                     npix, ntargets, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_select(const float* src, const int* sel, int64_t npix, int ntargets,
+                             float* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_vec_select, dim3(grid_for(npix * ntargets, 256)), dim3(256), 0, s, src, sel,
+                     npix, ntargets, dst);
+  return hipGetLastError();
+}
+
+}  // namespace huvk

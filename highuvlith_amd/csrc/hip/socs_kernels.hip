@@ -303,17 +303,17 @@ __global__ __launch_bounds__((TEAMS * team_size<N1, N2>())) void k_coarse_cols(
 
 // ---------------------------------------------------------------------------
 // band_image: rows y = P j + p (P = n / 64) of the n x n image.
-// Phase 1 (threads gx in [0, 2B]):  U[j][gx] = IFFT_64( fold_64( Ib[gx][gy] e^{2 pi i gy p / n} ) )
+// Phase 1 (threads gx in [0, G]):  U[j][gx] = IFFT_64( fold_64( Ib[gx][gy] e^{2 pi i gy p / n} ) )
 // Phase 2: row pairs via C2R packing, four-step IFFT_n, coalesced row writes.
 // ---------------------------------------------------------------------------
 template <int N1, int N2, int TEAMS>
-__global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ Ib, int n, int B,
+__global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ Ib, int n, int G,
                                                     const float2* __restrict__ tw_n,
                                                     float* __restrict__ out) {
   constexpr int N = N1 * N2;
   constexpr int JR = 64;  // rows per work-group
   extern __shared__ c32 smem[];
-  const int G = 2 * B;
+  // G: half-width of the (Hermitian) intensity band
   const int nk = 2 * G + 1;
   const int UP = G + 2;  // padded U row
   c32* U = smem;                                  // JR x UP
@@ -520,26 +520,26 @@ hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2*
 }
 
 template <int N1, int N2>
-static hipError_t band_image_n(int B, int nimg, const float2* Ib, const float2* tw_n, float* out,
+static hipError_t band_image_n(int G, int nimg, const float2* Ib, const float2* tw_n, float* out,
                                hipStream_t s) {
   constexpr int N = N1 * N2;
   constexpr int TS = team_size<N1, N2>();
   constexpr int TEAMS = 256 / TS;
-  const int G = 2 * B, nk = 2 * G + 1;
+  const int nk = 2 * G + 1;
   const size_t words = (size_t)64 * (G + 2) + ((nk + 1) & ~1) + (size_t)TEAMS * (N1 + 1) * N2;
   const size_t shmem = words * sizeof(c32);
   auto kern = k_band_image<N1, N2, TEAMS>;
   hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  HUV_LAUNCH(kern, dim3(N / 64, nimg), dim3(256), shmem, s, Ib, N, B, tw_n, out);
+  HUV_LAUNCH(kern, dim3(N / 64, nimg), dim3(256), shmem, s, Ib, N, G, tw_n, out);
 }
 
-hipError_t launch_band_image(int n, int B, int nimg, const float2* Ib, const float2* tw_n,
+hipError_t launch_band_image(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
                              float* out, hipStream_t s) {
   switch (n) {
-    case 256: return band_image_n<16, 16>(B, nimg, Ib, tw_n, out, s);
-    case 512: return band_image_n<16, 32>(B, nimg, Ib, tw_n, out, s);
-    case 1024: return band_image_n<32, 32>(B, nimg, Ib, tw_n, out, s);
-    case 2048: return band_image_n<32, 64>(B, nimg, Ib, tw_n, out, s);
+    case 256: return band_image_n<16, 16>(G, nimg, Ib, tw_n, out, s);
+    case 512: return band_image_n<16, 32>(G, nimg, Ib, tw_n, out, s);
+    case 1024: return band_image_n<32, 32>(G, nimg, Ib, tw_n, out, s);
+    case 2048: return band_image_n<32, 64>(G, nimg, Ib, tw_n, out, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -24,7 +24,7 @@ hipError_t launch_coarse_rows(int m, int B, int nimg, const float* coarse, const
                               float2* tmp1, hipStream_t s);
 hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2* tmp1,
                               const float2* tw_m, float2* Ib, hipStream_t s);
-hipError_t launch_band_image(int n, int B, int nimg, const float2* Ib, const float2* tw_n,
+hipError_t launch_band_image(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
                              float* out, hipStream_t s);
 
 }  // namespace huvk

@@ -190,6 +190,52 @@ huv_status huv_socs_images_host(huv_ctx* ctx, int nclips, int n, const float* ma
 huv_status huv_band_spectrum(huv_ctx* ctx, int nclips, int n, int band, const float* masks,
                              float* out, int pixel_transfer);
 
+/* ======================================================================== */
+/* ILT on the device (ilt.rs)                                                */
+/* ======================================================================== */
+
+typedef struct huv_ilt huv_ilt;
+
+/* IltProblem settings (ilt.rs:159-224).  cost_kind 0 = image fidelity,
+ * 1 = sigmoid resist contour (threshold, steepness).  beta = mask sigmoid
+ * steepness; tv_weight / tv_epsilon / bin_weight = regularisers; absorber =
+ * amplitude of the m = 0 state (real values only on the device path).  The
+ * Gaussian density filter (min_feature_nm) is not part of the device path. */
+typedef struct {
+  int cost_kind;
+  double threshold;
+  double steepness;
+  double beta;
+  double tv_weight;
+  double tv_epsilon;
+  double bin_weight;
+  double absorber_re;
+  double absorber_im;
+} huv_ilt_config;
+
+/* Bind process conditions (one kernel set per focus, from huv_kset_upload
+ * with eigenvalues divided by the clear field) with dose and weight
+ * (replaces IltProblem::new, ilt.rs:389-478).  Coarse grids up to 64. */
+huv_status huv_ilt_create(huv_ctx* ctx, int n, int nconds, huv_kset* const* conds,
+                          const double* dose, const double* weight, const huv_ilt_config* cfg,
+                          huv_ilt** out);
+void huv_ilt_destroy(huv_ilt* ilt);
+
+/* Batched cost (host, ntargets doubles) and, if grad != NULL, dC/dtheta
+ * (device, ntargets x n x n) at theta (device) for targets (device)
+ * (replaces IltProblem::cost / cost_and_gradient_with, ilt.rs:648-738). */
+huv_status huv_ilt_cost_grad(huv_ilt* ilt, int ntargets, const float* targets, const float* theta,
+                             double* cost, float* grad);
+
+/* One CG (PR+) / steepest-descent stage with Armijo backtracking for every
+ * target (replaces run_stage, ilt.rs:755-866).  theta (device) is updated in
+ * place; cost_hist (host) receives ntargets x (max_iterations + 1) costs
+ * (NaN after a target stops); iterations and termination (0 converged,
+ * 1 max iterations, 2 line search failed) per target. */
+huv_status huv_ilt_optimize(huv_ilt* ilt, int ntargets, const float* targets, float* theta,
+                            int max_iterations, double learning_rate, double convergence_tol,
+                            int use_cg, double* cost_hist, int* iterations, int* termination);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,131 @@
+"""GPU ILT cost and exact adjoint gradient against the FP64 oracle
+(ilt.rs:648-738) and central finite differences; the GPU CG / Armijo loop
+against the oracle's run_stage (ilt.rs:755-866)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import highuvlith_amd as H
+from highuvlith_amd import ilt as L
+from highuvlith_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def c4_engine(n=512):
+    src, opt, grid, K = W.c4_setup()
+    grid = H.GridConfig(n, grid.pixel_nm)
+    return H.AerialImageEngine(src, opt, grid, max_kernels=K)
+
+
+def oracle_problem(eng, cfg, target):
+    conds = [(c.dose, c.weight, eng.kernels(c.defocus_nm)) for c in cfg.conditions]
+    return oracle.IltProblem(eng.grid.size, target, conds, eng.flare_fraction(), cost=cfg.cost,
+                             threshold=cfg.threshold, steepness=cfg.steepness,
+                             beta=cfg.sigmoid_steepness, tv_weight=cfg.regularization,
+                             tv_eps=cfg.tv_epsilon, bin_weight=cfg.binarization_weight,
+                             absorber=cfg.absorber, pixel_nm=eng.grid.pixel_nm)
+
+
+def perturbed_theta(target, seed):
+    n = target.shape[-1]
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    return (target - 0.5) + 0.3 * np.sin(i * 0.037 + seed) * np.cos(j * 0.023 - seed)
+
+
+@pytest.mark.parametrize("case", ["c4_resist", "multi_condition"])
+def test_cost_and_gradient_match_oracle(case):
+    if case == "c4_resist":
+        eng = c4_engine(512)
+        cfg = L.ILTConfig(cost="resist", regularization=0.0)
+        targets = W.c4_targets(2, seed=11, grid=eng.grid)
+    else:
+        eng = c4_engine(256)
+        cfg = L.ILTConfig(cost="image", regularization=0.05, binarization_weight=0.02,
+                          absorber=-math.sqrt(0.06),
+                          conditions=[L.ProcessCondition(0.0, 1.0, 1.0),
+                                      L.ProcessCondition(60.0, 1.05, 0.5)])
+        targets = W.c4_targets(2, seed=12, grid=eng.grid)
+    prob = L.IltProblem(eng, cfg)
+    thetas = np.stack([perturbed_theta(t, s) for s, t in enumerate(targets)])
+    cost, grad = prob.cost_and_gradient(targets, thetas)
+    grad = grad.cpu().numpy()
+    for t in range(len(targets)):
+        orc = oracle_problem(eng, cfg, targets[t])
+        c_ref, g_ref = orc.cost_and_gradient(thetas[t].astype(np.float32).astype(np.float64))
+        rel_c = abs(cost[t] - c_ref) / abs(c_ref)
+        rel_g = np.abs(grad[t] - g_ref).max() / np.abs(g_ref).max()
+        print(case, t, "cost", cost[t], c_ref, rel_c, "grad rel", rel_g)
+        assert rel_c <= 1e-5
+        assert rel_g <= 1e-3
+
+
+def test_gradient_against_finite_differences():
+    """Directional central differences of the GPU cost, and the GPU gradient at
+    32 random pixels against central differences of the FP64 oracle cost."""
+    eng = c4_engine(512)
+    cfg = L.ILTConfig(cost="resist", regularization=0.0)
+    target = W.c4_targets(1, seed=21, grid=eng.grid)
+    theta = perturbed_theta(target[0], 3)[None].astype(np.float32)
+    prob = L.IltProblem(eng, cfg)
+    _, grad = prob.cost_and_gradient(target, theta)
+    g = grad.cpu().numpy()[0]
+    rng = np.random.default_rng(5)
+    n = eng.grid.size
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    h = 1e-2
+    for d in range(3):
+        v = (((i * 31 + j * 17 + d * 7) % 13) / 13.0 - 0.5).astype(np.float32)
+        cp = prob.cost(target, theta + h * v[None])[0]
+        cm = prob.cost(target, theta - h * v[None])[0]
+        fd = (cp - cm) / (2 * h)
+        an = float((g.astype(np.float64) * v).sum())
+        print("directional", fd, an, abs(fd - an) / abs(an))
+        assert abs(fd - an) / abs(an) <= 1e-3
+    orc = oracle_problem(eng, cfg, target[0])
+    th64 = theta[0].astype(np.float64)
+    mag = np.abs(g)
+    cand = np.flatnonzero(mag.ravel() > 1e-2 * mag.max())
+    pix = rng.choice(cand, 32, replace=False)
+    hh = 1e-5
+    worst = 0.0
+    for p in pix:
+        e = np.zeros(n * n)
+        e[p] = hh
+        e = e.reshape(n, n)
+        fd = (orc.cost(th64 + e) - orc.cost(th64 - e)) / (2 * hh)
+        worst = max(worst, abs(g.ravel()[p] - fd) / abs(fd))
+    print("32-pixel FD worst rel", worst)
+    assert worst <= 1e-3
+
+
+def test_gpu_optimizer_tracks_oracle_run_stage():
+    eng = c4_engine(256)
+    cfg = L.ILTConfig(cost="resist", regularization=0.0, max_iterations=12)
+    targets = W.c4_targets(3, seed=31, grid=eng.grid)
+    res = L.optimize_ilt(eng, targets, cfg)
+    for t in range(len(targets)):
+        orc = oracle_problem(eng, cfg, targets[t])
+        _, hist, _ = orc.run_stage(targets[t] - 0.5, 12)
+        g = res.cost_history[t]
+        print(t, g[:4], hist[:4], res.termination[t])
+        assert abs(g[0] - hist[0]) / hist[0] < 1e-5
+        k = min(len(g), len(hist), 6)
+        assert np.all(np.abs(g[:k] - hist[:k]) / hist[:k] < 1e-3)
+        assert np.all(np.diff(g) <= 0)
+        assert abs(g[-1] - hist[-1]) / hist[-1] < 2e-2
+
+
+def test_invalid_ilt_configs_rejected():
+    eng = c4_engine(256)
+    with pytest.raises(H.HuvError):
+        L.IltProblem(eng, L.ILTConfig(conditions=[]))
+    with pytest.raises(H.HuvError):
+        L.IltProblem(eng, L.ILTConfig(min_feature_nm=40.0))
+    with pytest.raises(H.HuvError):
+        L.IltProblem(eng, L.ILTConfig(absorber=0.2j))
+    prob = L.IltProblem(eng, L.ILTConfig())
+    with pytest.raises(H.HuvError):
+        prob.cost(np.zeros((1, 128, 128)), np.zeros((1, 128, 128)))
