@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -147,18 +148,43 @@ Ctx::~Ctx() {
   for (auto& kv : tw) hipFree(kv.second);
   for (auto& kv : sinc) hipFree(kv.second);
   for (auto& kv : plane_tables) hipFree(kv.second);
+  for (auto& kv : plane_packs) {
+    hipFree(kv.second.kern);
+    hipFree(kv.second.lam);
+  }
   if (own_stream) hipStreamDestroy(own_stream);
 }
 
-// Band spectrum stage for nclips real masks on the n grid into Mq (layout
-// [clip][q][kx+B][ky+B], twisted per x class of coarse grid m).
-huv_status band_stage(Ctx* c, int nclips, int n, int B, int m, const float* masks, float2* Mq,
+void Ctx::forget_kset(uint64_t id) {
+  bool busy = false;
+  for (auto it = plane_tables.begin(); it != plane_tables.end();) {
+    if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
+      if (!busy) hipStreamSynchronize(stream());
+      busy = true;
+      hipFree(it->second);
+      it = plane_tables.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  for (auto it = plane_packs.begin(); it != plane_packs.end();) {
+    if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
+      if (!busy) hipStreamSynchronize(stream());
+      busy = true;
+      hipFree(it->second.kern);
+      hipFree(it->second.lam);
+      it = plane_packs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* masks, float2* Mq,
                       bool pixel_transfer) {
   hipError_t e;
   const float2* tw_n = c->twiddles(n, e);
   if (!tw_n) return hip_fail(e, "twiddle table");
-  const float2* tw_m = c->twiddles(m, e);
-  if (!tw_m) return hip_fail(e, "twiddle table");
   const float* st = c->sinc_table(n, B, pixel_transfer, e);
   if (!st) return hip_fail(e, "sinc table");
   const size_t tmp_bytes = (size_t)nclips * (2 * B + 1) * (n / 2) * sizeof(float2);
@@ -170,7 +196,7 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int m, const float* mask
   if (e != hipSuccess) return hip_fail(e, "rows_band launch");
   c->stage_end(ST_ROWS_BAND, ev);
   c->stage_begin(ST_COLS_BAND, &ev);
-  e = huvk::launch_cols_band(n, B, m, nclips, tmp, tw_n, tw_m, st, Mq, c->stream());
+  e = huvk::launch_cols_band(n, B, BB, nclips, tmp, tw_n, st, Mq, c->stream());
   if (e != hipSuccess) return hip_fail(e, "cols_band launch");
   c->stage_end(ST_COLS_BAND, ev);
   return HUV_OK;
@@ -437,21 +463,24 @@ huv_status huv_kset_upload(huv_ctx* c, int n, int K, int64_t S, const int64_t* s
                                      " grid needs a coarse grid above 256 (or above n)");
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
+  static std::atomic<uint64_t> next_id{1};
   auto ks = std::make_unique<huv_kset>();
   ks->ctx = c;
+  ks->id = next_id.fetch_add(1);
   ks->n = n;
   ks->K = K;
   ks->S = S;
   ks->B = B;
   ks->m = m;
+  ks->BB = huvk::coarse_band(m);
   ks->flare = flare;
-  const int W = 2 * B + 1;
+  const int BB = ks->BB, W = 2 * BB + 1;
   std::vector<float2> box((size_t)K * W * W, make_float2(0.f, 0.f));
   for (int k = 0; k < K; ++k)
     for (int64_t t = 0; t < S; ++t) {
       const int64_t i = support[t] / n, j = support[t] % n;
       const int64_t ky = i < n / 2 ? i : i - n, kx = j < n / 2 ? j : j - n;
-      box[((size_t)k * W + (kx + B)) * W + (ky + B)] =
+      box[((size_t)k * W + (kx + BB)) * W + (ky + BB)] =
           make_float2((float)values[2 * (k * S + t)], (float)values[2 * (k * S + t) + 1]);
     }
   // Device eigenvalues carry the 1/n^4 of the two unnormalised coarse
@@ -474,7 +503,15 @@ huv_status huv_kset_upload(huv_ctx* c, int n, int K, int64_t S, const int64_t* s
   return HUV_OK;
 }
 
-void huv_kset_destroy(huv_kset* ks) { delete ks; }
+void huv_kset_destroy(huv_kset* ks) {
+  if (!ks) return;
+  if (ks->ctx) {
+    std::lock_guard<std::mutex> lk(ks->ctx->mu);
+    hipSetDevice(ks->ctx->device);
+    ks->ctx->forget_kset(ks->id);
+  }
+  delete ks;
+}
 
 huv_status huv_kset_shape(const huv_kset* ks, int* band, int* coarse) {
   if (!ks) return fail(HUV_INVALID_PARAMETER, "null kernel set");
@@ -507,7 +544,8 @@ huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes) 
 // tables, uploaded once per distinct plane list.
 huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
                        const float2* const** dkern, const float* const** dlam) {
-  std::vector<const void*> key(planes, planes + nplanes);
+  std::vector<uint64_t> key;
+  for (int p = 0; p < nplanes; ++p) key.push_back(planes[p]->id);
   const size_t offk = 0, offp = ((size_t)nplanes * sizeof(int) + 63) / 64 * 64;
   const size_t offl = offp + (size_t)nplanes * sizeof(void*);
   const size_t tbl = offl + (size_t)nplanes * sizeof(void*);
@@ -537,34 +575,75 @@ huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int**
   return HUV_OK;
 }
 
+// Kernel stacks of a plane list packed into one zero-padded buffer
+// [nplanes][Kmax][W][W] with eigenvalues [nplanes][Kmax] (0 for padding).
+huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, const float2** kern,
+                      const float** lam, int* Kmax) {
+  std::vector<uint64_t> key;
+  for (int p = 0; p < nplanes; ++p) key.push_back(planes[p]->id);
+  auto it = c->plane_packs.find(key);
+  if (it == c->plane_packs.end()) {
+    Ctx::PlanePack pk;
+    for (int p = 0; p < nplanes; ++p) pk.Kmax = std::max(pk.Kmax, planes[p]->K);
+    const int W = 2 * planes[0]->BB + 1;
+    const size_t box = (size_t)W * W;
+    std::vector<float2> hk((size_t)nplanes * pk.Kmax * box, make_float2(0.f, 0.f));
+    std::vector<float> hl((size_t)nplanes * pk.Kmax, 0.f);
+    hipError_t e = hipSuccess;
+    for (int p = 0; p < nplanes && e == hipSuccess; ++p) {
+      const huv_kset* ks = planes[p];
+      e = hipMemcpy(hk.data() + (size_t)p * pk.Kmax * box, ks->kern, (size_t)ks->K * box * sizeof(float2),
+                    hipMemcpyDeviceToHost);
+      if (e == hipSuccess)
+        e = hipMemcpy(hl.data() + (size_t)p * pk.Kmax, ks->lam, (size_t)ks->K * sizeof(float),
+                      hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return hip_fail(e, "plane pack readback");
+    e = hipMalloc(&pk.kern, hk.size() * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&pk.lam, hl.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(pk.kern, hk.data(), hk.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(pk.lam, hl.data(), hl.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(pk.kern);
+      hipFree(pk.lam);
+      return hip_fail(e, "plane pack upload");
+    }
+    it = c->plane_packs.emplace(key, pk).first;
+  }
+  *kern = it->second.kern;
+  *lam = it->second.lam;
+  *Kmax = it->second.Kmax;
+  return HUV_OK;
+}
+
 // Forward images of real masks: band spectrum -> fused fields -> epilogue.
 // Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
                         huv_kset* const* planes, float* out, bool pt, float2** Mq_out) {
-  const int B = planes[0]->B, m = planes[0]->m;
+  const int B = planes[0]->B, m = planes[0]->m, BB = planes[0]->BB;
   const double flare = planes[0]->flare;
   hipError_t e;
-  const int Q = huvk::coarse_classes(m), W = 2 * B + 1;
+  const int W = 2 * BB + 1;
   const int nimg = nclips * nplanes;
-  float2* Mq = (float2*)c->ws.get(Workspace::BAND, (size_t)nclips * Q * W * W * sizeof(float2), e);
+  float2* Mq = (float2*)c->ws.get(Workspace::BAND, (size_t)nclips * W * W * sizeof(float2), e);
   if (!Mq) return hip_fail(e, "workspace");
   if (Mq_out) *Mq_out = Mq;
-  huv_status st = band_stage(c, nclips, n, B, m, masks, Mq, pt);
+  huv_status st = band_stage(c, nclips, n, B, BB, masks, Mq, pt);
   if (st != HUV_OK) return st;
-  const int* dK;
-  const float2* const* dkern;
-  const float* const* dlam;
-  st = plane_table(c, nplanes, planes, &dK, &dkern, &dlam);
+  const float2* kern;
+  const float* lam;
+  int Kmax;
+  st = plane_pack(c, nplanes, planes, &kern, &lam, &Kmax);
   if (st != HUV_OK) return st;
   float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)nimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");
-  hipEvent_t ev;
-  c->stage_begin(ST_FIELDS, &ev);
-  e = huvk::launch_socs_fields(m, B, nclips, nplanes, dK, dkern, dlam, Mq, coarse, c->stream());
-  if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
-  c->stage_end(ST_FIELDS, ev);
   const float2* tw_m = c->twiddles(m, e);
   if (!tw_m) return hip_fail(e, "twiddle table");
+  hipEvent_t ev;
+  c->stage_begin(ST_FIELDS, &ev);
+  e = huvk::launch_socs_fields(m, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
+  c->stage_end(ST_FIELDS, ev);
   const float2* tw_n = c->twiddles(n, e);
   if (!tw_n) return hip_fail(e, "twiddle table");
   const int G = 2 * B;
@@ -639,8 +718,7 @@ huv_status huv_band_spectrum(huv_ctx* c, int nclips, int n, int band, const floa
   if (band < 0 || 2 * band + 1 > n) return fail(HUV_INVALID_PARAMETER, "band out of range");
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  // m = 32 keeps a single x class (no twist): layout [kx+B][ky+B].
-  return band_stage(c, nclips, n, band, 32, masks, (float2*)out, pixel_transfer != 0);
+  return band_stage(c, nclips, n, band, band, masks, (float2*)out, pixel_transfer != 0);
 }
 
 }  // extern "C"

@@ -107,7 +107,7 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const float* theta,
   c->stage_end(ST_ILT_POINTWISE, ev);
   if (!grad) return HUV_OK;
   // |g| <= 2B band of F(G) for every (target, condition) image.
-  st = band_stage(c, T * C, n, 2 * B, 64, b.G, b.Gb, false);
+  st = band_stage(c, T * C, n, 2 * B, 2 * B, b.G, b.Gb, false);
   if (st != HUV_OK) return st;
   const int* dK;
   const float2* const* dkern;
@@ -162,8 +162,9 @@ huv_status huv_ilt_create(huv_ctx* c, int n, int ncond, huv_kset* const* conds, 
     return fail(HUV_INVALID_PARAMETER, "threshold / steepness: must be positive and finite");
   if (cfg->absorber_im != 0.0)
     return fail(HUV_UNSUPPORTED, "complex absorber amplitudes are not supported on the GPU path (real only)");
-  const int B = conds[0]->B, m = conds[0]->m;
-  if (m > 64 || B > 15)
+  // The adjoint works on the stored kernel boxes (half-width coarse_band(m)).
+  const int m = conds[0]->m, B = conds[0]->BB;
+  if (m > 64)
     return fail(HUV_UNSUPPORTED, "GPU ILT adjoint supports coarse grids up to 64 (band <= 15)");
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);

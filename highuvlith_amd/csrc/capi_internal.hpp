@@ -34,8 +34,16 @@ struct Ctx {
   Workspace ws;
   std::map<int, float2*> tw;
   std::map<long, float*> sinc;
-  // Device copies of per-plane pointer tables, keyed by the plane list.
-  std::map<std::vector<const void*>, void*> plane_tables;
+  // Device copies of per-plane pointer tables and packed kernel stacks, keyed
+  // by the ids of the kernel sets in a plane list (dropped when a set dies).
+  std::map<std::vector<uint64_t>, void*> plane_tables;
+  struct PlanePack {
+    float2* kern = nullptr;  // [nplanes][Kmax][W][W]
+    float* lam = nullptr;    // [nplanes][Kmax], zero-padded
+    int Kmax = 0;
+  };
+  std::map<std::vector<uint64_t>, PlanePack> plane_packs;
+  void forget_kset(uint64_t id);
   // Optional per-stage timing with HIP events on the active stream.
   bool profiling = false;
   struct StageEvents { int stage; hipEvent_t a, b; };
@@ -54,12 +62,16 @@ struct Ctx {
 enum Stage { ST_ROWS_BAND = 0, ST_COLS_BAND, ST_FIELDS, ST_COARSE_ROWS, ST_COARSE_COLS,
              ST_BAND_IMAGE, ST_ILT_POINTWISE, ST_ILT_ADJOINT, ST_ILT_OTHER, ST_COUNT };
 
-huv_status band_stage(Ctx* c, int nclips, int n, int B, int m, const float* masks, float2* Mq,
+// Band spectrum of nclips real n x n masks: bins |k| <= B computed, stored
+// in boxes of half-width BB >= B (zero beyond B).  Layout Mq[clip][kx + BB][ky + BB].
+huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* masks, float2* Mq,
                       bool pixel_transfer);
 
 huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes);
 huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
                        const float2* const** dkern, const float* const** dlam);
+huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, const float2** kern,
+                      const float** lam, int* Kmax);
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
                         huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out);
 
@@ -69,10 +81,12 @@ struct huv_ctx : huvi::Ctx {};
 
 struct huv_kset {
   huv_ctx* ctx = nullptr;
+  uint64_t id = 0;  // unique per process, keys the plane caches
   int n = 0, K = 0, B = 0, m = 0;
+  int BB = 0;       // stored box half-width (coarse_band(m) >= B)
   int64_t S = 0;
   double flare = 0.0;
-  float2* kern = nullptr;  // K x W x W, [k][kx+B][ky+B]
+  float2* kern = nullptr;  // K x W x W (W = 2 BB + 1), [k][kx+BB][ky+BB]
   float* lam = nullptr;    // K, eigenvalue / n^4
   std::vector<double> lam_host;
   std::vector<int64_t> support;

@@ -203,10 +203,42 @@ HUV_HD void tw_half(c32* o) {
   }
 }
 
-// In-register FFT of length N = 2^a (N <= 64), natural order in and out.
+template <int SIGN, int N>
+HUV_HD void fft_reg(c32* v);
+
+template <int SIGN, int L, int K>
+HUV_HD void tw_third(c32 (&s)[3][L]) {
+  if constexpr (K < L) {
+    s[1][K] = twiddle<SIGN, K, 3 * L>(s[1][K]);
+    s[2][K] = twiddle<SIGN, 2 * K, 3 * L>(s[2][K]);
+    tw_third<SIGN, L, K + 1>(s);
+  }
+}
+
+// In-register FFT of length N = 2^a or 3 * 2^a (N <= 64), natural order in
+// and out.  Lengths with a factor 3 split off a radix-3 decimation-in-time step.
 template <int SIGN, int N>
 HUV_HD void fft_reg(c32* v) {
-  if constexpr (N == 32) {
+  if constexpr (N == 1) {
+  } else if constexpr (N % 3 == 0) {
+    constexpr int L = N / 3;
+    c32 s[3][L];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int j = 0; j < L; ++j) s[q][j] = v[3 * j + q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) fft_reg<SIGN, L>(s[q]);
+    tw_third<SIGN, L, 0>(s);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      c32 a0 = s[0][k], a1 = s[1][k], a2 = s[2][k];
+      dft3<SIGN>(a0, a1, a2);
+      v[k] = a0;
+      v[k + L] = a1;
+      v[k + 2 * L] = a2;
+    }
+  } else if constexpr (N == 32) {
     // 32 = 2 x 16: decimation in time.
     c32 e[16], o[16];
 #pragma unroll

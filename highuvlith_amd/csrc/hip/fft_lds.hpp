@@ -18,12 +18,7 @@ __device__ __forceinline__ c32 tw_lookup(const float2* __restrict__ tw, int k) {
 
 template <int SIGN, int N>
 __device__ __forceinline__ void fft_small(c32* v) {
-  if constexpr (N == 1) {
-  } else if constexpr (N == 3) {
-    dft3<SIGN>(v[0], v[1], v[2]);
-  } else {
-    fft_reg<SIGN, N>(v);
-  }
+  fft_reg<SIGN, N>(v);
 }
 
 // Threads per team for a four-step FFT of N1 x N2.

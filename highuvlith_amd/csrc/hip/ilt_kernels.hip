@@ -291,8 +291,8 @@ __global__ __launch_bounds__(M) void k_ilt_adjoint(
   }
 }
 
-// H[fx][fy] (fx in [0, B], fy in [-B, B]) = sum_c w_c (1-a) (R_c[f] + conj R_c[-f]) / n^2,
-// layout [fx][fy + B] for band_image with band half-width B.
+// H[fy][fx] (fx in [0, B], fy in [-B, B]) = sum_c w_c (1-a) (R_c[f] + conj R_c[-f]) / n^2,
+// layout [fy + B][fx] (the Hermitian half band read by band_image, half-width B).
 __global__ void k_ilt_hband(const float2* __restrict__ Rin, int B, int ncond,
                             const float* __restrict__ weight, float scale,
                             float2* __restrict__ H) {
@@ -300,7 +300,7 @@ __global__ void k_ilt_hband(const float2* __restrict__ Rin, int B, int ncond,
   const int W = 2 * B + 1;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (B + 1) * W) return;
-  const int fx = idx / W, fy = idx % W - B;
+  const int fx = idx % (B + 1), fy = idx / (B + 1) - B;
   c32 acc{0.f, 0.f};
   for (int c = 0; c < ncond; ++c) {
     const float2* r = Rin + ((size_t)tgt * ncond + c) * W * W;

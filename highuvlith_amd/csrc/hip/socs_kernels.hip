@@ -61,27 +61,29 @@ __global__ __launch_bounds__(TEAMS * N2) void k_rows_band(const float* __restric
 // ---------------------------------------------------------------------------
 // cols_band: for column kx in [0, B], assemble the column of row spectra
 // from the packed pairs, FFT_n forward along y, keep ky in [-B, B].  Writes
-// Mq[clip][q][kx + B][ky + B] = M[ky][kx] * pt * e^{+2 pi i kx q / m} and the
-// Hermitian mirror at (-kx, -ky).  pt = pixel transfer sinc(pi kx/n) sinc(pi ky/n)
-// (table sinc_t[k + B], all ones when disabled).
+// Mq[clip][kx + BB][ky + BB] = M[ky][kx] * pt and the Hermitian mirror at
+// (-kx, -ky); the rest of the (2BB+1)^2 box (BB >= B) is written as zeros.
+// pt = pixel transfer sinc(pi kx/n) sinc(pi ky/n) (table sinc_t[k + B], all
+// ones when disabled).
 // ---------------------------------------------------------------------------
 template <int N1, int N2, int TEAMS>
 __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
-    const float2* __restrict__ tmp, int B, int m, int Q, const float2* __restrict__ tw_n,
-    const float2* __restrict__ tw_m, const float* __restrict__ sinc_t, float2* __restrict__ Mq) {
+    const float2* __restrict__ tmp, int B, int BB, const float2* __restrict__ tw_n,
+    const float* __restrict__ sinc_t, float2* __restrict__ Mq) {
   constexpr int N = N1 * N2;
   __shared__ c32 scratch[TEAMS][(N1 + 1) * N2];
   const int team = threadIdx.x / N2, t = threadIdx.x % N2;
   const int clip = blockIdx.y;
   const int kx = blockIdx.x * TEAMS + team;
-  const bool active = kx <= B;
-  const int kxa = active ? kx : 0;
+  const bool active = kx <= BB;
+  const bool live = kx <= B;
+  const int kxa = live ? kx : 0;
   const int npairs = N / 2;
   const int nk = 2 * B + 1;
-  const int W = 2 * B + 1;
+  const int W = 2 * BB + 1;
   const float2* col_pos = tmp + ((size_t)clip * nk + kxa) * npairs;
   const float2* col_neg = tmp + ((size_t)clip * nk + (kxa == 0 ? 0 : B + kxa)) * npairs;
-  float2* mq = Mq + (size_t)clip * Q * W * W;
+  float2* mq = Mq + (size_t)clip * W * W;
   fft_four_step<-1, N1, N2>(
       t, scratch[team], tw_n,
       [&](int i) {
@@ -94,165 +96,200 @@ __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
       [&](int k, c32 v) {
         if (!active) return;
         int ky;
-        if (k <= B) ky = k;
-        else if (k >= N - B) ky = k - N;
+        if (k <= BB) ky = k;
+        else if (k >= N - BB) ky = k - N;
         else return;
-        const float pt = sinc_t[kx + B] * sinc_t[ky + B];
-        v = cscale(v, pt);
-        for (int q = 0; q < Q; ++q) {
-          const c32 w = tw_lookup<+1>(tw_m, (kx * q) % m);
-          const c32 a = cmul(v, w);
-          mq[((size_t)q * W + (kx + B)) * W + (ky + B)] = make_float2(a.re, a.im);
-          if (kx > 0) {
-            // conj(M) e^{-2 pi i kx q / m} = conj(M e^{+...})
-            mq[((size_t)q * W + (B - kx)) * W + (B - ky)] = make_float2(a.re, -a.im);
-          }
-        }
+        if (live && ky >= -B && ky <= B) v = cscale(v, sinc_t[kx + B] * sinc_t[ky + B]);
+        else v = c32{0.f, 0.f};
+        mq[(size_t)(kx + BB) * W + (ky + BB)] = make_float2(v.re, v.im);
+        if (kx > 0) mq[(size_t)(BB - kx) * W + (BB - ky)] = make_float2(v.re, -v.im);
       });
 }
 
 // ---------------------------------------------------------------------------
-// socs_fields: one work-group per (clip, x class q); all planes of the clip.
-// FX = M/Q columns per class, FY = M/R rows per y task.  Threads: FX * R.
-// x task (band row iy):  c[h] = sum_{kx = h mod FX} K[k][kx][iy] Mq[q][kx][iy];
-//                        T[iy][j] = IFFT_FX(c)[j]   (E row at x = Q j + q)
-// y task (column j, residue r):
-//                        d[h] = e^{2 pi i h r/M} sum_{ky = h mod FY} T[ky][j] e^{2 pi i t r/R}
-//                        E[R j' + r][Q j + q] = IFFT_FY(d)[j'];  acc += lambda |E|^2
-// Coarse intensity layout: coarse[img][q][y][j].
+// socs_fields: one work-group per (clip, x class q), all planes of the clip.
+// Coarse grid M = 32 Q, Q = R = M / 32 classes and residues, band half-width
+// BB = (M - 1) / 4 at compile time (kernel boxes are zero beyond the set's
+// own band).  With P = K . M (untwisted band product), a field sample is
+//   E[y][x] = sum_{kx, ky} P[kx][ky] e^{2 pi i (kx x + ky y) / M}.
+// x group, band row iy (ky = iy - BB), for the columns x = Q j + q:
+//   c[h] = e^{2 pi i h q / M} sum_t P[h + 32 t][ky] e^{2 pi i t q / Q}
+//   T[iy][j] = IFFT_32(c)[j]
+// y group, column j and residue r, for the rows y = R j' + r:
+//   d[h] = e^{2 pi i h r / M} sum_t T[h + 32 t][j] e^{2 pi i t r / R}
+//   E[R j' + r][Q j + q] = IFFT_32(d)[j'];  acc += lambda |E|^2
+// The groups run software-pipelined over a double-buffered LDS tile (step s
+// synthesises kernel s while kernel s - 1 is squared and summed), one barrier
+// per step.  Coarse intensity layout: coarse[img][y][x].
+// Work-groups are ordered so that the Q classes of one clip share an XCD
+// (consecutive work-group ids are dealt round-robin over the 8 XCDs).
 // ---------------------------------------------------------------------------
-template <int M, int Q, int R>
-struct FieldCfg {
-  static constexpr int FX = M / Q;
-  static constexpr int FY = M / R;
-  static constexpr int THREADS = FX * R;
+template <int M>
+struct CoarseCfg {
+  static constexpr int F = 32;  // transform length of both passes
+  static constexpr int Q = M / F;
+  static constexpr int R = Q;
+  static constexpr int BB = (M - 1) / 4;
+  static constexpr int W = 2 * BB + 1;
+  static constexpr int TP = F + 1;
+  static constexpr int TMIN = -((BB + F - 1) / F);  // fold range of k = h + 32 t
+  static constexpr int TMAX = BB / F;
+  static constexpr int YT = F * R;
+  static constexpr int XOFF = ((YT + 63) / 64) * 64;
+  static constexpr int XT = ((W + 63) / 64) * 64;
+  static constexpr int THREADS = XOFF + XT;
+  static constexpr int LDS_C32 = 2 * W * TP + R * F + F;
 };
 
-// d[h] *= e^{SIGN 2 pi i h r / M} for h < FY (compile-time r).
-template <int SIGN, int M, int r, int H, int FY>
-__device__ __forceinline__ void ytask_twiddle(c32* d) {
-  if constexpr (H < FY) {
-    d[H] = twiddle<SIGN, (long long)H * r, M>(d[H]);
-    ytask_twiddle<SIGN, M, r, H + 1, FY>(d);
-  }
+constexpr int kXcds = 8;
+
+// e^{+2 pi i e / M} from the forward W_M table.
+__device__ __forceinline__ c32 tw_pos(const float2* __restrict__ tw_m, int e, int M) {
+  e %= M;
+  if (e < 0) e += M;
+  return tw_lookup<+1>(tw_m, e);
 }
 
-// y task of socs_fields for residue r: fold the band rows of column j into
-// FY bins, twist, inverse FFT_FY, accumulate lambda |E|^2.
-template <int M, int R, int FY, int TP, int r>
-__device__ __forceinline__ void ytask(const c32* __restrict__ T, int B, int j, float l,
-                                      float* acc) {
-  c32 d[FY];
-#pragma unroll
-  for (int h = 0; h < FY; ++h) {
-    d[h] = c32{0.f, 0.f};
-    if (h <= B) d[h] = T[(h + B) * TP + j];
-  }
-  // negative ky = h - FY carries e^{-2 pi i r / R}
-#pragma unroll
-  for (int h = 0; h < FY; ++h) {
-    if (h - FY >= -B) d[h] = cadd(d[h], twiddle<+1, -(long long)r, R>(T[(h - FY + B) * TP + j]));
-  }
-  ytask_twiddle<+1, M, r, 0, FY>(d);
-  fft_reg<+1, FY>(d);
-#pragma unroll
-  for (int i = 0; i < FY; ++i) acc[i] += l * cnorm(d[i]);
+// Raw buffer resource over `count` complex values (gfx9 word 3; reads past
+// the end return zero).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t box_rsrc(const float2* p, int count) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(p), (short)0,
+                                           count * (int)sizeof(float2), 0x00020000);
 }
 
-template <int M, int Q, int R>
-__global__ __launch_bounds__((FieldCfg<M, Q, R>::THREADS)) void k_socs_fields(
-    const float2* __restrict__ Mq, int B, int nplanes, const int* __restrict__ plane_K,
-    const float2* const* __restrict__ plane_kern, const float* const* __restrict__ plane_lam,
-    float* __restrict__ coarse) {
-  using C = FieldCfg<M, Q, R>;
-  constexpr int FX = C::FX, FY = C::FY;
-  constexpr int TP = FX + 1;  // padded T row
-  extern __shared__ c32 T[];  // (2B+1) x TP
-  const int q = blockIdx.x, clip = blockIdx.y;
-  const int W = 2 * B + 1;
+__device__ __forceinline__ c32 load_c32(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  const float2 f = __builtin_bit_cast(float2, v);
+  return c32{f.x, f.y};
+}
+
+template <int M>
+__global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_socs_fields(
+    const float2* __restrict__ Mq, const float2* __restrict__ kern, const float* __restrict__ lam,
+    int nclips, int nplanes, int Kmax, const float2* __restrict__ tw_m, float* __restrict__ coarse) {
+  using C = CoarseCfg<M>;
+  constexpr int F = C::F, Q = C::Q, R = C::R, BB = C::BB, W = C::W, TP = C::TP;
+  constexpr int TMIN = C::TMIN, TMAX = C::TMAX, NT = TMAX - TMIN + 1;
+  extern __shared__ c32 lds[];
+  c32* T = lds;                    // 2 x W x TP
+  c32* twy = lds + 2 * W * TP;     // [r][h] = e^{2 pi i h r / M}
+  c32* twx = twy + R * F;          // [h] = e^{2 pi i h q / M}
+  const int L = blockIdx.x;
+  const int slot = L / kXcds;
+  const int q = slot % Q;
+  const int clip = (slot / Q) * kXcds + L % kXcds;
+  if (clip >= nclips) return;  // whole work-group, before any barrier
   const int tid = threadIdx.x;
-  const float2* mq = Mq + ((size_t)clip * Q + q) * W * W;
-  const int j = tid % FX, r = tid / FX;
-
-  for (int p = 0; p < nplanes; ++p) {
-    const int K = plane_K[p];
-    const float2* kern = plane_kern[p];
-    const float* lam = plane_lam[p];
-    float acc[FY];
+  for (int i = tid; i < R * F; i += C::THREADS) twy[i] = tw_pos(tw_m, (i % F) * (i / F), M);
+  for (int i = tid; i < F; i += C::THREADS) twx[i] = tw_pos(tw_m, i * q, M);
+  const __amdgpu_buffer_rsrc_t mr = box_rsrc(Mq + (size_t)clip * W * W, W * W);
+  const int steps = nplanes * Kmax;
+  const bool is_y = tid < C::YT;
+  const int j = tid % F, r = tid / F;
+  const int iy = tid - C::XOFF;
+  // fold factors e^{2 pi i t q / Q} (x) or e^{2 pi i t r / R} (y): M / Q = 32
+  c32 wf[NT];
+  {
+    const int u = is_y ? r : q;
 #pragma unroll
-    for (int i = 0; i < FY; ++i) acc[i] = 0.f;
-    for (int k = 0; k < K; ++k) {
-      // ---- x pass ----
-      for (int iy = tid; iy < W; iy += C::THREADS) {
-        const float2* kk = kern + (size_t)k * W * W + iy;
-        const float2* mm = mq + iy;
-        c32 c[FX];
+    for (int t = 0; t < NT; ++t) wf[t] = tw_pos(tw_m, (t + TMIN) * u * F, M);
+  }
+  float acc[F];
 #pragma unroll
-        for (int h = 0; h < FX; ++h) {
-          c[h] = c32{0.f, 0.f};
-          // kx = h (t = 0) and kx = h - FX (t = -1)
-          if (h <= B) {
-            const float2 a = kk[(size_t)(h + B) * W], b = mm[(size_t)(h + B) * W];
-            c[h] = cadd(c[h], cmul(c32{a.x, a.y}, c32{b.x, b.y}));
+  for (int i = 0; i < F; ++i) acc[i] = 0.f;
+  __syncthreads();
+  for (int s = 0; s <= steps; ++s) {
+    if (iy >= 0) {
+      if (s < steps && iy < W) {
+        // Buffer loads: one VGPR offset per lane, the column offset in an SGPR.
+        const __amdgpu_buffer_rsrc_t kr = box_rsrc(kern + (size_t)s * W * W, W * W);
+        const int vo = iy * (int)sizeof(float2);
+        c32 c[F];
+#pragma unroll
+        for (int h = 0; h < F; ++h) {
+          c32 v{0.f, 0.f};
+#pragma unroll
+          for (int t = TMIN; t <= TMAX; ++t) {
+            const int kx = h + F * t;
+            if (kx >= -BB && kx <= BB) {
+              const int so = (kx + BB) * W * (int)sizeof(float2);
+              const c32 a = load_c32(kr, vo, so), b = load_c32(mr, vo, so);
+              const c32 p = cmul(a, b);
+              v = cadd(v, t == 0 ? p : cmul(p, wf[t - TMIN]));
+            }
           }
-          if (h - FX >= -B) {
-            const float2 a = kk[(size_t)(h - FX + B) * W], b = mm[(size_t)(h - FX + B) * W];
-            c[h] = cadd(c[h], cmul(c32{a.x, a.y}, c32{b.x, b.y}));
+          c[h] = cmul(v, twx[h]);
+        }
+        fft_reg<+1, F>(c);
+        c32* Tb = T + (s & 1) * W * TP;
+#pragma unroll
+        for (int jj = 0; jj < F; ++jj) Tb[iy * TP + jj] = c[jj];
+      }
+    } else if (is_y && s >= 1) {
+      const int sp = s - 1;
+      const c32* Tb = T + (sp & 1) * W * TP;
+      const c32* ty = twy + r * F;
+      c32 d[F];
+#pragma unroll
+      for (int h = 0; h < F; ++h) {
+        c32 v{0.f, 0.f};
+#pragma unroll
+        for (int t = TMIN; t <= TMAX; ++t) {
+          const int ky = h + F * t;
+          if (ky >= -BB && ky <= BB) {
+            const c32 e = Tb[(ky + BB) * TP + j];
+            v = cadd(v, t == 0 ? e : cmul(e, wf[t - TMIN]));
           }
         }
-        fft_reg<+1, FX>(c);
-#pragma unroll
-        for (int jj = 0; jj < FX; ++jj) T[iy * TP + jj] = c[jj];
+        d[h] = cmul(v, ty[h]);
       }
-      __syncthreads();
-      // ---- y pass ----
-      {
-        const float l = lam[k];
-        switch (r) {
-          case 0: ytask<M, R, FY, TP, 0>(T, B, j, l, acc); break;
-          case 1: if constexpr (R > 1) ytask<M, R, FY, TP, 1>(T, B, j, l, acc); break;
-          case 2: if constexpr (R > 2) ytask<M, R, FY, TP, 2>(T, B, j, l, acc); break;
-          case 3: if constexpr (R > 3) ytask<M, R, FY, TP, 3>(T, B, j, l, acc); break;
-          default: break;
+      fft_reg<+1, F>(d);
+      const float l = lam[sp];
+#pragma unroll
+      for (int i = 0; i < F; ++i) acc[i] += l * cnorm(d[i]);
+      if (sp % Kmax == Kmax - 1) {
+        float* out = coarse + ((size_t)clip * nplanes + sp / Kmax) * M * M;
+#pragma unroll
+        for (int i = 0; i < F; ++i) {
+          out[(size_t)(R * i + r) * M + Q * j + q] = acc[i];
+          acc[i] = 0.f;
         }
       }
-      __syncthreads();
     }
-    float* out = coarse + (((size_t)clip * nplanes + p) * Q + q) * M * FX;
-#pragma unroll
-    for (int i = 0; i < FY; ++i) out[(size_t)(R * i + r) * FX + j] = acc[i];
+    __syncthreads();
   }
 }
 
+// Four-step factorisation M = N1 x N2 of the coarse transforms.
+template <int M>
+struct CoarseFft {
+  static constexpr int N1 = M == 32 ? 4 : M == 64 ? 8 : M == 128 ? 8 : M == 192 ? 12 : 16;
+  static constexpr int N2 = M / N1;
+};
+
 // ---------------------------------------------------------------------------
-// coarse_rows: coarse row pairs -> FFT_M forward, keep gx in [0, 2B] and
-// [M - 2B, M).  tmp1 layout [img][kk][pair] with kk in [0, 4B].
+// coarse_rows: coarse row pairs -> FFT_M forward, keep gx in [0, G] and
+// [M - G, M) (G = 2B).  tmp1 layout [img][kk][pair] with kk in [0, 2G].
 // ---------------------------------------------------------------------------
-template <int M, int Q, int N1, int N2, int TEAMS>
-__global__ __launch_bounds__((TEAMS * team_size<N1, N2>())) void k_coarse_rows(
+template <int M, int TEAMS>
+__global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_rows(
     const float* __restrict__ coarse, int B, const float2* __restrict__ tw_m,
     float2* __restrict__ tmp1) {
-  constexpr int TS = team_size<N1, N2>();
-  constexpr int FX = M / Q;
+  constexpr int N1 = CoarseFft<M>::N1, N2 = CoarseFft<M>::N2;
   __shared__ c32 scratch[TEAMS][(N1 + 1) * N2];
-  const int team = threadIdx.x / TS, t = threadIdx.x % TS;
+  const int team = threadIdx.x / N2, t = threadIdx.x % N2;
   const int img = blockIdx.y;
   const int pair = blockIdx.x * TEAMS + team;
   const int npairs = M / 2;
   const bool active = pair < npairs;
-  const int ya = 2 * (active ? pair : 0), yb = ya + 1;
-  const float* cimg = coarse + (size_t)img * M * M;
+  const float* ra = coarse + (size_t)img * M * M + (size_t)2 * (active ? pair : 0) * M;
+  const float* rb = ra + M;
   const int G = 2 * B;
   const int nk = 2 * G + 1;
   float2* out = tmp1 + (size_t)img * nk * npairs;
   fft_four_step<-1, N1, N2>(
       t, scratch[team], tw_m,
-      [&](int x) {
-        if (!active) return c32{0.f, 0.f};
-        const int qq = x % Q, jj = x / Q;
-        const float* base = cimg + (size_t)qq * M * FX;
-        return c32{base[(size_t)ya * FX + jj], base[(size_t)yb * FX + jj]};
-      },
+      [&](int x) { return active ? c32{ra[x], rb[x]} : c32{0.f, 0.f}; },
       [&](int k, c32 v) {
         if (!active) return;
         int kk = -1;
@@ -262,15 +299,15 @@ __global__ __launch_bounds__((TEAMS * team_size<N1, N2>())) void k_coarse_rows(
       });
 }
 
-// coarse_cols: column gx in [0, 2B]: separate the packed pairs, FFT_M along y,
-// keep gy in [-2B, 2B]; Ib[img][gx][gy + 2B] = v / M^2, times (1 - phi) except DC.
-template <int M, int N1, int N2, int TEAMS>
-__global__ __launch_bounds__((TEAMS * team_size<N1, N2>())) void k_coarse_cols(
+// coarse_cols: column gx in [0, G]: separate the packed pairs, FFT_M along y,
+// keep gy in [-G, G]; Ib[img][gy + G][gx] = v / M^2, times (1 - phi) except DC.
+template <int M, int TEAMS>
+__global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_cols(
     const float2* __restrict__ tmp1, int B, float flare, const float2* __restrict__ tw_m,
     float2* __restrict__ Ib) {
-  constexpr int TS = team_size<N1, N2>();
+  constexpr int N1 = CoarseFft<M>::N1, N2 = CoarseFft<M>::N2;
   __shared__ c32 scratch[TEAMS][(N1 + 1) * N2];
-  const int team = threadIdx.x / TS, t = threadIdx.x % TS;
+  const int team = threadIdx.x / N2, t = threadIdx.x % N2;
   const int img = blockIdx.y;
   const int G = 2 * B;
   const int gx = blockIdx.x * TEAMS + team;
@@ -280,7 +317,7 @@ __global__ __launch_bounds__((TEAMS * team_size<N1, N2>())) void k_coarse_cols(
   const int nk = 2 * G + 1;
   const float2* col_pos = tmp1 + ((size_t)img * nk + gxa) * npairs;
   const float2* col_neg = tmp1 + ((size_t)img * nk + (gxa == 0 ? 0 : G + gxa)) * npairs;
-  float2* out = Ib + ((size_t)img * (G + 1) + gxa) * nk;
+  float2* out = Ib + (size_t)img * nk * (G + 1) + gxa;
   const float inv = 1.0f / ((float)M * (float)M);
   fft_four_step<-1, N1, N2>(
       t, scratch[team], tw_m,
@@ -297,13 +334,14 @@ __global__ __launch_bounds__((TEAMS * team_size<N1, N2>())) void k_coarse_cols(
         else if (k >= M - G) gy = k - M;
         else return;
         const float s = (gx == 0 && gy == 0) ? inv : inv * (1.0f - flare);
-        out[gy + G] = make_float2(v.re * s, v.im * s);
+        out[(size_t)(gy + G) * (G + 1)] = make_float2(v.re * s, v.im * s);
       });
 }
 
 // ---------------------------------------------------------------------------
-// band_image: rows y = P j + p (P = n / 64) of the n x n image.
-// Phase 1 (threads gx in [0, G]):  U[j][gx] = IFFT_64( fold_64( Ib[gx][gy] e^{2 pi i gy p / n} ) )
+// band_image: rows y = P j + p (P = n / 64) of the n x n image from the
+// Hermitian half band Ib[gy + G][gx] (gx in [0, G]).
+// Phase 1 (threads gx):  U[j][gx] = IFFT_64( fold_64( Ib[gy][gx] e^{2 pi i gy p / n} ) )
 // Phase 2: row pairs via C2R packing, four-step IFFT_n, coalesced row writes.
 // ---------------------------------------------------------------------------
 template <int N1, int N2, int TEAMS>
@@ -313,7 +351,6 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
   constexpr int N = N1 * N2;
   constexpr int JR = 64;  // rows per work-group
   extern __shared__ c32 smem[];
-  // G: half-width of the (Hermitian) intensity band
   const int nk = 2 * G + 1;
   const int UP = G + 2;  // padded U row
   c32* U = smem;                                  // JR x UP
@@ -328,9 +365,8 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     wt[i] = tw_lookup<+1>(tw_n, e);
   }
   __syncthreads();
-  const float2* ib = Ib + (size_t)img * (G + 1) * nk;
+  const float2* ib = Ib + (size_t)img * nk * (G + 1);
   for (int gx = tid; gx <= G; gx += 256) {
-    const float2* col = ib + (size_t)gx * nk;
     c32 d[JR];
 #pragma unroll
     for (int h = 0; h < JR; ++h) {
@@ -339,7 +375,7 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
       for (int tt = -2; tt <= 1; ++tt) {
         const int gy = h + JR * tt;
         if (gy >= -G && gy <= G) {
-          const float2 v = col[gy + G];
+          const float2 v = ib[(size_t)(gy + G) * (G + 1) + gx];
           d[h] = cadd(d[h], cmul(c32{v.x, v.y}, wt[gy + G]));
         }
       }
@@ -387,19 +423,13 @@ namespace huvk {
 int choose_coarse(int n, int B) {
   const int sizes[] = {32, 64, 128, 192, 256};
   for (int m : sizes)
-    if (m >= 4 * B + 1 && m <= n && B < m / coarse_classes(m)) return m;
+    if (m >= 4 * B + 1 && m <= n) return m;
   return 0;
 }
 
-int coarse_classes(int m) {
-  switch (m) {
-    case 32: case 64: return 1;
-    case 128: return 2;
-    case 192: return 3;
-    case 256: return 4;
-    default: return 0;
-  }
-}
+int coarse_classes(int m) { return m % 32 == 0 ? m / 32 : 0; }
+
+int coarse_band(int m) { return (m - 1) / 4; }
 
 #define HUV_LAUNCH(kern, grid, block, shmem, s, ...)                   \
   do {                                                                 \
@@ -429,92 +459,87 @@ hipError_t launch_rows_band(int n, int B, int nclips, const float* masks, const 
 }
 
 template <int N1, int N2>
-static hipError_t cols_band_n(int B, int m, int nclips, const float2* tmp, const float2* tw_n,
-                              const float2* tw_m, const float* sinc_t, float2* Mq,
-                              hipStream_t s) {
+static hipError_t cols_band_n(int B, int BB, int nclips, const float2* tmp, const float2* tw_n,
+                              const float* sinc_t, float2* Mq, hipStream_t s) {
   constexpr int TEAMS = 128 / N2 > 0 ? 128 / N2 : 1;
-  dim3 grid((B + 1 + TEAMS - 1) / TEAMS, nclips);
-  HUV_LAUNCH((k_cols_band<N1, N2, TEAMS>), grid, dim3(TEAMS * N2), 0, s, tmp, B, m,
-             coarse_classes(m), tw_n, tw_m, sinc_t, Mq);
+  dim3 grid((BB + 1 + TEAMS - 1) / TEAMS, nclips);
+  HUV_LAUNCH((k_cols_band<N1, N2, TEAMS>), grid, dim3(TEAMS * N2), 0, s, tmp, B, BB, tw_n, sinc_t,
+             Mq);
 }
 
-hipError_t launch_cols_band(int n, int B, int m, int nclips, const float2* tmp,
-                            const float2* tw_n, const float2* tw_m, const float* sinc_t,
-                            float2* Mq, hipStream_t s) {
+hipError_t launch_cols_band(int n, int B, int BB, int nclips, const float2* tmp,
+                            const float2* tw_n, const float* sinc_t, float2* Mq, hipStream_t s) {
+  if (BB < B || 2 * BB + 1 > n) return hipErrorInvalidValue;
   switch (n) {
-    case 256: return cols_band_n<16, 16>(B, m, nclips, tmp, tw_n, tw_m, sinc_t, Mq, s);
-    case 512: return cols_band_n<16, 32>(B, m, nclips, tmp, tw_n, tw_m, sinc_t, Mq, s);
-    case 1024: return cols_band_n<32, 32>(B, m, nclips, tmp, tw_n, tw_m, sinc_t, Mq, s);
-    case 2048: return cols_band_n<32, 64>(B, m, nclips, tmp, tw_n, tw_m, sinc_t, Mq, s);
+    case 256: return cols_band_n<16, 16>(B, BB, nclips, tmp, tw_n, sinc_t, Mq, s);
+    case 512: return cols_band_n<16, 32>(B, BB, nclips, tmp, tw_n, sinc_t, Mq, s);
+    case 1024: return cols_band_n<32, 32>(B, BB, nclips, tmp, tw_n, sinc_t, Mq, s);
+    case 2048: return cols_band_n<32, 64>(B, BB, nclips, tmp, tw_n, sinc_t, Mq, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int M, int Q, int R>
-static hipError_t fields_m(int B, int nclips, int nplanes, const int* plane_K,
-                           const float2* const* plane_kern, const float* const* plane_lam,
-                           const float2* Mq, float* coarse, hipStream_t s) {
-  using C = FieldCfg<M, Q, R>;
-  const size_t shmem = (size_t)(2 * B + 1) * (C::FX + 1) * sizeof(c32);
-  auto kern = k_socs_fields<M, Q, R>;
-  hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  HUV_LAUNCH(kern, dim3(Q, nclips), dim3(C::THREADS), shmem, s, Mq, B, nplanes, plane_K,
-             plane_kern, plane_lam, coarse);
+template <int M>
+static hipError_t fields_m(int nclips, int nplanes, int Kmax, const float2* kern, const float* lam,
+                           const float2* Mq, const float2* tw_m, float* coarse, hipStream_t s) {
+  using C = CoarseCfg<M>;
+  const size_t shmem = (size_t)C::LDS_C32 * sizeof(c32);
+  auto kernel = k_socs_fields<M>;
+  hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  const int groups = (nclips + kXcds - 1) / kXcds;
+  HUV_LAUNCH(kernel, dim3(groups * kXcds * C::Q), dim3(C::THREADS), shmem, s, Mq, kern, lam,
+             nclips, nplanes, Kmax, tw_m, coarse);
 }
 
-hipError_t launch_socs_fields(int m, int B, int nclips, int nplanes, const int* plane_K,
-                              const float2* const* plane_kern, const float* const* plane_lam,
-                              const float2* Mq, float* coarse, hipStream_t s) {
+hipError_t launch_socs_fields(int m, int nclips, int nplanes, int Kmax, const float2* kern,
+                              const float* lam, const float2* Mq, const float2* tw_m,
+                              float* coarse, hipStream_t s) {
   switch (m) {
-    case 32: return fields_m<32, 1, 1>(B, nclips, nplanes, plane_K, plane_kern, plane_lam, Mq, coarse, s);
-    case 64: return fields_m<64, 1, 1>(B, nclips, nplanes, plane_K, plane_kern, plane_lam, Mq, coarse, s);
-    case 128: return fields_m<128, 2, 2>(B, nclips, nplanes, plane_K, plane_kern, plane_lam, Mq, coarse, s);
-    case 192: return fields_m<192, 3, 3>(B, nclips, nplanes, plane_K, plane_kern, plane_lam, Mq, coarse, s);
-    case 256: return fields_m<256, 4, 4>(B, nclips, nplanes, plane_K, plane_kern, plane_lam, Mq, coarse, s);
+    case 32: return fields_m<32>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 64: return fields_m<64>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 128: return fields_m<128>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 192: return fields_m<192>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 256: return fields_m<256>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int M, int Q, int N1, int N2>
+template <int M>
 static hipError_t coarse_rows_m(int B, int nimg, const float* coarse, const float2* tw_m,
                                 float2* tmp1, hipStream_t s) {
-  constexpr int TS = team_size<N1, N2>();
-  constexpr int TEAMS = TS == 1 ? 64 : 256 / TS;
+  constexpr int TEAMS = 256 / CoarseFft<M>::N2;
   dim3 grid((M / 2 + TEAMS - 1) / TEAMS, nimg);
-  HUV_LAUNCH((k_coarse_rows<M, Q, N1, N2, TEAMS>), grid, dim3(TEAMS * TS), 0, s, coarse, B, tw_m,
-             tmp1);
+  HUV_LAUNCH((k_coarse_rows<M, TEAMS>), grid, dim3(256), 0, s, coarse, B, tw_m, tmp1);
 }
 
 hipError_t launch_coarse_rows(int m, int B, int nimg, const float* coarse, const float2* tw_m,
                               float2* tmp1, hipStream_t s) {
   switch (m) {
-    case 32: return coarse_rows_m<32, 1, 1, 32>(B, nimg, coarse, tw_m, tmp1, s);
-    case 64: return coarse_rows_m<64, 1, 1, 64>(B, nimg, coarse, tw_m, tmp1, s);
-    case 128: return coarse_rows_m<128, 2, 2, 64>(B, nimg, coarse, tw_m, tmp1, s);
-    case 192: return coarse_rows_m<192, 3, 3, 64>(B, nimg, coarse, tw_m, tmp1, s);
-    case 256: return coarse_rows_m<256, 4, 4, 64>(B, nimg, coarse, tw_m, tmp1, s);
+    case 32: return coarse_rows_m<32>(B, nimg, coarse, tw_m, tmp1, s);
+    case 64: return coarse_rows_m<64>(B, nimg, coarse, tw_m, tmp1, s);
+    case 128: return coarse_rows_m<128>(B, nimg, coarse, tw_m, tmp1, s);
+    case 192: return coarse_rows_m<192>(B, nimg, coarse, tw_m, tmp1, s);
+    case 256: return coarse_rows_m<256>(B, nimg, coarse, tw_m, tmp1, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int M, int N1, int N2>
+template <int M>
 static hipError_t coarse_cols_m(int B, int nimg, float flare, const float2* tmp1,
                                 const float2* tw_m, float2* Ib, hipStream_t s) {
-  constexpr int TS = team_size<N1, N2>();
-  constexpr int TEAMS = TS == 1 ? 64 : 256 / TS;
+  constexpr int TEAMS = 256 / CoarseFft<M>::N2;
   dim3 grid((2 * B + 1 + TEAMS - 1) / TEAMS, nimg);
-  HUV_LAUNCH((k_coarse_cols<M, N1, N2, TEAMS>), grid, dim3(TEAMS * TS), 0, s, tmp1, B, flare,
-             tw_m, Ib);
+  HUV_LAUNCH((k_coarse_cols<M, TEAMS>), grid, dim3(256), 0, s, tmp1, B, flare, tw_m, Ib);
 }
 
 hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2* tmp1,
                               const float2* tw_m, float2* Ib, hipStream_t s) {
   switch (m) {
-    case 32: return coarse_cols_m<32, 1, 32>(B, nimg, flare, tmp1, tw_m, Ib, s);
-    case 64: return coarse_cols_m<64, 1, 64>(B, nimg, flare, tmp1, tw_m, Ib, s);
-    case 128: return coarse_cols_m<128, 2, 64>(B, nimg, flare, tmp1, tw_m, Ib, s);
-    case 192: return coarse_cols_m<192, 3, 64>(B, nimg, flare, tmp1, tw_m, Ib, s);
-    case 256: return coarse_cols_m<256, 4, 64>(B, nimg, flare, tmp1, tw_m, Ib, s);
+    case 32: return coarse_cols_m<32>(B, nimg, flare, tmp1, tw_m, Ib, s);
+    case 64: return coarse_cols_m<64>(B, nimg, flare, tmp1, tw_m, Ib, s);
+    case 128: return coarse_cols_m<128>(B, nimg, flare, tmp1, tw_m, Ib, s);
+    case 192: return coarse_cols_m<192>(B, nimg, flare, tmp1, tw_m, Ib, s);
+    case 256: return coarse_cols_m<256>(B, nimg, flare, tmp1, tw_m, Ib, s);
     default: return hipErrorInvalidValue;
   }
 }
