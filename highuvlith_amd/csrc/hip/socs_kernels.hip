@@ -142,6 +142,7 @@ struct CoarseCfg {
 };
 
 constexpr int kXcds = 8;
+constexpr int XNB = 6;  // band columns per load block of the x pass
 
 // e^{+2 pi i e / M} from the forward W_M table.
 __device__ __forceinline__ c32 tw_pos(const float2* __restrict__ tw_m, int e, int M) {
@@ -194,69 +195,101 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
 #pragma unroll
     for (int t = 0; t < NT; ++t) wf[t] = tw_pos(tw_m, (t + TMIN) * u * F, M);
   }
-  float acc[F];
-#pragma unroll
-  for (int i = 0; i < F; ++i) acc[i] = 0.f;
   __syncthreads();
-  for (int s = 0; s <= steps; ++s) {
-    if (iy >= 0) {
+  // The two groups run separate loops (wave-uniform split) with the same
+  // number of barriers, so their register live ranges do not overlap.
+  if (iy >= 0) {
+    const int vo = iy * (int)sizeof(float2);
+    for (int s = 0; s <= steps; ++s) {
       if (s < steps && iy < W) {
         // Buffer loads: one VGPR offset per lane, the column offset in an SGPR.
         const __amdgpu_buffer_rsrc_t kr = box_rsrc(kern + (size_t)s * W * W, W * W);
-        const int vo = iy * (int)sizeof(float2);
         c32 c[F];
 #pragma unroll
-        for (int h = 0; h < F; ++h) {
-          c32 v{0.f, 0.f};
+        for (int h = 0; h < F; ++h) c[h] = c32{0.f, 0.f};
+        // Columns kx = -BB..BB in blocks of XNB; the loads of block b + 1 are
+        // issued before block b is consumed.
+        constexpr int NBLK = (W + XNB - 1) / XNB;
+        c32 ka[2][XNB], ma[2][XNB];
 #pragma unroll
-          for (int t = TMIN; t <= TMAX; ++t) {
-            const int kx = h + F * t;
-            if (kx >= -BB && kx <= BB) {
-              const int so = (kx + BB) * W * (int)sizeof(float2);
-              const c32 a = load_c32(kr, vo, so), b = load_c32(mr, vo, so);
-              const c32 p = cmul(a, b);
-              v = cadd(v, t == 0 ? p : cmul(p, wf[t - TMIN]));
+        for (int i = 0; i < XNB; ++i) {
+          const int so = i * W * (int)sizeof(float2);
+          ka[0][i] = load_c32(kr, vo, so);
+          ma[0][i] = load_c32(mr, vo, so);
+        }
+#pragma unroll
+        for (int blk = 0; blk < NBLK; ++blk) {
+          __builtin_amdgcn_sched_barrier(0);
+          if (blk + 1 < NBLK) {
+#pragma unroll
+            for (int i = 0; i < XNB; ++i) {
+              const int col = (blk + 1) * XNB + i;
+              if (col < W) {
+                const int so = col * W * (int)sizeof(float2);
+                ka[(blk + 1) & 1][i] = load_c32(kr, vo, so);
+                ma[(blk + 1) & 1][i] = load_c32(mr, vo, so);
+              }
             }
           }
-          c[h] = cmul(v, twx[h]);
+#pragma unroll
+          for (int i = 0; i < XNB; ++i) {
+            const int col = blk * XNB + i;
+            if (col < W) {
+              const int kx = col - BB;
+              const int h = ((kx % F) + F) % F;
+              const int t = (kx - h) / F;
+              const c32 p = cmul(ka[blk & 1][i], ma[blk & 1][i]);
+              c[h] = cadd(c[h], t == 0 ? p : cmul(p, wf[t - TMIN]));
+            }
+          }
         }
+#pragma unroll
+        for (int h = 0; h < F; ++h) c[h] = cmul(c[h], twx[h]);
         fft_reg<+1, F>(c);
         c32* Tb = T + (s & 1) * W * TP;
 #pragma unroll
         for (int jj = 0; jj < F; ++jj) Tb[iy * TP + jj] = c[jj];
       }
-    } else if (is_y && s >= 1) {
-      const int sp = s - 1;
-      const c32* Tb = T + (sp & 1) * W * TP;
-      const c32* ty = twy + r * F;
-      c32 d[F];
+      __syncthreads();
+    }
+  } else {
+    float acc[F];
 #pragma unroll
-      for (int h = 0; h < F; ++h) {
-        c32 v{0.f, 0.f};
+    for (int i = 0; i < F; ++i) acc[i] = 0.f;
+    const c32* ty = twy + r * F;
+    for (int s = 0; s <= steps; ++s) {
+      if (is_y && s >= 1) {
+        const int sp = s - 1;
+        const c32* Tb = T + (sp & 1) * W * TP;
+        c32 d[F];
 #pragma unroll
-        for (int t = TMIN; t <= TMAX; ++t) {
-          const int ky = h + F * t;
-          if (ky >= -BB && ky <= BB) {
-            const c32 e = Tb[(ky + BB) * TP + j];
-            v = cadd(v, t == 0 ? e : cmul(e, wf[t - TMIN]));
+        for (int h = 0; h < F; ++h) {
+          c32 v{0.f, 0.f};
+#pragma unroll
+          for (int t = TMIN; t <= TMAX; ++t) {
+            const int ky = h + F * t;
+            if (ky >= -BB && ky <= BB) {
+              const c32 e = Tb[(ky + BB) * TP + j];
+              v = cadd(v, t == 0 ? e : cmul(e, wf[t - TMIN]));
+            }
+          }
+          d[h] = cmul(v, ty[h]);
+        }
+        fft_reg<+1, F>(d);
+        const float l = lam[sp];
+#pragma unroll
+        for (int i = 0; i < F; ++i) acc[i] += l * cnorm(d[i]);
+        if (sp % Kmax == Kmax - 1) {
+          float* out = coarse + ((size_t)clip * nplanes + sp / Kmax) * M * M;
+#pragma unroll
+          for (int i = 0; i < F; ++i) {
+            out[(size_t)(R * i + r) * M + Q * j + q] = acc[i];
+            acc[i] = 0.f;
           }
         }
-        d[h] = cmul(v, ty[h]);
       }
-      fft_reg<+1, F>(d);
-      const float l = lam[sp];
-#pragma unroll
-      for (int i = 0; i < F; ++i) acc[i] += l * cnorm(d[i]);
-      if (sp % Kmax == Kmax - 1) {
-        float* out = coarse + ((size_t)clip * nplanes + sp / Kmax) * M * M;
-#pragma unroll
-        for (int i = 0; i < F; ++i) {
-          out[(size_t)(R * i + r) * M + Q * j + q] = acc[i];
-          acc[i] = 0.f;
-        }
-      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
