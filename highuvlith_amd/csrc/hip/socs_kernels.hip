@@ -372,77 +372,118 @@ __global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_cols(
 }
 
 // ---------------------------------------------------------------------------
-// band_image: rows y = P j + p (P = n / 64) of the n x n image from the
-// Hermitian half band Ib[gy + G][gx] (gx in [0, G]).
-// Phase 1 (threads gx):  U[j][gx] = IFFT_64( fold_64( Ib[gy][gx] e^{2 pi i gy p / n} ) )
-// Phase 2: row pairs via C2R packing, four-step IFFT_n, coalesced row writes.
+// band_image: the n x n real image of a Hermitian half band Ib[gy + G][gx]
+// (gx in [0, G]), I[y][x] = sum_{g} Ib[g] e^{2 pi i (gx x + gy y) / n}.
+// One work-group per (image, row class p) computes the 32 rows y = P j + p
+// (P = n / 32):
+//   phase 1, thread gx:  U[j][gx] = IFFT_32(u)[j],
+//       u[h] = e^{2 pi i h p / n} sum_t Ib[h + 32 t][gx] e^{2 pi i t p / P}
+//   phase 2, task (row pair, column class c), C = n / 32 classes, z = a + i b:
+//       X[C i + c] = IFFT_32(d)[i],
+//       d[h] = e^{2 pi i h c / n} sum_t z[h + 32 t] e^{2 pi i t c / C}
+//   with z[g] = Ua[g] + i Ub[g] (g >= 0), conj Ua[-g] + i conj Ub[-g] (g < 0);
+//   row a takes Re X, row b Im X.  Writes are coalesced across classes.
+// Folding ranges t in [TMIN, TMAX] cover |g| <= G (G <= 32 (TMAX + 1) - 1).
+// Work-groups of one image are dealt to one XCD so the band stays in its L2.
 // ---------------------------------------------------------------------------
-template <int N1, int N2, int TEAMS>
+template <int TMIN, int TMAX>
 __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ Ib, int n, int G,
-                                                    const float2* __restrict__ tw_n,
+                                                    int nimg, const float2* __restrict__ tw_n,
                                                     float* __restrict__ out) {
-  constexpr int N = N1 * N2;
-  constexpr int JR = 64;  // rows per work-group
+  constexpr int F = 32;
   extern __shared__ c32 smem[];
-  const int nk = 2 * G + 1;
-  const int UP = G + 2;  // padded U row
-  c32* U = smem;                                  // JR x UP
-  c32* wt = smem + JR * UP;                       // nk twiddles
-  c32* scratch = wt + ((nk + 1) & ~1);            // TEAMS x (N1+1) N2
-  const int p = blockIdx.x, img = blockIdx.y;
-  const int P = N / JR;
+  const int UP = G + 1;
+  c32* U = smem;           // F x UP
+  c32* twn = U + F * UP;   // e^{+2 pi i k / n}, k < n
+  const int P = n / F, C = n / F;
+  const int L = blockIdx.x;
+  const int slot = L / kXcds;
+  const int p = slot % P;
+  const int img = (slot / P) * kXcds + L % kXcds;
+  if (img >= nimg) return;  // whole work-group, before any barrier
   const int tid = threadIdx.x;
-  for (int i = tid; i < nk; i += 256) {
-    const int gy = i - G;
-    const int e = (int)(((long long)gy * p % N + N) % N);
-    wt[i] = tw_lookup<+1>(tw_n, e);
+  for (int i = tid; i < n; i += 256) twn[i] = tw_lookup<+1>(tw_n, i);
+  __syncthreads();
+  const int nk = 2 * G + 1;
+  // ---- phase 1: two threads per column gx, each folding half of the t range ----
+  constexpr int TMID = TMIN + (TMAX - TMIN + 1) / 2;
+  const int half = tid / (G + 1), gx = tid % (G + 1);
+  c32 u[F];
+  if (half < 2) {
+    const __amdgpu_buffer_rsrc_t br = box_rsrc(Ib + (size_t)img * nk * (G + 1), nk * (G + 1));
+    int vo = gx * (int)sizeof(float2);
+#pragma unroll
+    for (int h = 0; h < F; ++h) u[h] = c32{0.f, 0.f};
+#pragma unroll
+    for (int t = TMIN; t <= TMAX; ++t) {
+      if ((t < TMID) != (half == 0)) continue;
+      // Opaque to the optimiser: keeps each band row block's loads after the
+      // previous block instead of hoisting every load to the top.
+      asm volatile("" : "+v"(vo));
+      const c32 w = twn[(((F * t * p) % n) + n) % n];  // e^{2 pi i t p / P}
+      c32 e[F];
+#pragma unroll
+      for (int h = 0; h < F; ++h) {
+        const int gy = h + F * t;
+        e[h] = c32{0.f, 0.f};
+        if (gy >= -G && gy <= G) e[h] = load_c32(br, vo, (gy + G) * (G + 1) * (int)sizeof(float2));
+      }
+#pragma unroll
+      for (int h = 0; h < F; ++h) u[h] = cadd(u[h], t == 0 ? e[h] : cmul(e[h], w));
+    }
+#pragma unroll
+    for (int h = 0; h < F; ++h) u[h] = cmul(u[h], twn[h * p]);
+    fft_reg<+1, F>(u);
+    if (half == 0) {
+#pragma unroll
+      for (int j = 0; j < F; ++j) U[j * UP + gx] = u[j];
+    }
   }
   __syncthreads();
-  const float2* ib = Ib + (size_t)img * nk * (G + 1);
-  for (int gx = tid; gx <= G; gx += 256) {
-    c32 d[JR];
+  if (half == 1) {
 #pragma unroll
-    for (int h = 0; h < JR; ++h) {
-      d[h] = c32{0.f, 0.f};
+    for (int j = 0; j < F; ++j) U[j * UP + gx] = cadd(U[j * UP + gx], u[j]);
+  }
+  __syncthreads();
+  // ---- phase 2 ----
+  float* img_out = out + (size_t)img * n * n;
+  for (int task = tid; task < (F / 2) * C; task += 256) {
+    const int pr = task / C, c = task % C;
+    const c32* ua = U + (2 * pr) * UP;
+    const c32* ub = ua + UP;
+    c32 d[F];
 #pragma unroll
-      for (int tt = -2; tt <= 1; ++tt) {
-        const int gy = h + JR * tt;
-        if (gy >= -G && gy <= G) {
-          const float2 v = ib[(size_t)(gy + G) * (G + 1) + gx];
-          d[h] = cadd(d[h], cmul(c32{v.x, v.y}, wt[gy + G]));
+    for (int h = 0; h < F; ++h) d[h] = c32{0.f, 0.f};
+#pragma unroll
+    for (int t = TMIN; t <= TMAX; ++t) {
+      __builtin_amdgcn_sched_barrier(0);
+      const c32 w = twn[(((F * t * c) % n) + n) % n];  // e^{2 pi i t c / C}
+#pragma unroll
+      for (int h = 0; h < F; ++h) {
+        const int g = h + F * t;
+        if (g >= -G && g <= G) {
+          c32 z;
+          if (g >= 0) {
+            const c32 a = ua[g], b = ub[g];
+            z = c32{a.re - b.im, a.im + b.re};
+          } else {
+            const c32 a = ua[-g], b = ub[-g];
+            z = c32{a.re + b.im, -a.im + b.re};
+          }
+          d[h] = cadd(d[h], t == 0 ? z : cmul(z, w));
         }
       }
     }
-    fft_reg<+1, JR>(d);
 #pragma unroll
-    for (int jj = 0; jj < JR; ++jj) U[jj * UP + gx] = d[jj];
-  }
-  __syncthreads();
-  constexpr int TS = team_size<N1, N2>();
-  const int team = tid / TS, t = tid % TS;
-  float* img_out = out + (size_t)img * N * N;
-  for (int base = 0; base < JR; base += 2 * TEAMS) {
-    const int ja = base + 2 * team;
-    const c32* ua = U + ja * UP;
-    const c32* ub = ua + UP;
-    float* ra = img_out + (size_t)(P * ja + p) * N;
-    float* rb = img_out + (size_t)(P * (ja + 1) + p) * N;
-    fft_four_step<+1, N1, N2>(
-        t, scratch + team * (N1 + 1) * N2, tw_n,
-        [&](int i) {
-          const int g = i < N / 2 ? i : i - N;
-          if (g > G || g < -G) return c32{0.f, 0.f};
-          if (g >= 0) {
-            const c32 a = ua[g], b = ub[g];
-            return c32{a.re - b.im, a.im + b.re};  // a + i b
-          }
-          const c32 a = ua[-g], b = ub[-g];
-          return c32{a.re + b.im, -a.im + b.re};  // conj(a) + i conj(b)
-        },
-        [&](int k, c32 v) {
-          ra[k] = v.re;
-          rb[k] = v.im;
-        });
+    for (int h = 0; h < F; ++h) d[h] = cmul(d[h], twn[h * c]);
+    fft_reg<+1, F>(d);
+    float* ra = img_out + (size_t)(P * (2 * pr) + p) * n + c;
+    float* rb = ra + (size_t)P * n;
+#pragma unroll
+    for (int i = 0; i < F; ++i) {
+      ra[(size_t)C * i] = d[i].re;
+      rb[(size_t)C * i] = d[i].im;
+    }
   }
 }
 
@@ -577,29 +618,26 @@ hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2*
   }
 }
 
-template <int N1, int N2>
-static hipError_t band_image_n(int G, int nimg, const float2* Ib, const float2* tw_n, float* out,
-                               hipStream_t s) {
-  constexpr int N = N1 * N2;
-  constexpr int TS = team_size<N1, N2>();
-  constexpr int TEAMS = 256 / TS;
-  const int nk = 2 * G + 1;
-  const size_t words = (size_t)64 * (G + 2) + ((nk + 1) & ~1) + (size_t)TEAMS * (N1 + 1) * N2;
-  const size_t shmem = words * sizeof(c32);
-  auto kern = k_band_image<N1, N2, TEAMS>;
+template <int TMIN, int TMAX>
+static hipError_t band_image_t(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
+                               float* out, hipStream_t s) {
+  const size_t shmem = ((size_t)32 * (G + 1) + n) * sizeof(c32);
+  auto kern = k_band_image<TMIN, TMAX>;
   hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  HUV_LAUNCH(kern, dim3(N / 64, nimg), dim3(256), shmem, s, Ib, N, G, tw_n, out);
+  const int groups = (nimg + kXcds - 1) / kXcds;
+  HUV_LAUNCH(kern, dim3(groups * kXcds * (n / 32)), dim3(256), shmem, s, Ib, n, G, nimg, tw_n,
+             out);
 }
 
 hipError_t launch_band_image(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
                              float* out, hipStream_t s) {
-  switch (n) {
-    case 256: return band_image_n<16, 16>(G, nimg, Ib, tw_n, out, s);
-    case 512: return band_image_n<16, 32>(G, nimg, Ib, tw_n, out, s);
-    case 1024: return band_image_n<32, 32>(G, nimg, Ib, tw_n, out, s);
-    case 2048: return band_image_n<32, 64>(G, nimg, Ib, tw_n, out, s);
-    default: return hipErrorInvalidValue;
-  }
+  if (n < 256 || n > 2048 || (n & (n - 1)) != 0 || G < 0 || G > 255 || 2 * G + 1 > n)
+    return hipErrorInvalidValue;
+  if (G <= 31) return band_image_t<-1, 0>(n, G, nimg, Ib, tw_n, out, s);
+  if (G <= 63) return band_image_t<-2, 1>(n, G, nimg, Ib, tw_n, out, s);
+  if (G <= 95) return band_image_t<-3, 2>(n, G, nimg, Ib, tw_n, out, s);
+  if (G <= 127) return band_image_t<-4, 3>(n, G, nimg, Ib, tw_n, out, s);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace huvk
