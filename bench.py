@@ -30,14 +30,22 @@ def fft_flops(n):
     return 5.0 * n * max(1.0, __import__("math").log2(n))
 
 
-def fields_flops(nimg, K, B, m, Q):
-    """FP32 flops of socs_fields per launch: FFTs actually executed (5 N log2 N),
-    band folds (8 per complex multiply-add, 6 per twiddle) and 3 per |E|^2."""
-    R = Q
-    FX, FY = m // Q, m // R
-    W = 2 * B + 1
-    x = Q * W * (fft_flops(FX) + 8.0 * W)
-    y = m * R * (fft_flops(FY) + 2.0 * W + 6.0 * FY) + 3.0 * m * m
+def fields_flops(nimg, K, m):
+    """FP32 flops of one socs_fields launch as the kernel executes them.
+
+    Coarse grid m = 32 Q, band box W = 2 ((m - 1) // 4) + 1, 32-point
+    transforms (5 N log2 N each).  Per image and kernel:
+      x pass, Q W tasks:  W products (6) + W accumulates (2) + (W - 32) fold
+                          twiddles (6) + 32 class twists (6) + FFT_32
+      y pass, 32 Q^2 tasks: W accumulates (2) + (W - 32) fold twiddles (6)
+                          + 32 residue twists (6) + FFT_32 + 32 x lambda|E|^2 (4)
+    """
+    F = 32
+    Q = m // F
+    W = 2 * ((m - 1) // 4) + 1
+    fold = 6.0 * max(W - F, 0)
+    x = Q * W * (8.0 * W + fold + 6.0 * F + fft_flops(F))
+    y = F * Q * Q * (2.0 * W + fold + 6.0 * F + fft_flops(F) + 4.0 * F)
     return nimg * K * (x + y)
 
 
@@ -50,6 +58,9 @@ def main():
     ap.add_argument("--cpu-clips", type=int, default=2,
                     help="clips in the CPU-baseline sample (x 5 planes)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB before the steps "
+                         "(a known-size reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -72,7 +83,6 @@ def main():
     Ks = [s.num_kernels for s in sets]
     S = [len(s.support) for s in sets]
     B, m = planes[0].band, planes[0].coarse
-    Q = {32: 1, 64: 1, 128: 2, 192: 3, 256: 4}[m]
     P = len(planes)
     nclips = args.clips
     masks = W.manhattan_clips(nclips, n, seed=2026, clear_fraction=0.4)
@@ -82,6 +92,15 @@ def main():
 
     def step():
         H.engine.socs_images_device(ctx, masks, planes, out)
+
+    if args.copy_calibration_mib > 0:
+        nel = args.copy_calibration_mib * (1 << 20) // 4
+        src_c = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst_c = torch.empty_like(src_c)
+        torch.cuda.synchronize()
+        dst_c.copy_(src_c)
+        torch.cuda.synchronize()
+        del src_c, dst_c
 
     for _ in range(args.warmup):
         step()
@@ -104,7 +123,7 @@ def main():
     achieved = algo_bytes / dt / 1e9
     f_ms, f_calls = stages["socs_fields"]
     f_ms_per = f_ms / max(f_calls, 1)
-    flops = fields_flops(nimg, sum(Ks) / P, B, m, Q)
+    flops = fields_flops(nimg, max(Ks), m)
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items() if v[1] > 0}
     pipeline_ms = sum(stage_ms.values())
 

@@ -17,7 +17,7 @@
 
 namespace huvk {
 
-struct c32 {
+struct alignas(8) c32 {
   float re, im;
 };
 

@@ -256,10 +256,13 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
     float acc[F];
 #pragma unroll
     for (int i = 0; i < F; ++i) acc[i] = 0.f;
-    const c32* ty = twy + r * F;
     for (int s = 0; s <= steps; ++s) {
       if (is_y && s >= 1) {
         const int sp = s - 1;
+        // Re-read the twiddles every step rather than holding them in registers.
+        int tyo = r * F;
+        asm volatile("" : "+v"(tyo));
+        const c32* ty = twy + tyo;
         const c32* Tb = T + (sp & 1) * W * TP;
         c32 d[F];
 #pragma unroll
