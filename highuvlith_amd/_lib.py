@@ -67,7 +67,22 @@ class huv_kernels_info(C.Structure):
 
 
 class huv_socs_opts(C.Structure):
-    _fields_ = [("pixel_transfer", C.c_int)]
+    _fields_ = [("pixel_transfer", C.c_int), ("graph", C.c_int)]
+
+
+class huv_socs_tuning(C.Structure):
+    _fields_ = [("m", C.c_int), ("n1", C.c_int), ("xnb", C.c_int)]
+
+
+class huv_clip_group(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("nclips", C.c_int),
+        ("masks", C.c_void_p),
+        ("nplanes", C.c_int),
+        ("planes", C.POINTER(C.c_void_p)),
+        ("out", C.c_void_p),
+    ]
 
 
 class huv_ilt_config(C.Structure):
@@ -124,6 +139,13 @@ SIGNATURES = {
     "huv_socs_images_host": (C.c_int, [P, C.c_int, C.c_int, PF, C.c_int, C.POINTER(P), PF,
                                        C.POINTER(huv_socs_opts)]),
     "huv_band_spectrum": (C.c_int, [P, C.c_int, C.c_int, C.c_int, PF, PF, C.c_int]),
+    "huv_socs_images_batch": (C.c_int, [P, C.c_int, C.POINTER(huv_clip_group),
+                                        C.POINTER(huv_socs_opts)]),
+    "huv_ctx_set_autotune": (C.c_int, [P, C.c_int]),
+    "huv_ctx_tuning_get": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(huv_socs_tuning), C.POINTER(C.c_int)]),
+    "huv_ctx_tuning_set": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(huv_socs_tuning)]),
     "huv_ilt_create": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(P), PD, PD,
                                  C.POINTER(huv_ilt_config), C.POINTER(P)]),
     "huv_ilt_destroy": (None, [P]),

@@ -46,6 +46,7 @@ void* Workspace::get(int slot, size_t bytes, hipError_t& err) {
     if (err != hipSuccess) return nullptr;
   }
   err = hipMalloc(&ptr[slot], bytes);
+  ++generation;
   if (err != hipSuccess) {
     ptr[slot] = nullptr;
     return nullptr;
@@ -152,11 +153,30 @@ Ctx::~Ctx() {
     hipFree(kv.second.kern);
     hipFree(kv.second.lam);
   }
+  for (auto& kv : graphs) drop_graph(kv.second);
   if (own_stream) hipStreamDestroy(own_stream);
+}
+
+void Ctx::drop_graph(GraphEntry& g) {
+  if (g.exec) hipGraphExecDestroy(g.exec);
+  if (g.graph) hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
 }
 
 void Ctx::forget_kset(uint64_t id) {
   bool busy = false;
+  for (auto it = graphs.begin(); it != graphs.end();) {
+    if (std::find(it->second.ksets.begin(), it->second.ksets.end(), id) != it->second.ksets.end()) {
+      if (!busy) hipStreamSynchronize(stream());
+      if (!busy && own_stream) hipStreamSynchronize(own_stream);
+      busy = true;
+      drop_graph(it->second);
+      it = graphs.erase(it);
+    } else {
+      ++it;
+    }
+  }
   for (auto it = plane_tables.begin(); it != plane_tables.end();) {
     if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
       if (!busy) hipStreamSynchronize(stream());
@@ -181,7 +201,7 @@ void Ctx::forget_kset(uint64_t id) {
 }
 
 huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* masks, float2* Mq,
-                      bool pixel_transfer) {
+                      bool pixel_transfer, int n1) {
   hipError_t e;
   const float2* tw_n = c->twiddles(n, e);
   if (!tw_n) return hip_fail(e, "twiddle table");
@@ -192,11 +212,11 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* mas
   if (!tmp) return hip_fail(e, "workspace");
   hipEvent_t ev;
   c->stage_begin(ST_ROWS_BAND, &ev);
-  e = huvk::launch_rows_band(n, B, nclips, masks, tw_n, tmp, c->stream());
+  e = huvk::launch_rows_band(n, n1, B, nclips, masks, tw_n, tmp, c->stream());
   if (e != hipSuccess) return hip_fail(e, "rows_band launch");
   c->stage_end(ST_ROWS_BAND, ev);
   c->stage_begin(ST_COLS_BAND, &ev);
-  e = huvk::launch_cols_band(n, B, BB, nclips, tmp, tw_n, st, Mq, c->stream());
+  e = huvk::launch_cols_band(n, n1, B, BB, nclips, tmp, tw_n, st, Mq, c->stream());
   if (e != hipSuccess) return hip_fail(e, "cols_band launch");
   c->stage_end(ST_COLS_BAND, ev);
   return HUV_OK;
@@ -576,30 +596,41 @@ huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int**
 }
 
 // Kernel stacks of a plane list packed into one zero-padded buffer
-// [nplanes][Kmax][W][W] with eigenvalues [nplanes][Kmax] (0 for padding).
-huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, const float2** kern,
+// [nplanes][Kmax][W][W] (W = 2 BB + 1, BB >= every set's band) with
+// eigenvalues / n^4 in [nplanes][Kmax] (0 for padding), built from the
+// host copies of the sets.
+huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int BB, const float2** kern,
                       const float** lam, int* Kmax) {
   std::vector<uint64_t> key;
   for (int p = 0; p < nplanes; ++p) key.push_back(planes[p]->id);
+  key.push_back((uint64_t)BB);
   auto it = c->plane_packs.find(key);
   if (it == c->plane_packs.end()) {
     Ctx::PlanePack pk;
-    for (int p = 0; p < nplanes; ++p) pk.Kmax = std::max(pk.Kmax, planes[p]->K);
-    const int W = 2 * planes[0]->BB + 1;
+    for (int p = 0; p < nplanes; ++p) {
+      pk.Kmax = std::max(pk.Kmax, planes[p]->K);
+      if (planes[p]->B > BB) return fail(HUV_INTERNAL_ERROR, "kernel band exceeds the packed box");
+    }
+    const int W = 2 * BB + 1;
     const size_t box = (size_t)W * W;
     std::vector<float2> hk((size_t)nplanes * pk.Kmax * box, make_float2(0.f, 0.f));
     std::vector<float> hl((size_t)nplanes * pk.Kmax, 0.f);
-    hipError_t e = hipSuccess;
-    for (int p = 0; p < nplanes && e == hipSuccess; ++p) {
+    for (int p = 0; p < nplanes; ++p) {
       const huv_kset* ks = planes[p];
-      e = hipMemcpy(hk.data() + (size_t)p * pk.Kmax * box, ks->kern, (size_t)ks->K * box * sizeof(float2),
-                    hipMemcpyDeviceToHost);
-      if (e == hipSuccess)
-        e = hipMemcpy(hl.data() + (size_t)p * pk.Kmax, ks->lam, (size_t)ks->K * sizeof(float),
-                      hipMemcpyDeviceToHost);
+      const int64_t n = ks->n;
+      const double n4 = std::pow((double)n, 4.0);
+      for (int k = 0; k < ks->K; ++k) {
+        hl[(size_t)p * pk.Kmax + k] = (float)(ks->lam_host[k] / n4);
+        float2* dst = hk.data() + ((size_t)p * pk.Kmax + k) * box;
+        for (int64_t t = 0; t < ks->S; ++t) {
+          const int64_t i = ks->support[t] / n, j = ks->support[t] % n;
+          const int64_t ky = i < n / 2 ? i : i - n, kx = j < n / 2 ? j : j - n;
+          dst[(kx + BB) * W + (ky + BB)] = make_float2((float)ks->values[2 * (k * ks->S + t)],
+                                                       (float)ks->values[2 * (k * ks->S + t) + 1]);
+        }
+      }
     }
-    if (e != hipSuccess) return hip_fail(e, "plane pack readback");
-    e = hipMalloc(&pk.kern, hk.size() * sizeof(float2));
+    hipError_t e = hipMalloc(&pk.kern, hk.size() * sizeof(float2));
     if (e == hipSuccess) e = hipMalloc(&pk.lam, hl.size() * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(pk.kern, hk.data(), hk.size() * sizeof(float2), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(pk.lam, hl.data(), hl.size() * sizeof(float), hipMemcpyHostToDevice);
@@ -619,8 +650,13 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, const float2
 // Forward images of real masks: band spectrum -> fused fields -> epilogue.
 // Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
-                        huv_kset* const* planes, float* out, bool pt, float2** Mq_out) {
-  const int B = planes[0]->B, m = planes[0]->m, BB = planes[0]->BB;
+                        huv_kset* const* planes, float* out, bool pt, float2** Mq_out,
+                        const huv_socs_tuning* tune) {
+  const int B = planes[0]->B;
+  const int m = tune && tune->m ? tune->m : planes[0]->m;
+  const int BB = huvk::coarse_band(m);
+  const int n1 = tune ? tune->n1 : 0, xnb = tune ? tune->xnb : 0;
+  if (B > BB || m > n) return fail(HUV_INVALID_PARAMETER, "coarse grid too small for the kernel band");
   const double flare = planes[0]->flare;
   hipError_t e;
   const int W = 2 * BB + 1;
@@ -628,12 +664,12 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   float2* Mq = (float2*)c->ws.get(Workspace::BAND, (size_t)nclips * W * W * sizeof(float2), e);
   if (!Mq) return hip_fail(e, "workspace");
   if (Mq_out) *Mq_out = Mq;
-  huv_status st = band_stage(c, nclips, n, B, BB, masks, Mq, pt);
+  huv_status st = band_stage(c, nclips, n, B, BB, masks, Mq, pt, n1);
   if (st != HUV_OK) return st;
   const float2* kern;
   const float* lam;
   int Kmax;
-  st = plane_pack(c, nplanes, planes, &kern, &lam, &Kmax);
+  st = plane_pack(c, nplanes, planes, BB, &kern, &lam, &Kmax);
   if (st != HUV_OK) return st;
   float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)nimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");
@@ -641,7 +677,8 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   if (!tw_m) return hip_fail(e, "twiddle table");
   hipEvent_t ev;
   c->stage_begin(ST_FIELDS, &ev);
-  e = huvk::launch_socs_fields(m, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, c->stream());
+  e = huvk::launch_socs_fields(m, xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse,
+                               c->stream());
   if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
   c->stage_end(ST_FIELDS, ev);
   const float2* tw_n = c->twiddles(n, e);
@@ -670,6 +707,212 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
 
 extern "C" {
 
+}  // extern "C"
+
+namespace huvi {
+
+std::vector<int> tuning_key(int n, const huv_kset* const* planes, int nplanes) {
+  int kmax = 0;
+  for (int p = 0; p < nplanes; ++p) kmax = std::max(kmax, planes[p]->K);
+  return {n, planes[0]->B, nplanes, kmax};
+}
+
+// Times every launch shape on this call's inputs (coordinate descent: coarse
+// grid, then band-FFT factor, then x-pass block) and records the fastest.
+// The last run leaves a valid result in `out`.
+huv_status autotune_socs(Ctx* c, int nclips, int n, const float* masks, int nplanes,
+                         huv_kset* const* planes, float* out, bool pt, huv_socs_tuning* best) {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  if (e != hipSuccess) return hip_fail(e, "autotune events");
+  const bool prof = c->profiling;
+  c->profiling = false;
+  huv_status st = HUV_OK;
+  auto time_one = [&](const huv_socs_tuning& t, double& ms) -> huv_status {
+    double bestms = 1e30;
+    for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and builds tables
+      hipEventRecord(a, c->stream());
+      huv_status s2 = socs_forward(c, nclips, n, masks, nplanes, planes, out, pt, nullptr, &t);
+      if (s2 != HUV_OK) return s2;
+      hipEventRecord(b, c->stream());
+      hipError_t e2 = hipEventSynchronize(b);
+      if (e2 != hipSuccess) return hip_fail(e2, "autotune");
+      float f = 0.f;
+      hipEventElapsedTime(&f, a, b);
+      if (rep > 0) bestms = std::min(bestms, (double)f);
+    }
+    ms = bestms;
+    return HUV_OK;
+  };
+  huv_socs_tuning cur{planes[0]->m, 0, 0};
+  double cur_ms = 1e30;
+  const int B = planes[0]->B;
+  for (int m : {32, 64, 128, 192, 256}) {
+    if (m < 4 * B + 1 || m > n) continue;
+    huv_socs_tuning t{m, 0, 0};
+    double ms;
+    st = time_one(t, ms);
+    if (st != HUV_OK) break;
+    if (ms < cur_ms) { cur_ms = ms; cur = t; }
+  }
+  int n1s[4];
+  const int nn1 = huvk::band_fft_options(n, n1s);
+  for (int i = 0; i < nn1 && st == HUV_OK; ++i) {
+    huv_socs_tuning t = cur;
+    t.n1 = n1s[i];
+    double ms;
+    st = time_one(t, ms);
+    if (st == HUV_OK && ms < cur_ms) { cur_ms = ms; cur = t; }
+  }
+  for (int xnb : {4, 6, 8}) {
+    if (st != HUV_OK) break;
+    huv_socs_tuning t = cur;
+    t.xnb = xnb;
+    double ms;
+    st = time_one(t, ms);
+    if (st == HUV_OK && ms < cur_ms) { cur_ms = ms; cur = t; }
+  }
+  // leave the chosen shape's result in `out`
+  if (st == HUV_OK) st = socs_forward(c, nclips, n, masks, nplanes, planes, out, pt, nullptr, &cur);
+  c->profiling = prof;
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (st == HUV_OK) *best = cur;
+  return st;
+}
+
+// Launch shape for a call: the table entry, a fresh autotune, or defaults.
+huv_status socs_shape(Ctx* c, int nclips, int n, const float* masks, int nplanes,
+                      huv_kset* const* planes, float* out, bool pt, huv_socs_tuning* t,
+                      bool* computed) {
+  *computed = false;
+  const auto key = tuning_key(n, planes, nplanes);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) {
+    *t = it->second;
+    return HUV_OK;
+  }
+  *t = huv_socs_tuning{0, 0, 0};
+  if (!c->autotune) return HUV_OK;
+  huv_status st = autotune_socs(c, nclips, n, masks, nplanes, planes, out, pt, t);
+  if (st != HUV_OK) return st;
+  c->tuned[key] = *t;
+  *computed = true;
+  return HUV_OK;
+}
+
+// Runs body() eagerly, or replays / records a graph of it keyed by `key`.
+template <class Body>
+huv_status graph_run(Ctx* c, const std::vector<uint64_t>& key, const std::vector<uint64_t>& ksets,
+                     Body body) {
+  auto it = c->graphs.find(key);
+  if (it != c->graphs.end() && it->second.generation == c->ws.generation) {
+    hipStream_t own = c->own_stream, act = c->active;
+    hipError_t e = hipSuccess;
+    if (act != own) {
+      hipEvent_t ev;
+      e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(ev, act);
+      if (e == hipSuccess) e = hipStreamWaitEvent(own, ev, 0);
+      hipEventDestroy(ev);
+    }
+    if (e == hipSuccess) e = hipGraphLaunch(it->second.exec, own);
+    if (e == hipSuccess && act != own) {
+      hipEvent_t ev;
+      e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(ev, own);
+      if (e == hipSuccess) e = hipStreamWaitEvent(act, ev, 0);
+      hipEventDestroy(ev);
+    }
+    return e == hipSuccess ? HUV_OK : hip_fail(e, "graph launch");
+  }
+  if (it != c->graphs.end()) {
+    hipStreamSynchronize(c->own_stream);
+    c->drop_graph(it->second);
+    c->graphs.erase(it);
+  }
+  huv_status st = body();  // this call's result, eagerly
+  if (st != HUV_OK) return st;
+  // Record the same launches for later calls (capture executes nothing).
+  const uint64_t gen = c->ws.generation;
+  hipStream_t saved = c->active;
+  const bool prof = c->profiling;
+  c->active = c->own_stream;
+  c->profiling = false;
+  hipError_t e = hipStreamBeginCapture(c->own_stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    c->active = saved;
+    c->profiling = prof;
+    return hip_fail(e, "graph capture");
+  }
+  st = body();
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(c->own_stream, &g);
+  c->active = saved;
+  c->profiling = prof;
+  if (st != HUV_OK || e != hipSuccess || c->ws.generation != gen) {
+    if (g) hipGraphDestroy(g);
+    if (st != HUV_OK) return st;
+    return e != hipSuccess ? hip_fail(e, "graph capture") : fail(HUV_INTERNAL_ERROR, "workspace grew during capture");
+  }
+  Ctx::GraphEntry ge;
+  ge.graph = g;
+  ge.generation = gen;
+  ge.ksets = ksets;
+  e = hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(g);
+    return hip_fail(e, "graph instantiate");
+  }
+  c->graphs[key] = ge;
+  return HUV_OK;
+}
+
+struct GroupPlan {
+  huv_clip_group g;
+  huv_socs_tuning t;
+};
+
+huv_status socs_groups(Ctx* c, int ngroups, const huv_clip_group* groups, const huv_socs_opts* opts,
+                       int tag) {
+  const bool pt = opts && opts->pixel_transfer;
+  std::vector<GroupPlan> plan;
+  std::vector<uint64_t> key{(uint64_t)tag, (uint64_t)pt, (uint64_t)ngroups};
+  std::vector<uint64_t> ksets;
+  for (int i = 0; i < ngroups; ++i) {
+    const huv_clip_group& g = groups[i];
+    if (g.nclips == 0) continue;
+    GroupPlan gp{g, {0, 0, 0}};
+    bool computed = false;
+    huv_status st = socs_shape(c, g.nclips, g.n, g.masks, g.nplanes, g.planes, g.out, pt, &gp.t,
+                               &computed);
+    if (st != HUV_OK) return st;
+    plan.push_back(gp);
+    key.insert(key.end(), {(uint64_t)g.n, (uint64_t)g.nclips, (uint64_t)(uintptr_t)g.masks,
+                           (uint64_t)(uintptr_t)g.out, (uint64_t)g.nplanes, (uint64_t)gp.t.m,
+                           (uint64_t)gp.t.n1, (uint64_t)gp.t.xnb});
+    for (int p = 0; p < g.nplanes; ++p) {
+      key.push_back(g.planes[p]->id);
+      ksets.push_back(g.planes[p]->id);
+    }
+  }
+  auto body = [&]() -> huv_status {
+    for (const auto& gp : plan) {
+      huv_status st = socs_forward(c, gp.g.nclips, gp.g.n, gp.g.masks, gp.g.nplanes, gp.g.planes,
+                                   gp.g.out, pt, nullptr, &gp.t);
+      if (st != HUV_OK) return st;
+    }
+    return HUV_OK;
+  };
+  if (opts && opts->graph) return graph_run(c, key, ksets, body);
+  return body();
+}
+
+}  // namespace huvi
+
+extern "C" {
+
 huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, int nplanes,
                            huv_kset* const* planes, float* out, const huv_socs_opts* opts) {
   if (!c || !masks || !planes || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
@@ -679,7 +922,62 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
   if (nclips == 0) return HUV_OK;
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  return socs_forward(c, nclips, n, masks, nplanes, planes, out, opts && opts->pixel_transfer, nullptr);
+  huv_clip_group g{n, nclips, masks, nplanes, planes, out};
+  return socs_groups(c, 1, &g, opts, 1);
+}
+
+huv_status huv_socs_images_batch(huv_ctx* c, int ngroups, const huv_clip_group* groups,
+                                 const huv_socs_opts* opts) {
+  if (!c || (ngroups > 0 && !groups)) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (ngroups < 0) return fail(HUV_INVALID_PARAMETER, "need ngroups >= 0");
+  for (int i = 0; i < ngroups; ++i) {
+    const huv_clip_group& g = groups[i];
+    if (g.nclips < 0 || g.nplanes < 1) return fail(HUV_INVALID_PARAMETER, "group: need nclips >= 0 and nplanes >= 1");
+    if (g.nclips > 0 && (!g.masks || !g.out || !g.planes)) return fail(HUV_INVALID_PARAMETER, "group: null buffer");
+    if (g.nclips == 0) continue;
+    huv_status st = validate_planes(c, g.n, g.nplanes, g.planes);
+    if (st != HUV_OK) return st;
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return socs_groups(c, ngroups, groups, opts, 2);
+}
+
+huv_status huv_ctx_set_autotune(huv_ctx* c, int on) {
+  if (!c) return fail(HUV_INVALID_PARAMETER, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->autotune = on != 0;
+  return HUV_OK;
+}
+
+huv_status huv_ctx_tuning_get(huv_ctx* c, int n, int band, int nplanes, int kmax,
+                              huv_socs_tuning* out, int* found) {
+  if (!c || !out || !found) return fail(HUV_INVALID_PARAMETER, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->tuned.find({n, band, nplanes, kmax});
+  *found = it != c->tuned.end();
+  *out = *found ? it->second : huv_socs_tuning{0, 0, 0};
+  return HUV_OK;
+}
+
+huv_status huv_ctx_tuning_set(huv_ctx* c, int n, int band, int nplanes, int kmax,
+                              const huv_socs_tuning* t) {
+  if (!c || !t) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (t->m != 0 && (t->m < 4 * band + 1 || t->m > n ||
+                    (t->m != 32 && t->m != 64 && t->m != 128 && t->m != 192 && t->m != 256)))
+    return fail(HUV_INVALID_PARAMETER,
+                "tuning: coarse grid must be one of 32, 64, 128, 192, 256, >= 4 band + 1 and <= n");
+  if (t->n1 != 0) {
+    int n1s[4];
+    const int k = huvk::band_fft_options(n, n1s);
+    if (std::find(n1s, n1s + k, t->n1) == n1s + k)
+      return fail(HUV_INVALID_PARAMETER, "tuning: unsupported band-FFT factor for this n");
+  }
+  if (t->xnb != 0 && t->xnb != 4 && t->xnb != 6 && t->xnb != 8)
+    return fail(HUV_INVALID_PARAMETER, "tuning: xnb must be 4, 6 or 8");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->tuned[{n, band, nplanes, kmax}] = *t;
+  return HUV_OK;
 }
 
 huv_status huv_socs_images_host(huv_ctx* c, int nclips, int n, const float* masks, int nplanes,

@@ -22,6 +22,7 @@ struct Workspace {
               ILT6, ILT7, kSlots };
   void* ptr[kSlots] = {};
   size_t size[kSlots] = {};
+  uint64_t generation = 0;  // bumped whenever a slot is reallocated
   void* get(int slot, size_t bytes, hipError_t& err);
   void release();
 };
@@ -42,7 +43,21 @@ struct Ctx {
     float* lam = nullptr;    // [nplanes][Kmax], zero-padded
     int Kmax = 0;
   };
+  // keyed by the kernel-set ids followed by the box half-width
   std::map<std::vector<uint64_t>, PlanePack> plane_packs;
+  // Launch shapes chosen by the autotuner, keyed by (n, band, nplanes, Kmax).
+  std::map<std::vector<int>, huv_socs_tuning> tuned;
+  bool autotune = false;
+  // Captured launch sequences, keyed by the call's pointers, sizes and
+  // kernel-set ids; valid while the workspace generation is unchanged.
+  struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t generation = 0;
+    std::vector<uint64_t> ksets;
+  };
+  std::map<std::vector<uint64_t>, GraphEntry> graphs;
+  void drop_graph(GraphEntry& g);
   void forget_kset(uint64_t id);
   // Optional per-stage timing with HIP events on the active stream.
   bool profiling = false;
@@ -65,15 +80,18 @@ enum Stage { ST_ROWS_BAND = 0, ST_COLS_BAND, ST_FIELDS, ST_COARSE_ROWS, ST_COARS
 // Band spectrum of nclips real n x n masks: bins |k| <= B computed, stored
 // in boxes of half-width BB >= B (zero beyond B).  Layout Mq[clip][kx + BB][ky + BB].
 huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* masks, float2* Mq,
-                      bool pixel_transfer);
+                      bool pixel_transfer, int n1 = 0);
 
 huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes);
 huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
                        const float2* const** dkern, const float* const** dlam);
-huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, const float2** kern,
+huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int BB, const float2** kern,
                       const float** lam, int* Kmax);
+// Forward images; tune = NULL uses the kernel sets' own coarse grid and the
+// default launch shapes (the ILT adjoint relies on that band layout).
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
-                        huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out);
+                        huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out,
+                        const huv_socs_tuning* tune = nullptr);
 
 }  // namespace huvi
 

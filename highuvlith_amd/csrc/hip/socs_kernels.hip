@@ -142,7 +142,6 @@ struct CoarseCfg {
 };
 
 constexpr int kXcds = 8;
-constexpr int XNB = 6;  // band columns per load block of the x pass
 
 // e^{+2 pi i e / M} from the forward W_M table.
 __device__ __forceinline__ c32 tw_pos(const float2* __restrict__ tw_m, int e, int M) {
@@ -164,7 +163,8 @@ __device__ __forceinline__ c32 load_c32(__amdgpu_buffer_rsrc_t r, int voff, int 
   return c32{f.x, f.y};
 }
 
-template <int M>
+// XNB: band columns per load block of the x pass (a tuning parameter).
+template <int M, int XNB>
 __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_socs_fields(
     const float2* __restrict__ Mq, const float2* __restrict__ kern, const float* __restrict__ lam,
     int nclips, int nplanes, int Kmax, const float2* __restrict__ tw_m, float* __restrict__ coarse) {
@@ -524,15 +524,42 @@ static hipError_t rows_band_n(int B, int nclips, const float* masks, const float
              tmp);
 }
 
-hipError_t launch_rows_band(int n, int B, int nclips, const float* masks, const float2* tw_n,
-                            float2* tmp, hipStream_t s) {
+int band_fft_default(int n) {
   switch (n) {
-    case 256: return rows_band_n<16, 16>(B, nclips, masks, tw_n, tmp, s);
-    case 512: return rows_band_n<16, 32>(B, nclips, masks, tw_n, tmp, s);
-    case 1024: return rows_band_n<32, 32>(B, nclips, masks, tw_n, tmp, s);
-    case 2048: return rows_band_n<32, 64>(B, nclips, masks, tw_n, tmp, s);
-    default: return hipErrorInvalidValue;
+    case 256: return 16;
+    case 512: return 16;
+    case 1024: return 32;
+    case 2048: return 32;
+    default: return 0;
   }
+}
+
+int band_fft_options(int n, int* n1s) {
+  switch (n) {
+    case 256: n1s[0] = 8; n1s[1] = 16; return 2;
+    case 512: n1s[0] = 8; n1s[1] = 16; return 2;
+    case 1024: n1s[0] = 16; n1s[1] = 32; return 2;
+    case 2048: n1s[0] = 32; return 1;
+    default: return 0;
+  }
+}
+
+// Dispatch over the supported four-step factorisations n = N1 x N2.
+#define HUV_BAND_FFT_SWITCH(FN, ...)                                  \
+  switch (n * 100 + (n1 ? n1 : band_fft_default(n))) {                \
+    case 256 * 100 + 8: return FN<8, 32>(__VA_ARGS__);                 \
+    case 256 * 100 + 16: return FN<16, 16>(__VA_ARGS__);               \
+    case 512 * 100 + 8: return FN<8, 64>(__VA_ARGS__);                 \
+    case 512 * 100 + 16: return FN<16, 32>(__VA_ARGS__);               \
+    case 1024 * 100 + 16: return FN<16, 64>(__VA_ARGS__);              \
+    case 1024 * 100 + 32: return FN<32, 32>(__VA_ARGS__);              \
+    case 2048 * 100 + 32: return FN<32, 64>(__VA_ARGS__);              \
+    default: return hipErrorInvalidValue;                              \
+  }
+
+hipError_t launch_rows_band(int n, int n1, int B, int nclips, const float* masks,
+                            const float2* tw_n, float2* tmp, hipStream_t s) {
+  HUV_BAND_FFT_SWITCH(rows_band_n, B, nclips, masks, tw_n, tmp, s)
 }
 
 template <int N1, int N2>
@@ -544,39 +571,48 @@ static hipError_t cols_band_n(int B, int BB, int nclips, const float2* tmp, cons
              Mq);
 }
 
-hipError_t launch_cols_band(int n, int B, int BB, int nclips, const float2* tmp,
+hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const float2* tmp,
                             const float2* tw_n, const float* sinc_t, float2* Mq, hipStream_t s) {
   if (BB < B || 2 * BB + 1 > n) return hipErrorInvalidValue;
-  switch (n) {
-    case 256: return cols_band_n<16, 16>(B, BB, nclips, tmp, tw_n, sinc_t, Mq, s);
-    case 512: return cols_band_n<16, 32>(B, BB, nclips, tmp, tw_n, sinc_t, Mq, s);
-    case 1024: return cols_band_n<32, 32>(B, BB, nclips, tmp, tw_n, sinc_t, Mq, s);
-    case 2048: return cols_band_n<32, 64>(B, BB, nclips, tmp, tw_n, sinc_t, Mq, s);
-    default: return hipErrorInvalidValue;
-  }
+  HUV_BAND_FFT_SWITCH(cols_band_n, B, BB, nclips, tmp, tw_n, sinc_t, Mq, s)
 }
 
-template <int M>
+template <int M, int XNB>
 static hipError_t fields_m(int nclips, int nplanes, int Kmax, const float2* kern, const float* lam,
                            const float2* Mq, const float2* tw_m, float* coarse, hipStream_t s) {
   using C = CoarseCfg<M>;
   const size_t shmem = (size_t)C::LDS_C32 * sizeof(c32);
-  auto kernel = k_socs_fields<M>;
-  hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  auto kernel = k_socs_fields<M, XNB>;
+  // Once per instantiation (thread-safe static), never inside a graph capture.
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  if (attr != hipSuccess) return attr;
   const int groups = (nclips + kXcds - 1) / kXcds;
   HUV_LAUNCH(kernel, dim3(groups * kXcds * C::Q), dim3(C::THREADS), shmem, s, Mq, kern, lam,
              nclips, nplanes, Kmax, tw_m, coarse);
 }
 
-hipError_t launch_socs_fields(int m, int nclips, int nplanes, int Kmax, const float2* kern,
-                              const float* lam, const float2* Mq, const float2* tw_m,
-                              float* coarse, hipStream_t s) {
+template <int M>
+static hipError_t fields_x(int xnb, int nclips, int nplanes, int Kmax, const float2* kern,
+                           const float* lam, const float2* Mq, const float2* tw_m, float* coarse,
+                           hipStream_t s) {
+  switch (xnb ? xnb : kFieldsXnbDefault) {
+    case 4: return fields_m<M, 4>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 6: return fields_m<M, 6>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 8: return fields_m<M, 8>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_socs_fields(int m, int xnb, int nclips, int nplanes, int Kmax,
+                              const float2* kern, const float* lam, const float2* Mq,
+                              const float2* tw_m, float* coarse, hipStream_t s) {
   switch (m) {
-    case 32: return fields_m<32>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 64: return fields_m<64>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 128: return fields_m<128>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 192: return fields_m<192>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 256: return fields_m<256>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 32: return fields_x<32>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 64: return fields_x<64>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 128: return fields_x<128>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 192: return fields_x<192>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 256: return fields_x<256>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -626,7 +662,11 @@ static hipError_t band_image_t(int n, int G, int nimg, const float2* Ib, const f
                                float* out, hipStream_t s) {
   const size_t shmem = ((size_t)32 * (G + 1) + n) * sizeof(c32);
   auto kern = k_band_image<TMIN, TMAX>;
-  hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  // Sized for the largest band of this instantiation, once.
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)(((size_t)32 * (32 * (TMAX + 1)) + 2048) * sizeof(c32)));
+  if (attr != hipSuccess) return attr;
   const int groups = (nimg + kXcds - 1) / kXcds;
   HUV_LAUNCH(kern, dim3(groups * kXcds * (n / 32)), dim3(256), shmem, s, Ib, n, G, nimg, tw_n,
              out);

@@ -15,16 +15,24 @@ int coarse_classes(int m);
 // Kernel boxes and band spectra are stored at this width.
 int coarse_band(int m);
 
-hipError_t launch_rows_band(int n, int B, int nclips, const float* masks, const float2* tw_n,
-                            float2* tmp, hipStream_t s);
+// Band FFTs of length n run as four-step N1 x (n / N1): band_fft_default(n)
+// is the built-in N1, band_fft_options lists the supported ones (returns the
+// count).  n1 = 0 selects the default.
+int band_fft_default(int n);
+int band_fft_options(int n, int* n1s);
+// x-pass load block of the field kernel: kFieldsXnbDefault, or 4 / 6 / 8.
+constexpr int kFieldsXnbDefault = 6;
+
+hipError_t launch_rows_band(int n, int n1, int B, int nclips, const float* masks,
+                            const float2* tw_n, float2* tmp, hipStream_t s);
 // Mq: [nclips][2BB+1][2BB+1] band spectrum, zero beyond |k| <= B.
-hipError_t launch_cols_band(int n, int B, int BB, int nclips, const float2* tmp,
+hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const float2* tmp,
                             const float2* tw_n, const float* sinc_t, float2* Mq, hipStream_t s);
 // kern: [nplanes][Kmax][W][W] (W = 2 coarse_band(m) + 1), lam: [nplanes][Kmax];
-// Mq at the same W; coarse: [nclips][nplanes][m][m].
-hipError_t launch_socs_fields(int m, int nclips, int nplanes, int Kmax, const float2* kern,
-                              const float* lam, const float2* Mq, const float2* tw_m,
-                              float* coarse, hipStream_t s);
+// Mq at the same W; coarse: [nclips][nplanes][m][m].  xnb = 0 selects the default.
+hipError_t launch_socs_fields(int m, int xnb, int nclips, int nplanes, int Kmax,
+                              const float2* kern, const float* lam, const float2* Mq,
+                              const float2* tw_m, float* coarse, hipStream_t s);
 hipError_t launch_coarse_rows(int m, int B, int nimg, const float* coarse, const float2* tw_m,
                               float2* tmp1, hipStream_t s);
 hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2* tmp1,

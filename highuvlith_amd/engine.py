@@ -327,6 +327,22 @@ class DeviceContext:
                                            int(bool(reset))))
         return {s: (float(ms[i]), int(calls[i])) for i, s in enumerate(self.STAGES)}
 
+    def set_autotune(self, on=True):
+        """Time every launch shape on the first call of each new shape class."""
+        check(self.lib.huv_ctx_set_autotune(self.handle, int(bool(on))))
+
+    def tuning(self, n, band, nplanes, kmax):
+        """(m, n1, xnb) chosen for this shape class, or None."""
+        t = _lib.huv_socs_tuning()
+        found = C.c_int()
+        check(self.lib.huv_ctx_tuning_get(self.handle, n, band, nplanes, kmax, C.byref(t),
+                                          C.byref(found)))
+        return (t.m, t.n1, t.xnb) if found.value else None
+
+    def set_tuning(self, n, band, nplanes, kmax, m=0, n1=0, xnb=0):
+        t = _lib.huv_socs_tuning(int(m), int(n1), int(xnb))
+        check(self.lib.huv_ctx_tuning_set(self.handle, n, band, nplanes, kmax, C.byref(t)))
+
     def upload(self, ks, flare):
         """Upload a SparseKernelSet (eigenvalues normalised) -> DeviceKernelSet."""
         eig = np.ascontiguousarray(ks.eigenvalues / ks.normalization_divisor)
@@ -365,10 +381,12 @@ class DeviceKernelSet:
         self.handle = None
 
 
-def socs_images_device(ctx, masks, planes, out, pixel_transfer=False):
+def socs_images_device(ctx, masks, planes, out, pixel_transfer=False, graph=False):
     """Launch batched images on torch CUDA tensors (masks (N, n, n) f32, out (N, P, n, n) f32).
 
-    Runs on torch's current stream.
+    Runs on torch's current stream.  graph=True records the launch sequence
+    in a HIP graph on the first call and replays it for later calls on the
+    same tensors.
     """
     import torch
     assert masks.is_cuda and masks.dtype == torch.float32 and masks.is_contiguous()
@@ -378,10 +396,33 @@ def socs_images_device(ctx, masks, planes, out, pixel_transfer=False):
         raise HuvError(2, f"output shape {tuple(out.shape)} != {(nclips, len(planes), n, n)}")
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
-    opts = _lib.huv_socs_opts(int(bool(pixel_transfer)))
+    opts = _lib.huv_socs_opts(int(bool(pixel_transfer)), int(bool(graph)))
     check(ctx.lib.huv_socs_images(ctx.handle, nclips, n, C.c_void_p(masks.data_ptr()),
                                   len(planes), arr, C.c_void_p(out.data_ptr()), C.byref(opts)))
     return out
+
+
+def socs_images_batch_device(ctx, groups, pixel_transfer=False, graph=False):
+    """Uneven batch: groups = [(masks (N_i, n_i, n_i), planes_i, out (N_i, P_i, n_i, n_i)), ...]
+    of torch CUDA f32 tensors, all issued on torch's current stream; with
+    graph=True the whole batch replays as one graph launch."""
+    import torch
+    keep = []
+    arr = (_lib.huv_clip_group * len(groups))()
+    for i, (masks, planes, out) in enumerate(groups):
+        assert masks.is_cuda and masks.dtype == torch.float32 and masks.is_contiguous()
+        assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
+        nclips, n = masks.shape[0], masks.shape[-1]
+        if out.shape != (nclips, len(planes), n, n):
+            raise HuvError(2, f"group {i}: output shape {tuple(out.shape)} != "
+                              f"{(nclips, len(planes), n, n)}")
+        pl = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
+        keep.append(pl)
+        arr[i] = _lib.huv_clip_group(n, nclips, masks.data_ptr(), len(planes),
+                                     C.cast(pl, C.POINTER(C.c_void_p)), out.data_ptr())
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    opts = _lib.huv_socs_opts(int(bool(pixel_transfer)), int(bool(graph)))
+    check(ctx.lib.huv_socs_images_batch(ctx.handle, len(groups), arr, C.byref(opts)))
 
 
 def band_spectrum_device(ctx, masks, band, pixel_transfer=False):
@@ -407,7 +448,10 @@ class AerialImageEngine:
     kept in a bounded LRU cache.  Images are computed on the GPU.
     """
 
-    def __init__(self, source, optics, grid, max_kernels=None, settings=None, device=0):
+    def __init__(self, source, optics, grid, max_kernels=None, settings=None, device=0,
+                 context=None):
+        """context: an existing DeviceContext to share (engines of several grid
+        sizes feeding one uneven batch must share one)."""
         if settings is None:
             settings = ImagingSettings()
         if max_kernels is not None:
@@ -429,7 +473,9 @@ class AerialImageEngine:
         self._lock = threading.Lock()
         self._cache = collections.OrderedDict()
         self._focus = self._build(0.0)
-        self._ctx = None
+        self._ctx = context
+        if context is not None:
+            self.device = context.device
         self._dev = {}
 
     # -- kernel sets --------------------------------------------------------
@@ -501,7 +547,8 @@ class AerialImageEngine:
             self._dev[z] = self.context().upload(self.kernel_set(z), self.flare_fraction())
         return self._dev[z]
 
-    def compute_batch(self, masks, defocus_nm=(0.0,), pixel_transfer=False, out=None):
+    def compute_batch(self, masks, defocus_nm=(0.0,), pixel_transfer=False, out=None,
+                      graph=False):
         """Images of a batch of real masks at several focus planes on the GPU.
 
         masks: torch float32 CUDA tensor (N, n, n) (or numpy, copied).  Returns
@@ -521,7 +568,7 @@ class AerialImageEngine:
         if out is None:
             out = torch.empty((masks.shape[0], len(planes), n, n), dtype=torch.float32,
                               device=masks.device)
-        return socs_images_device(self.context(), masks, planes, out, pixel_transfer)
+        return socs_images_device(self.context(), masks, planes, out, pixel_transfer, graph)
 
     def compute_from_transmittance(self, transmittance, defocus_nm=0.0):
         t = np.asarray(transmittance)

@@ -170,7 +170,33 @@ typedef struct {
   /* Multiply the mask spectrum by sinc(pi kx/n) sinc(pi ky/n) (mask.rs:804-815):
    * with a pixel-aligned raster this reproduces AerialImageEngine::compute(mask). */
   int pixel_transfer;
+  /* Capture the call's launch sequence in a HIP graph the first time a given
+   * argument set is seen (the call itself runs eagerly) and replay the graph
+   * on later calls with the same device pointers, sizes and kernel sets.  The
+   * buffers must stay allocated while the context may replay them. */
+  int graph;
 } huv_socs_opts;
+
+/* Launch shape of the SOCS path for one (n, band, planes) combination:
+ * coarse grid m (32 Q; sets the number of 32-point transform classes Q),
+ * band-FFT factor n1 (four-step n = n1 x n / n1), x-pass load block xnb.
+ * Zero fields select the built-in defaults. */
+typedef struct {
+  int m;
+  int n1;
+  int xnb;
+} huv_socs_tuning;
+
+/* Autotuner: when on, the first huv_socs_images / huv_socs_images_batch call
+ * for a new (n, band, nplanes, max K) times every supported launch shape on
+ * that call's own inputs (HIP events on the context stream, results of the
+ * last run kept) and keeps the fastest for later calls.  The table can be
+ * read and preset (e.g. persisted across processes). */
+huv_status huv_ctx_set_autotune(huv_ctx* ctx, int on);
+huv_status huv_ctx_tuning_get(huv_ctx* ctx, int n, int band, int nplanes, int kmax,
+                              huv_socs_tuning* out, int* found);
+huv_status huv_ctx_tuning_set(huv_ctx* ctx, int n, int band, int nplanes, int kmax,
+                              const huv_socs_tuning* t);
 
 /* Batched aerial images (replaces socs_intensity + normalize + apply_flare,
  * aerial.rs:1682-1759, for compute_from_transmittance / compute_through_focus).
@@ -180,6 +206,21 @@ typedef struct {
  * out + (c * nplanes + p) * n * n.  Asynchronous on the context stream. */
 huv_status huv_socs_images(huv_ctx* ctx, int nclips, int n, const float* masks, int nplanes,
                            huv_kset* const* planes, float* out, const huv_socs_opts* opts);
+/* One group of equally sized clips of an uneven batch. */
+typedef struct {
+  int n;
+  int nclips;
+  const float* masks;      /* device, nclips x n x n */
+  int nplanes;
+  huv_kset* const* planes; /* nplanes kernel sets on the n grid */
+  float* out;              /* device, nclips x nplanes x n x n */
+} huv_clip_group;
+
+/* Uneven batch: every group in order on the context stream; with
+ * opts->graph the whole batch is one graph launch (batched
+ * compute_through_focus over clips of several sizes). */
+huv_status huv_socs_images_batch(huv_ctx* ctx, int ngroups, const huv_clip_group* groups,
+                                 const huv_socs_opts* opts);
 /* Same with host buffers (copies in, computes, copies out, synchronises). */
 huv_status huv_socs_images_host(huv_ctx* ctx, int nclips, int n, const float* masks,
                                 int nplanes, huv_kset* const* planes, float* out,

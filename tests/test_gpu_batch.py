@@ -1,0 +1,137 @@
+"""Batching layer on the GPU: HIP-graph replay, uneven multi-size batches and
+the launch-shape autotuner, each checked against eager calls and the FP64
+oracle (gate 1e-5 of clear-field intensity, aerial.rs:1682-1759)."""
+import numpy as np
+import pytest
+
+import oracle
+import highuvlith_amd as H
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _arf_engine(n, px, K, ctx=None):
+    return H.AerialImageEngine(H.Source.arf(H.IlluminationShape.Annular(0.7, 0.9)),
+                               H.ProjectionOptics.immersion_193i(), H.GridConfig(n, px),
+                               max_kernels=K, context=ctx)
+
+
+def _masks(n, count, seed):
+    rng = np.random.default_rng(seed)
+    m = np.zeros((count, n, n), np.float32)
+    for c in range(count):
+        for _ in range(10):
+            w, h = rng.integers(n // 32, n // 4, 2)
+            x, y = rng.integers(0, n - w), rng.integers(0, n - h)
+            m[c, y:y + h, x:x + w] = 1.0
+    return m
+
+
+def _oracle(eng, mask, z=0.0):
+    spec = oracle.fft2(mask.astype(np.complex128))
+    return oracle.socs_image(eng.kernel_set(z), spec, eng.flare_fraction(), threads=8)
+
+
+def test_graph_replays_match_eager_calls():
+    import torch
+    eng = _arf_engine(512, 2.0, 8)
+    zs = (0.0, 40.0)
+    planes = [eng.device_kernel_set(z) for z in zs]
+    ctx = eng.context()
+    masks = torch.from_numpy(_masks(512, 3, 1)).cuda()
+    out_g = torch.empty((3, 2, 512, 512), device="cuda")
+    out_e = torch.empty_like(out_g)
+    for rep in range(4):
+        # new contents in the same buffers, eager torch work between replays
+        masks.copy_(torch.from_numpy(_masks(512, 3, 10 + rep)).cuda())
+        scratch = torch.randn(1 << 20, device="cuda").sum()
+        torch.cuda.synchronize()
+        H.engine.socs_images_device(ctx, masks, planes, out_g, graph=True)
+        H.engine.socs_images_device(ctx, masks, planes, out_e, graph=False)
+        torch.cuda.synchronize()
+        diff = (out_g - out_e).abs().max().item()
+        print("replay", rep, "max |graph - eager|", diff, float(scratch))
+        assert diff == 0.0
+    ref = _oracle(eng, masks[1].cpu().numpy(), 40.0)
+    assert np.abs(out_g[1, 1].cpu().numpy() - ref).max() <= TOL
+
+
+def test_uneven_batch_matches_per_group_calls_and_oracle():
+    import torch
+    ctx = H.DeviceContext(0)
+    groups, singles = [], []
+    for n, count, K in [(256, 3, 12), (512, 2, 16), (1024, 1, 16)]:
+        eng = _arf_engine(n, 2.0, K, ctx)
+        planes = [eng.device_kernel_set(0.0)]
+        masks = torch.from_numpy(_masks(n, count, n)).cuda()
+        out = torch.empty((count, 1, n, n), device="cuda")
+        groups.append((masks, planes, out))
+        singles.append((eng, masks, planes))
+    for graph in (False, True, True):
+        for _, _, out in groups:
+            out.zero_()
+        H.engine.socs_images_batch_device(ctx, groups, graph=graph)
+        torch.cuda.synchronize()
+        for (eng, masks, planes), (_, _, out) in zip(singles, groups):
+            ref_gpu = torch.empty_like(out)
+            H.engine.socs_images_device(ctx, masks, planes, ref_gpu)
+            torch.cuda.synchronize()
+            assert (ref_gpu - out).abs().max().item() == 0.0
+    for eng, masks, _ in singles:
+        n = masks.shape[-1]
+        out = [g for g in groups if g[0].shape[-1] == n][0][2]
+        err = np.abs(out[0, 0].cpu().numpy() - _oracle(eng, masks[0].cpu().numpy())).max()
+        print("uneven", n, "err", err)
+        assert err <= TOL
+
+
+def test_autotuner_records_a_valid_shape_within_the_gate():
+    import torch
+    eng = _arf_engine(1024, 2.0, 12)
+    ctx = eng.context()
+    ctx.set_autotune(True)
+    planes = [eng.device_kernel_set(0.0)]
+    B = planes[0].band
+    assert ctx.tuning(1024, B, 1, planes[0].host.num_kernels) is None
+    masks = torch.from_numpy(_masks(1024, 2, 5)).cuda()
+    out = torch.empty((2, 1, 1024, 1024), device="cuda")
+    H.engine.socs_images_device(ctx, masks, planes, out)
+    torch.cuda.synchronize()
+    t = ctx.tuning(1024, B, 1, planes[0].host.num_kernels)
+    print("band", B, "tuned (m, n1, xnb)", t)
+    assert t is not None and t[0] >= 4 * B + 1 and t[0] in (32, 64, 128, 192, 256)
+    assert t[1] in (16, 32) and t[2] in (4, 6, 8)
+    for c in range(2):
+        err = np.abs(out[c, 0].cpu().numpy() - _oracle(eng, masks[c].cpu().numpy())).max()
+        assert err <= TOL
+    # every forced shape stays within the gate too
+    for m in (128, 256):
+        ctx.set_tuning(1024, B, 1, planes[0].host.num_kernels, m=m, n1=16, xnb=4)
+        H.engine.socs_images_device(ctx, masks, planes, out)
+        err = np.abs(out[1, 0].cpu().numpy() - _oracle(eng, masks[1].cpu().numpy())).max()
+        print("forced m", m, "err", err)
+        assert err <= TOL
+    with pytest.raises(H.HuvError):
+        ctx.set_tuning(1024, B, 1, 12, m=64)   # below 4B + 1
+    with pytest.raises(H.HuvError):
+        ctx.set_tuning(1024, B, 1, 12, n1=8)   # no 8 x 128 factorisation
+    with pytest.raises(H.HuvError):
+        ctx.set_tuning(1024, B, 1, 12, xnb=5)
+
+
+def test_graph_is_dropped_with_its_kernel_sets():
+    import torch
+    ctx = H.DeviceContext(0)
+    masks = torch.from_numpy(_masks(256, 2, 7)).cuda()
+    out = torch.empty((2, 1, 256, 256), device="cuda")
+    for K in (6, 10):
+        eng = _arf_engine(256, 4.0, K, ctx)
+        planes = [eng.device_kernel_set(0.0)]
+        H.engine.socs_images_device(ctx, masks, planes, out, graph=True)
+        H.engine.socs_images_device(ctx, masks, planes, out, graph=True)
+        torch.cuda.synchronize()
+        err = np.abs(out[0, 0].cpu().numpy() - _oracle(eng, masks[0].cpu().numpy())).max()
+        print("K", K, "err", err)
+        assert err <= TOL
+        del planes, eng
