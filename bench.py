@@ -1,21 +1,36 @@
 #!/usr/bin/env python3
-"""Benchmark of the batched SOCS aerial-image path on one MI355X.
+"""Benchmark of the SOCS aerial-image path and the GPU ILT on one MI355X.
 
-Workload (config 2): 512 seeded Manhattan clips, 1024^2 px at 1 nm (~40 %
-clear), EUV 13.5 nm NA 0.33, x-dipole annular sectors sigma 0.7-0.9 with a
-35 deg opening, 5 focus planes (-60/-30/0/+30/+60 nm), K = 32 kernels per
-plane built on the host in FP64.  One step = one huv_socs_images call over
-the whole batch (2560 images), inputs resident in HBM.
+Headline (one step = one batched SOCS pass over the whole batch, inputs
+resident in HBM, replayed as one HIP graph launch): 512 seeded Manhattan
+clips, 1024^2 px at 1 nm (~40 % clear), EUV 13.5 nm NA 0.33, x-dipole
+sectors sigma 0.7-0.9 with a 35 deg opening, 5 focus planes
+(-60/-30/0/+30/+60 nm), K = 32 kernels per plane built on the host in FP64
+(2560 images per step).
 
-Prints one JSON line.  `roofline` is the path's algorithmic HBM bytes per
-step over the step time (events on the context stream); `roofline_fp32` is
-the FP32 vector rate of the dominant kernel (socs_fields).  `cpu_baseline`
-times the FP64 CPU restatement of the reference algorithm on a bounded
-sample of the same workload on this host's cores.
+Also reported in the same JSON line (each skipped with --no-uneven /
+--no-ilt):
+  uneven   1000 clips with edges {256, 512, 1024, 2048} px (p .4/.3/.2/.1),
+           30-60 % clear, ArF 193i annular 0.7/0.9 at 2 nm, K = 32, one graph
+           launch per batch, autotuned launch shapes; aggregate Mpix/s and
+           p50/p99 per-clip latency (completion of the clip's size group in an
+           eager pass, groups issued smallest first).
+  ilt      64 contact-array targets 512^2 at 2 nm, ArF 193i quasar, K = 24,
+           resist cost (t 0.25, alpha 50), 50 CG/Armijo iterations; clips/s.
+
+`roofline`: the dominant kernel (socs_fields) against the FP32 peak with
+algorithmic flops per launch over its HIP-event duration; `traffic` is the
+PMC-measured HBM bytes per launch of that kernel (profiles/).
+`roofline_path_hbm`: the whole path's algorithmic HBM bytes per step over
+the step time against 8 TB/s.  `cpu_baseline`: the FP64 CPU restatement of
+the reference algorithm on a bounded sample of the headline workload on this
+host's cores.
 """
 import argparse
 import json
+import math
 import os
+import re
 import sys
 import time
 
@@ -23,11 +38,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP32_VECTOR_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3
+METRIC = "SOCS Mpix/s at K=32; ILT clips/s to fixed cost: plain vs NN warm-start"
+COUNTERS_SUMMARY = os.path.join(ROOT, "profiles", "socs_batch_1024_euv_k32_counters.md")
 
 
 def fft_flops(n):
-    return 5.0 * n * max(1.0, __import__("math").log2(n))
+    return 5.0 * n * max(1.0, math.log2(n))
 
 
 def fields_flops(nimg, K, m):
@@ -49,37 +66,24 @@ def fields_flops(nimg, K, m):
     return nimg * K * (x + y)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--clips", type=int, default=512)
-    ap.add_argument("--cpu-clips", type=int, default=2,
-                    help="clips in the CPU-baseline sample (x 5 planes)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calibration-mib", type=int, default=0,
-                    help="run one plain device copy of this many MiB before the steps "
-                         "(a known-size reference for HBM byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(clips are independent, see the multi-GPU note in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (None if absent)."""
+    try:
+        text = open(COUNTERS_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s+" + re.escape(kernel) + r"\s+([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
-    import numpy as np
-    import torch
 
-    import highuvlith_amd as H
-    from highuvlith_amd import workloads as W
-
-    src, opt, grid, K = W.c2_setup()
+def bench_fixed(args, H, W, np, torch):
+    src, opt, grid, K = W.euv_dipole_1024_setup()
     n = grid.size
     t0 = time.time()
     eng = H.AerialImageEngine(src, opt, grid, max_kernels=K)
-    planes = [eng.device_kernel_set(z) for z in W.C2_FOCUS_NM]
+    planes = [eng.device_kernel_set(z) for z in W.EUV_FOCUS_PLANES_NM]
     host_build_s = time.time() - t0
-    sets = [eng.kernel_set(z) for z in W.C2_FOCUS_NM]
+    sets = [eng.kernel_set(z) for z in W.EUV_FOCUS_PLANES_NM]
     Ks = [s.num_kernels for s in sets]
     S = [len(s.support) for s in sets]
     B, m = planes[0].band, planes[0].coarse
@@ -90,9 +94,6 @@ def main():
     ctx = eng.context()
     torch.cuda.synchronize()
 
-    def step():
-        H.engine.socs_images_device(ctx, masks, planes, out)
-
     if args.copy_calibration_mib > 0:
         nel = args.copy_calibration_mib * (1 << 20) // 4
         src_c = torch.ones(nel, dtype=torch.float32, device="cuda")
@@ -102,33 +103,41 @@ def main():
         torch.cuda.synchronize()
         del src_c, dst_c
 
-    for _ in range(args.warmup):
-        step()
+    def step(graph):
+        H.engine.socs_images_device(ctx, masks, planes, out, graph=graph)
+
+    for _ in range(max(args.warmup, 1)):
+        step(True)  # the first call records the graph
     torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+
+    # Per-kernel durations: the same launches issued eagerly with HIP events
+    # on the context stream (graph replays carry no events).
     ctx.set_profiling(True)
     ctx.stage_times(reset=True)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        step(False)
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t) / args.steps
+    dt_eager = (time.perf_counter() - t) / args.steps
     stages = ctx.stage_times(reset=True)
     ctx.set_profiling(False)
 
     nimg = nclips * P
-    mpix = nimg * n * n / 1e6
-    value = mpix / dt
+    value = nimg * n * n / 1e6 / dt
     algo_bytes = 4.0 * n * n * nclips + 4.0 * n * n * nimg + sum(8.0 * k * s for k, s in zip(Ks, S))
-    achieved = algo_bytes / dt / 1e9
-    f_ms, f_calls = stages["socs_fields"]
-    f_ms_per = f_ms / max(f_calls, 1)
-    flops = fields_flops(nimg, max(Ks), m)
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items() if v[1] > 0}
-    pipeline_ms = sum(stage_ms.values())
-
+    f_ms = stage_ms["socs_fields"]
+    flops = fields_flops(nimg, max(Ks), m)
+    achieved_tf = flops / (f_ms * 1e-3) / 1e12
+    traffic = measured_traffic("socs_fields")
     result = {
-        "metric": "SOCS Mpix/s at K=32 (config 2: 512 clips x 5 focus planes, 1024^2, EUV)",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -138,27 +147,32 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (complex FP32 fields and FFTs; kernels built in FP64 on the host)",
-        "data": "synthetic seeded Manhattan clips generated on the GPU (~40% clear)",
-        "config": {"workload": "config 2: batched fixed-size SOCS", "clips": nclips,
-                   "grid": f"{n}x{n} @ {grid.pixel_nm} nm", "focus_nm": list(W.C2_FOCUS_NM),
-                   "kernels_per_plane": Ks, "support": S, "band": B, "coarse_grid": m,
-                   "source_points": int(len(eng.source_points)),
-                   "method": sets[0].method, "parallelism": "1 GPU"},
+        "dtype": "fp32",
+        "data": "synthetic seeded Manhattan clips generated on the GPU (~40% clear); "
+                "FP64 host kernel sets",
+        "config": {"workload": "batched SOCS, 512 clips x 5 focus planes, 1024x1024 @ 1 nm, "
+                               "EUV NA 0.33 x-dipole, K=32, one graph launch per step",
+                   "clips": nclips, "grid": f"{n}x{n} @ {grid.pixel_nm} nm",
+                   "focus_nm": list(W.EUV_FOCUS_PLANES_NM), "kernels_per_plane": Ks,
+                   "support": S, "band": B, "coarse_grid": m,
+                   "source_points": int(len(eng.source_points)), "method": sets[0].method,
+                   "parallelism": "1 GPU"},
         "images_per_s": round(nimg / dt, 1),
+        "eager_ms_per_step": round(dt_eager * 1e3, 4),
         "host_kernel_build_s": round(host_build_s, 2),
         "stage_ms_per_launch": stage_ms,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "whole SOCS path per step (6 launches)",
-                     "algorithmic_bytes": algo_bytes,
-                     "event_ms_per_step": round(pipeline_ms, 4)},
-        "roofline_fp32": {"kernel": "socs_fields", "achieved": round(flops / (f_ms_per * 1e-3) / 1e12, 2),
-                          "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                          "frac": round(flops / (f_ms_per * 1e-3) / 1e12 / FP32_VECTOR_PEAK_TFLOPS, 4),
-                          "flops_per_launch": flops, "ms_per_launch": round(f_ms_per, 4)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4),
+                     "traffic": traffic, "kernel": "socs_fields",
+                     "compute": "FP32 on the vector ALUs (compute roof; no matrix instructions)",
+                     "flops_per_launch": flops, "ms_per_launch": f_ms},
+        "roofline_path_hbm": {"bound": "hbm", "achieved": round(algo_bytes / dt / 1e9, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(algo_bytes / dt / 1e9 / HBM_PEAK_GBS, 4),
+                              "algorithmic_bytes_per_step": algo_bytes,
+                              "band_image_write_GBps": round(
+                                  4.0 * n * n * nimg / (stage_ms["band_image"] * 1e-3) / 1e9, 1)},
     }
-
     if not args.no_cpu:
         import oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -180,6 +194,129 @@ def main():
                       f"structure (row-pruned inverse FFT per kernel, <=16 groups), {cdt:.1f} s"}
         result["speedup_vs_cpu"] = round(value / cpu_mpix, 1)
         result["max_abs_err_vs_fp64_sample"] = worst
+    return result
+
+
+def bench_uneven(args, H, W, np, torch):
+    edges = W.uneven_edges(args.uneven_clips, seed=3)
+    ctx = H.DeviceContext(0)
+    ctx.set_autotune(True)
+    rng = np.random.default_rng(33)
+    groups, pix = [], 0
+    t0 = time.time()
+    for e in sorted(set(int(x) for x in edges)):
+        cnt = int((edges == e).sum())
+        src, opt, grid, K = W.arf_annular_2nm_setup(e)
+        eng = H.AerialImageEngine(src, opt, grid, max_kernels=K, context=ctx)
+        planes = [eng.device_kernel_set(0.0)]
+        cf = torch.from_numpy(rng.uniform(0.3, 0.6, cnt))
+        masks = W.manhattan_clips(cnt, e, seed=100 + e, clear_fraction=cf,
+                                  max_side=max(16, e // 8))
+        out = torch.empty((cnt, 1, e, e), dtype=torch.float32, device="cuda")
+        groups.append((masks, planes, out, eng))
+        pix += cnt * e * e
+    host_s = time.time() - t0
+    batch = [(g[0], g[1], g[2]) for g in groups]
+    # tune each size class on its own inputs, then record the batch graph
+    for g in batch:
+        H.engine.socs_images_device(ctx, g[0], g[1], g[2])
+    for _ in range(2):
+        H.engine.socs_images_batch_device(ctx, batch, graph=True)
+    torch.cuda.synchronize()
+    reps = max(2, args.steps // 2)
+    t = time.perf_counter()
+    for _ in range(reps):
+        H.engine.socs_images_batch_device(ctx, batch, graph=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    # per-clip latency: completion of each size group, groups issued smallest first
+    ev0 = torch.cuda.Event(enable_timing=True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in batch]
+    ev0.record()
+    for g, ev in zip(batch, evs):
+        H.engine.socs_images_device(ctx, g[0], g[1], g[2])
+        ev.record()
+    torch.cuda.synchronize()
+    lat = []
+    for g, ev in zip(batch, evs):
+        lat += [ev0.elapsed_time(ev)] * g[0].shape[0]
+    lat = np.array(lat)
+    tuned = {}
+    for masks, planes, _, eng in groups:
+        e = masks.shape[-1]
+        tuned[str(e)] = {"band": planes[0].band, "count": int(masks.shape[0]),
+                         "m_n1_xnb": ctx.tuning(e, planes[0].band, 1, planes[0].host.num_kernels)}
+    return {"workload": f"{len(edges)} clips, edges 256/512/1024/2048 px at 2 nm, ArF 193i "
+                        "annular 0.7/0.9, K=32, one graph launch",
+            "mpix_per_batch": round(pix / 1e6, 1), "ms_per_batch": round(dt * 1e3, 3),
+            "value": round(pix / 1e6 / dt, 1), "unit": "Mpix/s",
+            "p50_clip_latency_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_clip_latency_ms": round(float(np.percentile(lat, 99)), 3),
+            "sizes": tuned, "host_kernel_build_s": round(host_s, 2)}
+
+
+def bench_ilt(args, H, W, np, torch):
+    from highuvlith_amd import ilt as L
+    src, opt, grid, K = W.arf_quasar_512_setup()
+    eng = H.AerialImageEngine(src, opt, grid, max_kernels=K)
+    targets = W.contact_array_targets(args.ilt_targets, seed=4, grid=grid)
+    cfg = L.ILTConfig(cost="resist", threshold=0.25, steepness=50.0, regularization=0.0,
+                      max_iterations=args.ilt_iterations)
+    L.optimize_ilt(eng, targets[:2], L.ILTConfig(cost="resist", regularization=0.0,
+                                                 max_iterations=2))  # warm-up
+    res = L.optimize_ilt(eng, targets, cfg)
+    return {"workload": f"{len(targets)} contact-array targets 512x512 @ 2 nm, ArF 193i quasar, "
+                        f"K={K}, resist cost, {args.ilt_iterations} CG/Armijo iterations",
+            "value": round(len(targets) / res.wall_s, 3), "unit": "clips/s",
+            "wall_s": round(res.wall_s, 3),
+            "mean_iterations": float(np.mean(res.iterations)),
+            "mean_initial_cost": float(np.mean([h[0] for h in res.cost_history])),
+            "mean_final_cost": float(np.mean(res.final_cost)),
+            "warm_start": "not built (plain ILT from the target only)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--clips", type=int, default=512)
+    ap.add_argument("--cpu-clips", type=int, default=2,
+                    help="clips in the CPU-baseline sample (x 5 planes)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-ilt", action="store_true")
+    ap.add_argument("--uneven-clips", type=int, default=1000)
+    ap.add_argument("--ilt-targets", type=int, default=64)
+    ap.add_argument("--ilt-iterations", type=int, default=50)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB before the steps "
+                         "(a known-size reference for HBM byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(clips and targets are independent; the sharding design is in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import highuvlith_amd as H
+    from highuvlith_amd import workloads as W
+
+    result = bench_fixed(args, H, W, np, torch)
+    print("fixed batch done", file=sys.stderr, flush=True)
+    for name, fn, skip in (("uneven", bench_uneven, args.no_uneven),
+                           ("ilt", bench_ilt, args.no_ilt)):
+        if skip:
+            continue
+        try:
+            result[name] = fn(args, H, W, np, torch)
+        except Exception as exc:  # report, keep the headline line
+            result[name] = {"error": f"{type(exc).__name__}: {exc}"}
+        print(f"{name} done", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     print(json.dumps(result))
 
 

@@ -997,7 +997,9 @@ huv_status huv_socs_images_host(huv_ctx* c, int nclips, int n, const float* mask
   huv_status st = HUV_OK;
   e = hipMemcpy(dm, masks, in_b, hipMemcpyHostToDevice);
   if (e != hipSuccess) st = hip_fail(e, "input copy");
-  if (st == HUV_OK) st = huv_socs_images(c, nclips, n, dm, nplanes, planes, dout, opts);
+  // Per-call device buffers: never recorded into a graph.
+  huv_socs_opts o{opts ? opts->pixel_transfer : 0, 0};
+  if (st == HUV_OK) st = huv_socs_images(c, nclips, n, dm, nplanes, planes, dout, &o);
   if (st == HUV_OK) {
     e = hipStreamSynchronize(c->stream());
     if (e == hipSuccess) e = hipMemcpy(out, dout, out_b, hipMemcpyDeviceToHost);

@@ -1,14 +1,18 @@
 """Seeded synthetic workloads of the imaging and ILT benchmarks.
 
-  config 1  one 256^2 clip at 4 nm: 65/130 nm lines + 8 Manhattan rectangles
-            (sides U(40, 200) nm), ArF 193i (NA 1.35), annular 0.7/0.9, K <= 24
-  config 2  512 Manhattan clips, 1024^2 at 1 nm, ~40 % clear, EUV NA 0.33,
-            x-dipole sectors 0.7-0.9 / 35 deg, focus -60..60 nm (5 planes), K = 32
-  config 3  1000 clips with edges {256, 512, 1024, 2048} px (p = .4/.3/.2/.1),
-            30-60 % density, ArF 193i annular 0.7/0.9 at 2 nm, K = 32
-  config 4  64 contact-array targets 512^2 at 2 nm, ArF 193i quasar, K = 24
-Masks are generated on the GPU (torch) for the batched configs, so the timed
-region starts with inputs already resident in HBM.
+  arf_lines_256      one 256^2 clip at 4 nm: 65/130 nm lines + 8 Manhattan
+                     rectangles (sides U(40, 200) nm), ArF 193i (NA 1.35),
+                     annular 0.7/0.9, K <= 24
+  euv_dipole_1024    512 Manhattan clips, 1024^2 at 1 nm, ~40 % clear, EUV NA
+                     0.33, x-dipole sectors 0.7-0.9 / 35 deg, focus -60..60 nm
+                     (5 planes), K = 32
+  arf_annular_2nm    1000 clips with edges {256, 512, 1024, 2048} px
+                     (p = .4/.3/.2/.1), 30-60 % density, ArF 193i annular
+                     0.7/0.9 at 2 nm, K = 32
+  arf_quasar_512     64 contact-array targets 512^2 at 2 nm, ArF 193i quasar,
+                     K = 24
+Masks are generated on the GPU (torch) for the batched workloads, so the
+timed region starts with inputs already resident in HBM.
 """
 import math
 
@@ -16,16 +20,16 @@ import numpy as np
 
 from . import engine as E
 
-C2_FOCUS_NM = (-60.0, -30.0, 0.0, 30.0, 60.0)
+EUV_FOCUS_PLANES_NM = (-60.0, -30.0, 0.0, 30.0, 60.0)
 
 
-def c1_setup():
+def arf_lines_256_setup():
     grid = E.GridConfig(256, 4.0)
     src = E.Source.arf(E.IlluminationShape.Annular(0.7, 0.9))
     return src, E.ProjectionOptics.immersion_193i(), grid, 24
 
 
-def c1_features(seed=0):
+def arf_lines_256_features(seed=0):
     """Bright-field mask: opaque 65 nm lines on a 130 nm pitch (one centred at
     x = 0, truncated at the 1024 nm field edge) plus 8 opaque rectangles."""
     rng = np.random.default_rng(seed)
@@ -38,12 +42,12 @@ def c1_features(seed=0):
     return feats
 
 
-def c1_raster(seed=0):
-    """Area-coverage transmittance (n, n) of the config-1 mask (product raster)."""
-    src, opt, grid, _ = c1_setup()
+def arf_lines_256_raster(seed=0):
+    """Area-coverage transmittance (n, n) of the 256^2 lines mask (product raster)."""
+    src, opt, grid, _ = arf_lines_256_setup()
     rects = []
     half = grid.field_size_nm() / 2.0
-    for f in c1_features(seed):
+    for f in arf_lines_256_features(seed):
         if f[0] == "lines":
             _, cd, pitch, _, off = f
             k_lo = math.floor((-half - off - cd / 2) / pitch)
@@ -56,19 +60,19 @@ def c1_raster(seed=0):
     return E.rasterize_rects(rects, grid, background=1.0).real
 
 
-def c2_setup():
+def euv_dipole_1024_setup():
     grid = E.GridConfig(1024, 1.0)
     src = E.Source(E.EUV_WAVELENGTH_NM, E.IlluminationShape.SectorPoles(0.7, 0.9, 35.0, 0.0, 2))
     return src, E.EuvProjectionOptics.nxe_033(), grid, 32
 
 
-def c3_setup(edge):
+def arf_annular_2nm_setup(edge):
     grid = E.GridConfig(edge, 2.0)
     src = E.Source.arf(E.IlluminationShape.Annular(0.7, 0.9))
     return src, E.ProjectionOptics.immersion_193i(), grid, 32
 
 
-def c4_setup():
+def arf_quasar_512_setup():
     grid = E.GridConfig(512, 2.0)
     src = E.Source.arf(E.IlluminationShape.SectorPoles(0.7, 0.9, 30.0, 45.0, 4))
     return src, E.ProjectionOptics.immersion_193i(), grid, 24
@@ -110,16 +114,16 @@ def manhattan_clips(nclips, n, seed, clear_fraction=0.4, min_side=8, max_side=12
     return out
 
 
-def c3_edges(nclips=1000, seed=3):
+def uneven_edges(nclips=1000, seed=3):
     rng = np.random.default_rng(seed)
     return rng.choice([256, 512, 1024, 2048], size=nclips, p=[0.4, 0.3, 0.2, 0.1])
 
 
-def c4_targets(ntargets, seed=4, grid=None):
+def contact_array_targets(ntargets, seed=4, grid=None):
     """Contact arrays: 2x2..6x6 squares, side U(40, 60), pitch U(90, 140) nm,
     per-contact jitter up to 3 nm; 4x supersampled area coverage."""
     from .targets import create_target_holes
-    grid = grid or c4_setup()[2]
+    grid = grid or arf_quasar_512_setup()[2]
     rng = np.random.default_rng(seed)
     out = np.zeros((ntargets, grid.size, grid.size))
     for t in range(ntargets):

@@ -162,6 +162,7 @@ huv_status huv_ctx_stage_times(huv_ctx* ctx, double* ms, int64_t* calls, int nst
 huv_status huv_kset_upload(huv_ctx* ctx, int n, int num_kernels, int64_t num_support,
                            const int64_t* support, const double* eigenvalues,
                            const double* values, double flare, huv_kset** out);
+/* Destroy a kernel set (before its context); drops graphs recorded with it. */
 void huv_kset_destroy(huv_kset* ks);
 /* Band half-width B and coarse grid m chosen for this set. */
 huv_status huv_kset_shape(const huv_kset* ks, int* band, int* coarse);
@@ -221,7 +222,8 @@ typedef struct {
  * compute_through_focus over clips of several sizes). */
 huv_status huv_socs_images_batch(huv_ctx* ctx, int ngroups, const huv_clip_group* groups,
                                  const huv_socs_opts* opts);
-/* Same with host buffers (copies in, computes, copies out, synchronises). */
+/* Same with host buffers (copies in, computes, copies out, synchronises;
+ * opts->graph is ignored: the device copies are per call). */
 huv_status huv_socs_images_host(huv_ctx* ctx, int nclips, int n, const float* masks,
                                 int nplanes, huv_kset* const* planes, float* out,
                                 const huv_socs_opts* opts);

@@ -14,8 +14,8 @@ from highuvlith_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def c4_engine(n=512):
-    src, opt, grid, K = W.c4_setup()
+def quasar_engine(n=512):
+    src, opt, grid, K = W.arf_quasar_512_setup()
     grid = H.GridConfig(n, grid.pixel_nm)
     return H.AerialImageEngine(src, opt, grid, max_kernels=K)
 
@@ -35,19 +35,19 @@ def perturbed_theta(target, seed):
     return (target - 0.5) + 0.3 * np.sin(i * 0.037 + seed) * np.cos(j * 0.023 - seed)
 
 
-@pytest.mark.parametrize("case", ["c4_resist", "multi_condition"])
+@pytest.mark.parametrize("case", ["quasar_resist", "multi_condition"])
 def test_cost_and_gradient_match_oracle(case):
-    if case == "c4_resist":
-        eng = c4_engine(512)
+    if case == "quasar_resist":
+        eng = quasar_engine(512)
         cfg = L.ILTConfig(cost="resist", regularization=0.0)
-        targets = W.c4_targets(2, seed=11, grid=eng.grid)
+        targets = W.contact_array_targets(2, seed=11, grid=eng.grid)
     else:
-        eng = c4_engine(256)
+        eng = quasar_engine(256)
         cfg = L.ILTConfig(cost="image", regularization=0.05, binarization_weight=0.02,
                           absorber=-math.sqrt(0.06),
                           conditions=[L.ProcessCondition(0.0, 1.0, 1.0),
                                       L.ProcessCondition(60.0, 1.05, 0.5)])
-        targets = W.c4_targets(2, seed=12, grid=eng.grid)
+        targets = W.contact_array_targets(2, seed=12, grid=eng.grid)
     prob = L.IltProblem(eng, cfg)
     thetas = np.stack([perturbed_theta(t, s) for s, t in enumerate(targets)])
     cost, grad = prob.cost_and_gradient(targets, thetas)
@@ -65,9 +65,9 @@ def test_cost_and_gradient_match_oracle(case):
 def test_gradient_against_finite_differences():
     """Directional central differences of the GPU cost, and the GPU gradient at
     32 random pixels against central differences of the FP64 oracle cost."""
-    eng = c4_engine(512)
+    eng = quasar_engine(512)
     cfg = L.ILTConfig(cost="resist", regularization=0.0)
-    target = W.c4_targets(1, seed=21, grid=eng.grid)
+    target = W.contact_array_targets(1, seed=21, grid=eng.grid)
     theta = perturbed_theta(target[0], 3)[None].astype(np.float32)
     prob = L.IltProblem(eng, cfg)
     _, grad = prob.cost_and_gradient(target, theta)
@@ -102,9 +102,9 @@ def test_gradient_against_finite_differences():
 
 
 def test_gpu_optimizer_tracks_oracle_run_stage():
-    eng = c4_engine(256)
+    eng = quasar_engine(256)
     cfg = L.ILTConfig(cost="resist", regularization=0.0, max_iterations=12)
-    targets = W.c4_targets(3, seed=31, grid=eng.grid)
+    targets = W.contact_array_targets(3, seed=31, grid=eng.grid)
     res = L.optimize_ilt(eng, targets, cfg)
     for t in range(len(targets)):
         orc = oracle_problem(eng, cfg, targets[t])
@@ -119,7 +119,7 @@ def test_gpu_optimizer_tracks_oracle_run_stage():
 
 
 def test_invalid_ilt_configs_rejected():
-    eng = c4_engine(256)
+    eng = quasar_engine(256)
     with pytest.raises(H.HuvError):
         L.IltProblem(eng, L.ILTConfig(conditions=[]))
     with pytest.raises(H.HuvError):

@@ -62,15 +62,15 @@ def test_band_spectrum_matches_fft(gpu_ctx):
             assert err < 2e-6, (n, err)
 
 
-def test_c1_image_matches_oracle():
-    """256^2 @ 4 nm, ArF 193i annular 0.7/0.9, K=24 (config 1 shape)."""
+def test_lines_256_image_matches_oracle():
+    """256^2 @ 4 nm, ArF 193i annular 0.7/0.9, K=24 (lines + rectangles clip)."""
     eng = _engine(256, 4.0)
     rng = np.random.default_rng(11)
     masks = np.stack([_manhattan(256, 8, rng) for _ in range(3)])
     got = eng.compute_batch(masks, (0.0,)).cpu().numpy()
     ref = _oracle_images(eng, masks, (0.0,))
     err = np.abs(got - ref).max()
-    print("C1 max err", err, "K", eng.num_kernels())
+    print("lines-256 max err", err, "K", eng.num_kernels())
     assert err <= TOL
 
 
@@ -78,7 +78,7 @@ def test_c1_image_matches_oracle():
     (256, 2.0, "arf", 16),    # coarse 32
     (512, 2.0, "arf", 24),    # coarse 64
     (1024, 2.0, "arf", 12),   # coarse 128
-    (1024, 1.0, "euv", 32),   # coarse 192 (config 2 shape)
+    (1024, 1.0, "euv", 32),   # coarse 192 (the 1024^2 EUV batch shape)
     (2048, 2.0, "arf", 6),    # coarse 256
 ])
 def test_sizes_match_oracle(n, px, kind, K):

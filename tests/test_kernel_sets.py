@@ -183,7 +183,7 @@ def test_pupil_sampling_guard_and_settings_validation():
 
 
 def test_config_kernel_sets_have_expected_shapes():
-    # config 1: ArF 193i annular 0.7/0.9, 256 x 4 nm, K <= 24: S = 577, randomized path
+    # 256 x 4 nm ArF 193i annular 0.7/0.9, K <= 24: S = 577, randomized path
     eng = H.AerialImageEngine(H.Source.arf(I.Annular(0.7, 0.9)), H.ProjectionOptics.immersion_193i(),
                               H.GridConfig(256, 4.0), max_kernels=24)
     d = eng.kernel_diagnostics()
