@@ -149,9 +149,19 @@ SIGNATURES = {
     "huv_ilt_create": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(P), PD, PD,
                                  C.POINTER(huv_ilt_config), C.POINTER(P)]),
     "huv_ilt_destroy": (None, [P]),
+    "huv_unet_create": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]),
+    "huv_unet_destroy": (None, [P]),
+    "huv_unet_num_params": (C.c_int, [P, PI64]),
+    "huv_unet_get_params": (C.c_int, [P, PF]),
+    "huv_unet_set_params": (C.c_int, [P, PF]),
+    "huv_unet_forward": (C.c_int, [P, C.c_int, C.c_int, C.c_int, PF, PF]),
+    "huv_unet_loss_grad": (C.c_int, [P, C.c_int, C.c_int, C.c_int, PF, PF, PD, PF]),
+    "huv_unet_adam": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.c_double]),
     "huv_ilt_cost_grad": (C.c_int, [P, C.c_int, PF, PF, PD, PF]),
     "huv_ilt_optimize": (C.c_int, [P, C.c_int, PF, PF, C.c_int, C.c_double, C.c_double, C.c_int,
                                    PD, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "huv_ilt_optimize_to": (C.c_int, [P, C.c_int, PF, PF, C.c_int, C.c_double, C.c_double,
+                                      C.c_int, PD, PD, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }
 
 _lib = None

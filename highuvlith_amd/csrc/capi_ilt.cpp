@@ -206,9 +206,15 @@ huv_status huv_ilt_cost_grad(huv_ilt* p, int T, const float* targets, const floa
   return e == hipSuccess ? HUV_OK : hip_fail(e, "cost copy");
 }
 
-huv_status huv_ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta,
-                            int max_iterations, double learning_rate, double tol, int use_cg,
-                            double* cost_hist, int* iterations, int* termination) {
+}  // extern "C"
+
+namespace {
+
+// run_stage for every target; stop_cost (nullable) ends a target as soon as
+// its cost is at or below stop_cost[t] (termination 3).
+huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, int max_iterations,
+                        double learning_rate, double tol, int use_cg, const double* stop_cost,
+                        double* cost_hist, int* iterations, int* termination) {
   if (!p || !targets || !theta || !cost_hist || !iterations || !termination)
     return fail(HUV_INVALID_PARAMETER, "null argument");
   if (T < 1 || max_iterations < 0) return fail(HUV_INVALID_PARAMETER, "bad target count or budget");
@@ -271,7 +277,13 @@ huv_status huv_ilt_optimize(huv_ilt* p, int T, const float* targets, float* thet
     for (int k = 0; k < H; ++k) cost_hist[(size_t)t * H + k] = NAN;
   st = eval(theta, grad, cost);
   if (st != HUV_OK) return st;
-  for (int t = 0; t < T; ++t) cost_hist[(size_t)t * H] = cost[t];
+  for (int t = 0; t < T; ++t) {
+    cost_hist[(size_t)t * H] = cost[t];
+    if (stop_cost && cost[t] <= stop_cost[t]) {
+      active[t] = 0;
+      term[t] = 3;
+    }
+  }
   if (!reduce(grad, grad, 0)) return fail_e("dot");
   for (int t = 0; t < T; ++t) gnorm[t] = std::sqrt(h_red[t]);
   for (int t = 0; t < T; ++t) { h_a[t] = -1.0; h_b[t] = 0.0; }
@@ -410,6 +422,10 @@ huv_status huv_ilt_optimize(huv_ilt* p, int T, const float* targets, float* thet
         active[t] = 0;
         term[t] = 0;
       }
+      if (stop_cost && cost[t] <= stop_cost[t]) {
+        active[t] = 0;
+        term[t] = 3;
+      }
     }
     if (!upload_coefs()) return fail_e("coef upload");
     e = huvk::launch_vec_dir(gnew, coefA, coefB, N, T, dir, s);
@@ -426,6 +442,26 @@ huv_status huv_ilt_optimize(huv_ilt* p, int T, const float* targets, float* thet
     termination[t] = term[t];
   }
   return HUV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+huv_status huv_ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta,
+                            int max_iterations, double learning_rate, double tol, int use_cg,
+                            double* cost_hist, int* iterations, int* termination) {
+  return ilt_optimize(p, T, targets, theta, max_iterations, learning_rate, tol, use_cg, nullptr,
+                      cost_hist, iterations, termination);
+}
+
+huv_status huv_ilt_optimize_to(huv_ilt* p, int T, const float* targets, float* theta,
+                               int max_iterations, double learning_rate, double tol, int use_cg,
+                               const double* stop_cost, double* cost_hist, int* iterations,
+                               int* termination) {
+  if (!stop_cost) return fail(HUV_INVALID_PARAMETER, "null argument");
+  return ilt_optimize(p, T, targets, theta, max_iterations, learning_rate, tol, use_cg, stop_cost,
+                      cost_hist, iterations, termination);
 }
 
 }  // extern "C"

@@ -1,0 +1,623 @@
+// HIP kernels of the U-Net warm start (gfx950): BF16 activations and
+// weights, FP32 accumulation on the matrix cores.
+//
+// 3x3 convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16
+// (D[32 x 32] += A[32 x 16] B[16 x 32]; lane l supplies A row l % 32 and
+// B column l % 32 for k in 8 (l / 32) .. 8 (l / 32) + 7, and holds
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] in accumulator r):
+//   forward:  rows = output pixels, columns = output channels, k = (tap, ci);
+//             NHWC activations and [co][tap][ci] weights give each lane's
+//             8 consecutive k as one 16-byte load.
+//   data gradient: the forward kernel on the ReLU-masked output gradient
+//             with the rotated, transposed weights.
+//   weight gradient: rows = output channels, columns = input channels of
+//             one tap, k = pixels (16 consecutive x of one row), from NCHW
+//             copies; the +-1 column shift of a tap is a 2-byte funnel shift
+//             of aligned loads.  Split-K partials are summed in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "unet_kernels.hpp"
+
+namespace huvk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)0x7fc0;  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                              // round to nearest even
+  return (bf16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 ld8(const bf16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+
+// ---------------------------------------------------------------------------
+// Forward 3x3 convolution.  Work-group = 4 waves along pixels; each wave
+// computes BM x BN blocks of 32 pixels x 32 output channels.  The operands
+// of step s + 1 are loaded before the products of step s are issued.
+// ---------------------------------------------------------------------------
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_conv3x3_fwd(const bf16_t* __restrict__ X, int N, int H,
+                                                     int W, int Cin,
+                                                     const bf16_t* __restrict__ Wt,
+                                                     const float* __restrict__ bias, int Cout,
+                                                     int relu, bf16_t* __restrict__ Y) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int half = lane >> 5, li = lane & 31;
+  const int64_t NP = (int64_t)N * H * W;
+  const int64_t pbase = ((int64_t)blockIdx.x * 4 + wave) * (32 * BM);
+  const int cobase = blockIdx.y * 32 * BN;
+  if (pbase >= NP) return;  // whole wave; no barriers in this kernel
+  const int CB = Cin / 8, KB = 9 * CB, steps = (KB + 1) / 2;
+  const int HW = H * W;
+  int pn[BM], py[BM], px[BM];
+  bool pv[BM];
+#pragma unroll
+  for (int bm = 0; bm < BM; ++bm) {
+    const int64_t p = pbase + 32 * bm + li;
+    pv[bm] = p < NP;
+    const int64_t pp = pv[bm] ? p : 0;
+    pn[bm] = (int)(pp / HW);
+    const int rem = (int)(pp - (int64_t)pn[bm] * HW);
+    py[bm] = rem / W;
+    px[bm] = rem - py[bm] * W;
+  }
+  int co[BN];
+#pragma unroll
+  for (int bn = 0; bn < BN; ++bn) co[bn] = cobase + 32 * bn + li;
+
+  auto load = [&](int s, bf16x8* a, bf16x8* b) {
+    const int kb = 2 * s + half;
+    const bool kv = kb < KB;
+    const int tap = kv ? kb / CB : 0, cb = kv ? kb - (kb / CB) * CB : 0;
+    const int dy = tap / 3 - 1, dx = tap - (tap / 3) * 3 - 1;
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm) {
+      const int yy = py[bm] + dy, xx = px[bm] + dx;
+      const bool v = kv && pv[bm] && yy >= 0 && yy < H && xx >= 0 && xx < W;
+      a[bm] = v ? ld8(X + (((int64_t)pn[bm] * H + yy) * W + xx) * Cin + cb * 8) : zero8();
+    }
+#pragma unroll
+    for (int bn = 0; bn < BN; ++bn) {
+      const bool v = kv && co[bn] < Cout;
+      b[bn] = v ? ld8(Wt + ((int64_t)co[bn] * 9 + tap) * Cin + cb * 8) : zero8();
+    }
+  };
+
+  f32x16 acc[BM][BN];
+#pragma unroll
+  for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < BN; ++bn)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[bm][bn][r] = 0.f;
+  bf16x8 a[BM], b[BN], an[BM], bnx[BN];
+  load(0, a, b);
+  for (int s = 0; s < steps; ++s) {
+    if (s + 1 < steps) load(s + 1, an, bnx);
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+      for (int bn = 0; bn < BN; ++bn)
+        acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bm], b[bn], acc[bm][bn], 0, 0, 0);
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm) a[bm] = an[bm];
+#pragma unroll
+    for (int bn = 0; bn < BN; ++bn) b[bn] = bnx[bn];
+  }
+#pragma unroll
+  for (int bn = 0; bn < BN; ++bn) {
+    if (co[bn] >= Cout) continue;
+    const float bv = bias ? bias[co[bn]] : 0.f;
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = 8 * (r / 4) + 4 * half + (r % 4);
+        const int64_t p = pbase + 32 * bm + i;
+        if (p < NP) {
+          float v = acc[bm][bn][r] + bv;
+          if (relu) v = fmaxf(v, 0.f);
+          Y[p * Cout + co[bn]] = f2bf(v);
+        }
+      }
+  }
+}
+
+// Wr[ci][t][co] = W[co][8 - t][ci].
+__global__ void k_conv_weight_rot(const bf16_t* __restrict__ Wt, int Cout, int Cin,
+                                  bf16_t* __restrict__ Wr) {
+  const int64_t total = (int64_t)Cout * 9 * Cin;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int ci = (int)(i % Cin);
+    const int t = (int)((i / Cin) % 9);
+    const int co = (int)(i / (9 * (int64_t)Cin));
+    Wr[((int64_t)ci * 9 + (8 - t)) * Cout + co] = Wt[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// NHWC <-> NCHW through a 32 x 32 LDS tile (pixels x channels).
+// ---------------------------------------------------------------------------
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_transpose_t(bf16_t* __restrict__ dY,
+                                                     const bf16_t* __restrict__ Y, int N, int HW,
+                                                     int C, bf16_t* __restrict__ out) {
+  __shared__ bf16_t tile[32][33];
+  const int pt = blockIdx.x, ct = blockIdx.y;
+  const int64_t p0 = (int64_t)pt * 32;  // flat pixel over N * HW (HW % 32 == 0)
+  const int c0 = ct * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t p = p0 + r;
+    const int c = c0 + tx;
+    bf16_t v = 0;
+    if (c < C) {
+      const int64_t idx = p * C + c;
+      v = dY[idx];
+      if (MASK && Y && bf2f(Y[idx]) <= 0.f) {
+        v = 0;
+        dY[idx] = 0;
+      }
+    }
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  const int n = (int)(p0 / HW);
+  const int hw0 = (int)(p0 - (int64_t)n * HW);
+  for (int r = ty; r < 32; r += 8) {
+    const int c = c0 + r;
+    if (c < C) out[((int64_t)n * C + c) * HW + hw0 + tx] = tile[tx][r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient (split K over rows).  Grid x = co blocks * ci blocks * 9
+// taps, grid y = splits; the 4 waves take interleaved rows of the split and
+// are reduced through LDS in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16_t* __restrict__ dYT,
+                                                       const bf16_t* __restrict__ XT, int N,
+                                                       int H, int W, int Cin, int Cout,
+                                                       int rows_per_split,
+                                                       float* __restrict__ part) {
+  __shared__ float red[4][32][33];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int half = lane >> 5, li = lane & 31;
+  const int ciblocks = (Cin + 31) / 32;
+  const int tile = blockIdx.x;
+  const int tap = tile % 9, rest = tile / 9;
+  const int cib = rest % ciblocks, cob = rest / ciblocks;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const int split = blockIdx.y;
+  const int rows = N * H;
+  const int r0 = split * rows_per_split;
+  const int r1 = min(rows, r0 + rows_per_split);
+  const int coA = cob * 32 + li, ciB = cib * 32 + li;
+  const bool av = coA < Cout, bv = ciB < Cin;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  for (int row = r0 + wave; row < r1; row += 4) {
+    const int n = row / H, y = row - n * H;
+    const int yy = y + dy;
+    const bool rv = bv && yy >= 0 && yy < H;
+    const bf16_t* arow = dYT + (((int64_t)n * Cout + (av ? coA : 0)) * H + y) * W;
+    const bf16_t* brow = XT + (((int64_t)n * Cin + (bv ? ciB : 0)) * H + (rv ? yy : 0)) * W;
+    for (int x0 = 0; x0 < W; x0 += 16) {
+      const int xs = x0 + 8 * half;
+      const bf16x8 a = av ? ld8(arow + xs) : zero8();
+      bf16x8 b = zero8();
+      if (rv) {
+        uint4 w = *reinterpret_cast<const uint4*>(brow + xs);
+        if (dx < 0) {
+          const uint32_t prev = xs > 0 ? *reinterpret_cast<const uint32_t*>(brow + xs - 2) : 0u;
+          w = make_uint4(__builtin_amdgcn_alignbit(w.x, prev, 16),
+                         __builtin_amdgcn_alignbit(w.y, w.x, 16),
+                         __builtin_amdgcn_alignbit(w.z, w.y, 16),
+                         __builtin_amdgcn_alignbit(w.w, w.z, 16));
+        } else if (dx > 0) {
+          const uint32_t next = xs + 8 < W ? *reinterpret_cast<const uint32_t*>(brow + xs + 8) : 0u;
+          w = make_uint4(__builtin_amdgcn_alignbit(w.y, w.x, 16),
+                         __builtin_amdgcn_alignbit(w.z, w.y, 16),
+                         __builtin_amdgcn_alignbit(w.w, w.z, 16),
+                         __builtin_amdgcn_alignbit(next, w.w, 16));
+        }
+        b = __builtin_bit_cast(bf16x8, w);
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) red[wave][8 * (r / 4) + 4 * half + (r % 4)][li] = acc[r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int i = e / 32, j = e % 32;
+    const float v = ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
+    const int co = cob * 32 + i, ci = cib * 32 + j;
+    if (co < Cout && ci < Cin)
+      part[(((int64_t)split * Cout + co) * 9 + tap) * Cin + ci] = v;
+  }
+}
+
+__global__ void k_sum_splits(const float* __restrict__ part, int splits, int64_t count,
+                             float* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int k = 0; k < splits; ++k) s += part[(int64_t)k * count + i];
+    out[i] = s;
+  }
+}
+
+// db[c] = sum_n sum_hw dYT[n][c][hw]: one work-group per channel, fixed order.
+__global__ __launch_bounds__(256) void k_channel_sum(const bf16_t* __restrict__ dYT, int N,
+                                                     int HW, int C, float* __restrict__ db) {
+  __shared__ float sh[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (int n = 0; n < N; ++n) {
+    const bf16_t* p = dYT + ((int64_t)n * C + c) * HW;
+    for (int i = threadIdx.x; i < HW; i += 256) s += bf2f(p[i]);
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[c] = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// Pooling, upsampling + concatenation, head, loss, optimiser.
+// ---------------------------------------------------------------------------
+__global__ void k_maxpool_fwd(const bf16_t* __restrict__ X, int N, int H, int W, int C,
+                              bf16_t* __restrict__ Y) {
+  const int Ho = H / 2, Wo = W / 2;
+  const int64_t total = (int64_t)N * Ho * Wo * C;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const int64_t q = i / C;
+    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((int64_t)Wo * Ho));
+    const bf16_t* b = X + (((int64_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
+    float m = bf2f(b[0]);
+    m = fmaxf(m, bf2f(b[C]));
+    m = fmaxf(m, bf2f(b[(int64_t)W * C]));
+    m = fmaxf(m, bf2f(b[(int64_t)W * C + C]));
+    Y[i] = f2bf(m);
+  }
+}
+
+__global__ void k_maxpool_bwd(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY, int N,
+                              int H, int W, int C, bf16_t* __restrict__ dX) {
+  const int Ho = H / 2, Wo = W / 2;
+  const int64_t total = (int64_t)N * Ho * Wo * C;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const int64_t q = i / C;
+    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((int64_t)Wo * Ho));
+    const int64_t o[4] = {0, C, (int64_t)W * C, (int64_t)W * C + C};
+    const int64_t base = (((int64_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
+    int arg = 0;
+    float m = bf2f(X[base]);
+    for (int k = 1; k < 4; ++k) {
+      const float v = bf2f(X[base + o[k]]);
+      if (v > m) { m = v; arg = k; }
+    }
+    for (int k = 0; k < 4; ++k) dX[base + o[k]] = k == arg ? dY[i] : (bf16_t)0;
+  }
+}
+
+__global__ void k_upcat_fwd(const bf16_t* __restrict__ U, int Cu, const bf16_t* __restrict__ S,
+                            int Cs, int N, int H, int W, bf16_t* __restrict__ cat) {
+  const int Ct = Cu + Cs;
+  const int64_t total = (int64_t)N * H * W * Ct;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % Ct);
+    const int64_t p = i / Ct;
+    if (c < Cu) {
+      const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((int64_t)W * H));
+      cat[i] = U[(((int64_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Cu + c];
+    } else {
+      cat[i] = S[p * Cs + (c - Cu)];
+    }
+  }
+}
+
+__global__ void k_upcat_bwd_u(const bf16_t* __restrict__ dcat, int Cu, int Cs, int N, int H,
+                              int W, bf16_t* __restrict__ dU) {
+  const int Ct = Cu + Cs, Hh = H / 2, Wh = W / 2;
+  const int64_t total = (int64_t)N * Hh * Wh * Cu;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % Cu);
+    const int64_t q = i / Cu;
+    const int x = (int)(q % Wh), y = (int)((q / Wh) % Hh), n = (int)(q / ((int64_t)Wh * Hh));
+    const int64_t b = (((int64_t)n * H + 2 * y) * W + 2 * x) * Ct + c;
+    const float s = (bf2f(dcat[b]) + bf2f(dcat[b + Ct])) +
+                    (bf2f(dcat[b + (int64_t)W * Ct]) + bf2f(dcat[b + (int64_t)W * Ct + Ct]));
+    dU[i] = f2bf(s);
+  }
+}
+
+__global__ void k_upcat_bwd_s(const bf16_t* __restrict__ dcat, int Cu, int Cs, int64_t P,
+                              bf16_t* __restrict__ dS, int accumulate) {
+  const int Ct = Cu + Cs;
+  const int64_t total = P * Cs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % Cs);
+    const int64_t p = i / Cs;
+    const bf16_t g = dcat[p * Ct + Cu + c];
+    dS[i] = accumulate ? f2bf(bf2f(dS[i]) + bf2f(g)) : g;
+  }
+}
+
+template <int C>
+__global__ void k_head_fwd(const bf16_t* __restrict__ Y, int64_t P, const float* __restrict__ w,
+                           const float* __restrict__ b, float* __restrict__ out) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < P;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    float s = b[0];
+#pragma unroll
+    for (int c = 0; c < C; ++c) s += bf2f(Y[p * C + c]) * w[c];
+    out[p] = s;
+  }
+}
+
+// Block-partial dw (C values) and db (1 value) per block, fixed order.
+constexpr int kHeadThreads = 256;
+template <int C>
+__global__ __launch_bounds__(kHeadThreads) void k_head_bwd(
+    const bf16_t* __restrict__ Y, const float* __restrict__ dout, int64_t P,
+    const float* __restrict__ w, bf16_t* __restrict__ dY, float* __restrict__ part) {
+  __shared__ float sh[kHeadThreads];
+  float acc[C + 1];
+#pragma unroll
+  for (int c = 0; c <= C; ++c) acc[c] = 0.f;
+  for (int64_t p = blockIdx.x * (int64_t)kHeadThreads + threadIdx.x; p < P;
+       p += (int64_t)gridDim.x * kHeadThreads) {
+    const float g = dout[p];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      dY[p * C + c] = f2bf(g * w[c]);
+      acc[c] += g * bf2f(Y[p * C + c]);
+    }
+    acc[C] += g;
+  }
+#pragma unroll
+  for (int c = 0; c <= C; ++c) {
+    sh[threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = kHeadThreads / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * (C + 1) + c] = sh[0];
+    __syncthreads();
+  }
+}
+
+constexpr int kMseThreads = 256;
+__global__ __launch_bounds__(kMseThreads) void k_mse(const float* __restrict__ out,
+                                                     const float* __restrict__ target, int64_t P,
+                                                     float* __restrict__ dout,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[kMseThreads];
+  double s = 0.0;
+  const float sc = 2.0f / (float)P;
+  for (int64_t p = blockIdx.x * (int64_t)kMseThreads + threadIdx.x; p < P;
+       p += (int64_t)gridDim.x * kMseThreads) {
+    const float d = out[p] - target[p];
+    s += (double)d * d;
+    dout[p] = sc * d;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kMseThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_pack_input(const float* __restrict__ in, int64_t P, bf16_t* __restrict__ X8) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < P;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    uint4 v = make_uint4((uint32_t)f2bf(in[p]), 0u, 0u, 0u);
+    *reinterpret_cast<uint4*>(X8 + p * 8) = v;
+  }
+}
+
+__global__ void k_add_bf16(bf16_t* __restrict__ dS, const bf16_t* __restrict__ dX, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dS[i] = f2bf(bf2f(dS[i]) + bf2f(dX[i]));
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, int64_t n, bf16_t* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = f2bf(x[i]);
+}
+
+// ===========================================================================
+// Launchers
+// ===========================================================================
+static dim3 grid1d(int64_t n, int threads = 256) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+hipError_t launch_conv3x3_fwd(const bf16_t* X, int N, int H, int W, int Cin, const bf16_t* Wt,
+                              const float* bias, int Cout, int relu, bf16_t* Y, hipStream_t s) {
+  if (Cin % 8 != 0 || Cout < 1 || N < 1 || H < 1 || W < 1) return hipErrorInvalidValue;
+  const int64_t NP = (int64_t)N * H * W;
+  constexpr int BM = 2;
+  if (Cout <= 32) {
+    dim3 grid((unsigned)((NP + 4 * 32 * BM - 1) / (4 * 32 * BM)), (unsigned)((Cout + 31) / 32));
+    hipLaunchKernelGGL((k_conv3x3_fwd<BM, 1>), grid, dim3(256), 0, s, X, N, H, W, Cin, Wt, bias,
+                       Cout, relu, Y);
+  } else {
+    dim3 grid((unsigned)((NP + 4 * 32 * BM - 1) / (4 * 32 * BM)), (unsigned)((Cout + 63) / 64));
+    hipLaunchKernelGGL((k_conv3x3_fwd<BM, 2>), grid, dim3(256), 0, s, X, N, H, W, Cin, Wt, bias,
+                       Cout, relu, Y);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_weight_rot(const bf16_t* Wt, int Cout, int Cin, bf16_t* Wr,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(k_conv_weight_rot, grid1d((int64_t)Cout * 9 * Cin), dim3(256), 0, s, Wt,
+                     Cout, Cin, Wr);
+  return hipGetLastError();
+}
+
+hipError_t launch_nhwc_to_nchw(const bf16_t* X, int N, int H, int W, int C, bf16_t* XT,
+                               hipStream_t s) {
+  if ((H * W) % 32 != 0) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((int64_t)N * H * W / 32), (unsigned)((C + 31) / 32));
+  hipLaunchKernelGGL((k_transpose_t<false>), grid, dim3(256), 0, s, const_cast<bf16_t*>(X),
+                     nullptr, N, H * W, C, XT);
+  return hipGetLastError();
+}
+
+hipError_t launch_relu_mask_t(bf16_t* dY, const bf16_t* Y, int N, int H, int W, int C,
+                              bf16_t* dYT, hipStream_t s) {
+  if ((H * W) % 32 != 0) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((int64_t)N * H * W / 32), (unsigned)((C + 31) / 32));
+  hipLaunchKernelGGL((k_transpose_t<true>), grid, dim3(256), 0, s, dY, Y, N, H * W, C, dYT);
+  return hipGetLastError();
+}
+
+hipError_t launch_conv3x3_wgrad(const bf16_t* dYT, const bf16_t* XT, int N, int H, int W,
+                                int Cin, int Cout, int splits, float* part, hipStream_t s) {
+  if (W % 16 != 0 || Cin % 8 != 0 || splits < 1) return hipErrorInvalidValue;
+  const int rows = N * H;
+  const int rps = (rows + splits - 1) / splits;
+  const int tiles = ((Cout + 31) / 32) * ((Cin + 31) / 32) * 9;
+  hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(tiles, splits), dim3(256), 0, s, dYT, XT, N, H, W,
+                     Cin, Cout, rps, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_splits(const float* part, int splits, int64_t count, float* out,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_splits, grid1d(count), dim3(256), 0, s, part, splits, count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_channel_sum(const bf16_t* dYT, int N, int HW, int C, float* db, hipStream_t s) {
+  hipLaunchKernelGGL(k_channel_sum, dim3(C), dim3(256), 0, s, dYT, N, HW, C, db);
+  return hipGetLastError();
+}
+
+hipError_t launch_maxpool_fwd(const bf16_t* X, int N, int H, int W, int C, bf16_t* Y,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_maxpool_fwd, grid1d((int64_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s,
+                     X, N, H, W, C, Y);
+  return hipGetLastError();
+}
+
+hipError_t launch_maxpool_bwd(const bf16_t* X, const bf16_t* dY, int N, int H, int W, int C,
+                              bf16_t* dX, hipStream_t s) {
+  hipLaunchKernelGGL(k_maxpool_bwd, grid1d((int64_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s,
+                     X, dY, N, H, W, C, dX);
+  return hipGetLastError();
+}
+
+hipError_t launch_upcat_fwd(const bf16_t* U, int Cu, const bf16_t* S, int Cs, int N, int H,
+                            int W, bf16_t* cat, hipStream_t s) {
+  hipLaunchKernelGGL(k_upcat_fwd, grid1d((int64_t)N * H * W * (Cu + Cs)), dim3(256), 0, s, U, Cu,
+                     S, Cs, N, H, W, cat);
+  return hipGetLastError();
+}
+
+hipError_t launch_upcat_bwd(const bf16_t* dcat, int Cu, int Cs, int N, int H, int W, bf16_t* dU,
+                            bf16_t* dS, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_upcat_bwd_u, grid1d((int64_t)N * (H / 2) * (W / 2) * Cu), dim3(256), 0, s,
+                     dcat, Cu, Cs, N, H, W, dU);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_upcat_bwd_s, grid1d((int64_t)N * H * W * Cs), dim3(256), 0, s, dcat, Cu,
+                     Cs, (int64_t)N * H * W, dS, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_head_fwd(const bf16_t* Y, int64_t P, int C, const float* w, const float* b,
+                           float* out, hipStream_t s) {
+  switch (C) {
+    case 16: hipLaunchKernelGGL(k_head_fwd<16>, grid1d(P), dim3(256), 0, s, Y, P, w, b, out); break;
+    case 32: hipLaunchKernelGGL(k_head_fwd<32>, grid1d(P), dim3(256), 0, s, Y, P, w, b, out); break;
+    case 64: hipLaunchKernelGGL(k_head_fwd<64>, grid1d(P), dim3(256), 0, s, Y, P, w, b, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_head_bwd(const bf16_t* Y, const float* dout, int64_t P, int C, const float* w,
+                           bf16_t* dY, float* part, int nblk, hipStream_t s) {
+  switch (C) {
+    case 16: hipLaunchKernelGGL(k_head_bwd<16>, dim3(nblk), dim3(kHeadThreads), 0, s, Y, dout, P, w, dY, part); break;
+    case 32: hipLaunchKernelGGL(k_head_bwd<32>, dim3(nblk), dim3(kHeadThreads), 0, s, Y, dout, P, w, dY, part); break;
+    case 64: hipLaunchKernelGGL(k_head_bwd<64>, dim3(nblk), dim3(kHeadThreads), 0, s, Y, dout, P, w, dY, part); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mse(const float* out, const float* target, int64_t P, float* dout,
+                      double* part, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_mse, dim3(nblk), dim3(kMseThreads), 0, s, out, target, P, dout, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_input(const float* in, int64_t P, bf16_t* X8, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_input, grid1d(P), dim3(256), 0, s, in, P, X8);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_bf16(bf16_t* dS, const bf16_t* dX, int64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_bf16, grid1d(count), dim3(256), 0, s, dS, dX, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t count, float lr,
+                       float b1, float b2, float eps, float c1, float c2, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, grid1d(count), dim3(256), 0, s, p, g, m, v, count, lr, b1, b2, eps,
+                     c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float* x, int64_t count, bf16_t* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_bf16, grid1d(count), dim3(256), 0, s, x, count, y);
+  return hipGetLastError();
+}
+
+}  // namespace huvk

@@ -119,8 +119,11 @@ class IltProblem:
         import torch
         return torch.sigmoid(self.config.sigmoid_steepness * self._dev(theta))
 
-    def run_stage(self, targets, theta, max_iterations):
-        """CG / SD with Armijo on every target; theta (torch, device) updated in place."""
+    def run_stage(self, targets, theta, max_iterations, stop_cost=None):
+        """CG / SD with Armijo on every target; theta (torch, device) updated in place.
+
+        stop_cost: optional per-target costs; a target stops as soon as its
+        cost is at or below its value (termination "ReachedCost")."""
         import torch
         tg, th = self._dev(targets), self._dev(theta)
         T = tg.shape[0]
@@ -130,15 +133,22 @@ class IltProblem:
         term = (C.c_int * T)()
         cfg = self.config
         self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
-        check(self.ctx.lib.huv_ilt_optimize(
-            self.handle, T, C.c_void_p(tg.data_ptr()), C.c_void_p(th.data_ptr()),
-            max_iterations, cfg.learning_rate, cfg.convergence_tol, int(cfg.optimizer == "cg"),
-            dptr(hist), iters, term))
+        if stop_cost is None:
+            check(self.ctx.lib.huv_ilt_optimize(
+                self.handle, T, C.c_void_p(tg.data_ptr()), C.c_void_p(th.data_ptr()),
+                max_iterations, cfg.learning_rate, cfg.convergence_tol,
+                int(cfg.optimizer == "cg"), dptr(hist), iters, term))
+        else:
+            stop = np.ascontiguousarray(np.broadcast_to(np.asarray(stop_cost, np.float64), (T,)))
+            check(self.ctx.lib.huv_ilt_optimize_to(
+                self.handle, T, C.c_void_p(tg.data_ptr()), C.c_void_p(th.data_ptr()),
+                max_iterations, cfg.learning_rate, cfg.convergence_tol,
+                int(cfg.optimizer == "cg"), dptr(stop), dptr(hist), iters, term))
         its = np.array(list(iters))
         return th, [hist[t, :its[t] + 1] for t in range(T)], its, np.array(list(term))
 
 
-TERMINATION = {0: "Converged", 1: "MaxIterations", 2: "LineSearchFailed"}
+TERMINATION = {0: "Converged", 1: "MaxIterations", 2: "LineSearchFailed", 3: "ReachedCost"}
 
 
 @dataclass

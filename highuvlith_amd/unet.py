@@ -1,0 +1,147 @@
+"""U-Net warm start for the GPU ILT (include/huv.h, huv_unet_*).
+
+The network maps a target pattern (N, H, W) to an initial theta for
+`optimize_ilt`; it is trained on the GPU on (target, ILT-converged theta)
+pairs produced by the GPU ILT itself, with BF16 weights/activations and FP32
+accumulation on the matrix cores.  The reference has no learned part: its ILT
+starts from the target (ilt.rs:495-522), which stays the fallback here.
+"""
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import HuvError, check
+
+
+class UNet:
+    def __init__(self, ctx, levels=4, base=32, seed=0):
+        self.ctx, self.levels, self.base = ctx, levels, base
+        h = C.c_void_p()
+        check(ctx.lib.huv_unet_create(ctx.handle, levels, base, C.c_uint64(seed), C.byref(h)))
+        self.handle = h
+        n = C.c_int64()
+        check(ctx.lib.huv_unet_num_params(h, C.byref(n)))
+        self.num_params = n.value
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                self.ctx.lib.huv_unet_destroy(self.handle)
+        except Exception:
+            pass
+        self.handle = None
+
+    def get_params(self):
+        out = np.empty(self.num_params, np.float32)
+        check(self.ctx.lib.huv_unet_get_params(self.handle, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.num_params:
+            raise HuvError(2, f"expected {self.num_params} parameters, got {p.size}")
+        check(self.ctx.lib.huv_unet_set_params(self.handle, p.ctypes.data_as(C.c_void_p)))
+
+    @staticmethod
+    def _dev(x):
+        import torch
+        if not isinstance(x, torch.Tensor):
+            x = torch.as_tensor(np.ascontiguousarray(x, np.float32), device="cuda")
+        x = x.to(torch.float32).contiguous()
+        return x[None] if x.dim() == 2 else x
+
+    def forward(self, x):
+        """theta_0 = net(target) for a batch (torch CUDA or numpy (N, H, W))."""
+        import torch
+        x = self._dev(x)
+        out = torch.empty_like(x)
+        self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        N, H, W = x.shape
+        check(self.ctx.lib.huv_unet_forward(self.handle, N, H, W, C.c_void_p(x.data_ptr()),
+                                            C.c_void_p(out.data_ptr())))
+        return out
+
+    def loss_and_grad(self, x, target, want_grad=False):
+        import torch
+        x, t = self._dev(x), self._dev(target)
+        self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        N, H, W = x.shape
+        loss = C.c_double()
+        g = np.empty(self.num_params, np.float32) if want_grad else None
+        check(self.ctx.lib.huv_unet_loss_grad(
+            self.handle, N, H, W, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+            C.byref(loss), g.ctypes.data_as(C.c_void_p) if want_grad else None))
+        return (loss.value, g) if want_grad else loss.value
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        check(self.ctx.lib.huv_unet_adam(self.handle, lr, beta1, beta2, eps))
+
+    def train(self, inputs, targets, epochs=20, batch=8, lr=1e-3, seed=0, log=None):
+        """Adam on shuffled mini-batches; inputs/targets torch CUDA (S, H, W).
+        Returns the mean loss per epoch."""
+        import torch
+        rng = np.random.default_rng(seed)
+        S = inputs.shape[0]
+        hist = []
+        for ep in range(epochs):
+            order = rng.permutation(S)
+            tot, cnt = 0.0, 0
+            for i in range(0, S - batch + 1, batch):
+                idx = torch.as_tensor(order[i:i + batch], device=inputs.device)
+                loss = self.loss_and_grad(inputs.index_select(0, idx), targets.index_select(0, idx))
+                self.adam_step(lr)
+                tot += loss
+                cnt += 1
+            hist.append(tot / max(cnt, 1))
+            if log:
+                log(f"epoch {ep}: loss {hist[-1]:.5f}")
+        return hist
+
+
+@dataclass
+class WarmStartResult:
+    plain_cost: np.ndarray       # plain ILT cost after the fixed budget (the threshold)
+    plain_wall_s: float
+    plain_iterations: np.ndarray
+    warm_cost: np.ndarray        # warm-started cost at termination
+    warm_wall_s: float           # includes network inference
+    warm_iterations: np.ndarray
+    reached: np.ndarray          # warm start reached the plain cost
+    fallback: np.ndarray         # warm start worse than the target at iteration 0
+
+
+def compare_warm_start(engine, net, targets, config, budget=200):
+    """Plain ILT from the target for `budget` iterations vs network warm start
+    refined until each target reaches the plain cost (or the same budget).
+    Targets whose warm start is worse than the target initialisation fall
+    back to the target initialisation (counted in `fallback`)."""
+    import torch
+    from .ilt import IltProblem
+    prob = IltProblem(engine, config)
+    tg = torch.as_tensor(np.ascontiguousarray(targets, np.float32), device="cuda")
+    theta_plain = prob.initial_theta(tg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, hist_p, its_p, _ = prob.run_stage(tg, theta_plain, budget)
+    torch.cuda.synchronize()
+    plain_wall = time.perf_counter() - t0
+    plain_cost = np.array([h[-1] for h in hist_p])
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    theta_w = net.forward(tg)
+    c_warm = prob.cost(tg, theta_w)
+    c_tgt = prob.cost(tg, prob.initial_theta(tg))
+    fallback = c_warm > c_tgt
+    if fallback.any():
+        sel = torch.as_tensor(fallback, device="cuda")
+        theta_w[sel] = prob.initial_theta(tg)[sel]
+    _, hist_w, its_w, _ = prob.run_stage(tg, theta_w, budget, stop_cost=plain_cost)
+    torch.cuda.synchronize()
+    warm_wall = time.perf_counter() - t0
+    warm_cost = np.array([h[-1] for h in hist_w])
+    return WarmStartResult(plain_cost, plain_wall, np.asarray(its_p), warm_cost, warm_wall,
+                           np.asarray(its_w), warm_cost <= plain_cost, fallback)

@@ -278,6 +278,42 @@ huv_status huv_ilt_cost_grad(huv_ilt* ilt, int ntargets, const float* targets, c
 huv_status huv_ilt_optimize(huv_ilt* ilt, int ntargets, const float* targets, float* theta,
                             int max_iterations, double learning_rate, double convergence_tol,
                             int use_cg, double* cost_hist, int* iterations, int* termination);
+/* Same, ending each target as soon as its cost is <= stop_cost[t] (host,
+ * ntargets values; termination 3): refinement of a warm start to a fixed
+ * cost.  No counterpart in the reference; for the warm-started ILT. */
+huv_status huv_ilt_optimize_to(huv_ilt* ilt, int ntargets, const float* targets, float* theta,
+                               int max_iterations, double learning_rate, double convergence_tol,
+                               int use_cg, const double* stop_cost, double* cost_hist,
+                               int* iterations, int* termination);
+
+/* ======================================================================== */
+/* U-Net warm start for ILT                                                  */
+/* ======================================================================== */
+
+/* No counterpart in the reference (its ILT starts from the target,
+ * ilt.rs:495-522): a 2-D U-Net mapping a target pattern to an initial theta.
+ * `levels` resolution levels with base * 2^l channels, two 3x3 conv + ReLU per
+ * encoder level and decoder stage, 2x2 max pooling, nearest 2x upsampling
+ * with skip concatenation, 1x1 FP32 head.  BF16 weights and activations, FP32
+ * accumulation on the matrix cores, FP32 master parameters and Adam.
+ * Inputs and outputs are DEVICE FP32 [N][H][W]; H and W must be multiples of
+ * 16 * 2^(levels-1).  Parameters are one flat FP32 vector (per conv: weights
+ * [cout][3][3][cin] then bias [cout]; encoder levels, then decoder stages from
+ * the deepest; head weights [base] then bias). */
+typedef struct huv_unet huv_unet;
+huv_status huv_unet_create(huv_ctx* ctx, int levels, int base_channels, uint64_t seed,
+                           huv_unet** out);
+void huv_unet_destroy(huv_unet* net);
+huv_status huv_unet_num_params(const huv_unet* net, int64_t* count);
+huv_status huv_unet_get_params(huv_unet* net, float* host);
+huv_status huv_unet_set_params(huv_unet* net, const float* host);
+huv_status huv_unet_forward(huv_unet* net, int n, int h, int w, const float* in, float* out);
+/* Mean-squared error to `target` (host double) and its gradient, kept for
+ * huv_unet_adam and copied to grad_host (nparams floats) when not NULL. */
+huv_status huv_unet_loss_grad(huv_unet* net, int n, int h, int w, const float* in,
+                              const float* target, double* loss, float* grad_host);
+/* One Adam step with the last computed gradient (bias-corrected). */
+huv_status huv_unet_adam(huv_unet* net, double lr, double beta1, double beta2, double eps);
 
 #ifdef __cplusplus
 }
