@@ -8,8 +8,8 @@ sectors sigma 0.7-0.9 with a 35 deg opening, 5 focus planes
 (-60/-30/0/+30/+60 nm), K = 32 kernels per plane built on the host in FP64
 (2560 images per step).
 
-Also reported in the same JSON line (each skipped with --no-uneven /
---no-ilt):
+Also reported in the same JSON line (each skipped with --no-uneven,
+--no-ilt, --no-warm):
   uneven   1000 clips with edges {256, 512, 1024, 2048} px (p .4/.3/.2/.1),
            30-60 % clear, ArF 193i annular 0.7/0.9 at 2 nm, K = 32, one graph
            launch per batch, autotuned launch shapes; aggregate Mpix/s and
@@ -17,6 +17,14 @@ Also reported in the same JSON line (each skipped with --no-uneven /
            eager pass, groups issued smallest first).
   ilt      64 contact-array targets 512^2 at 2 nm, ArF 193i quasar, K = 24,
            resist cost (t 0.25, alpha 50), 50 CG/Armijo iterations; clips/s.
+  warm     U-Net warm start (4 levels, 32..256 channels, BF16 on the matrix
+           cores): trained on (target, 200-iteration ILT theta) pairs from
+           the GPU ILT (seeded contact and line/space targets, 512^2 at 2 nm),
+           then on held-out targets plain ILT (200 iterations from the
+           target) vs warm start refined to the same per-target cost;
+           clips/s to that fixed cost for both.  The default sizes are a
+           reduced version of the 20,000-pair / 20-epoch training
+           (--warm-pairs, --warm-epochs, --warm-eval).
 
 `roofline`: the dominant kernel (socs_fields) against the FP32 peak with
 algorithmic flops per launch over its HIP-event duration; `traffic` is the
@@ -272,7 +280,45 @@ def bench_ilt(args, H, W, np, torch):
             "mean_iterations": float(np.mean(res.iterations)),
             "mean_initial_cost": float(np.mean([h[0] for h in res.cost_history])),
             "mean_final_cost": float(np.mean(res.final_cost)),
-            "warm_start": "not built (plain ILT from the target only)"}
+            }
+
+
+def bench_warm(args, H, W, np, torch):
+    from highuvlith_amd import ilt as L
+    from highuvlith_amd import unet as UN
+    from highuvlith_amd import warmstart as WS
+    src, opt, grid, K = W.arf_quasar_512_setup()
+    eng = H.AerialImageEngine(src, opt, grid, max_kernels=K)
+    cfg = L.ILTConfig(cost="resist", threshold=0.25, steepness=50.0, regularization=0.0)
+    n = grid.size
+    train = WS.contact_line_targets(args.warm_pairs, n, grid.pixel_nm, seed=500)
+    held = WS.contact_line_targets(args.warm_eval, n, grid.pixel_nm, seed=900)
+    labels, lab_s = WS.ilt_labels(eng, train, cfg, iterations=200)
+    net = UN.UNet(eng.context(), levels=4, base=32, seed=7)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.train(train, labels, epochs=args.warm_epochs, batch=8, lr=1e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    res = UN.compare_warm_start(eng, net, held.cpu().numpy(), cfg, budget=200)
+    T = args.warm_eval
+    steps = args.warm_epochs * (args.warm_pairs // 8)
+    return {"workload": f"U-Net 4 levels 32/64/128/256 ch, {args.warm_pairs} training pairs "
+                        f"(512x512 @ 2 nm, 200-iteration ILT labels), {args.warm_epochs} epochs "
+                        f"Adam lr 1e-3 batch 8; {T} held-out targets",
+            "plain_clips_per_s": round(T / res.plain_wall_s, 3),
+            "warm_clips_per_s": round(T / res.warm_wall_s, 3),
+            "unit": "clips/s to the plain 200-iteration cost",
+            "warm_over_plain_wall": round(res.warm_wall_s / res.plain_wall_s, 4),
+            "plain_mean_iterations": float(np.mean(res.plain_iterations)),
+            "warm_mean_iterations": float(np.mean(res.warm_iterations)),
+            "reached_fraction": float(np.mean(res.reached)),
+            "fallback_count": int(np.sum(res.fallback)),
+            "mean_plain_cost": float(np.mean(res.plain_cost)),
+            "mean_warm_cost": float(np.mean(res.warm_cost)),
+            "label_generation_s": round(lab_s, 2), "train_s": round(train_s, 2),
+            "train_samples_per_s": round(steps * 8 / train_s, 1),
+            "train_loss_first_last": [round(hist[0], 5), round(hist[-1], 5)]}
 
 
 def main():
@@ -286,6 +332,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-ilt", action="store_true")
+    ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--warm-pairs", type=int, default=1024)
+    ap.add_argument("--warm-epochs", type=int, default=20)
+    ap.add_argument("--warm-eval", type=int, default=64)
     ap.add_argument("--uneven-clips", type=int, default=1000)
     ap.add_argument("--ilt-targets", type=int, default=64)
     ap.add_argument("--ilt-iterations", type=int, default=50)
@@ -308,7 +358,8 @@ def main():
     result = bench_fixed(args, H, W, np, torch)
     print("fixed batch done", file=sys.stderr, flush=True)
     for name, fn, skip in (("uneven", bench_uneven, args.no_uneven),
-                           ("ilt", bench_ilt, args.no_ilt)):
+                           ("ilt", bench_ilt, args.no_ilt),
+                           ("warm", bench_warm, args.no_warm)):
         if skip:
             continue
         try:

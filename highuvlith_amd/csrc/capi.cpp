@@ -745,12 +745,13 @@ huv_status autotune_socs(Ctx* c, int nclips, int n, const float* masks, int npla
     ms = bestms;
     return HUV_OK;
   };
-  huv_socs_tuning cur{planes[0]->m, 0, 0};
+  // explicit values throughout (0 would mean "default" in the table)
+  huv_socs_tuning cur{planes[0]->m, huvk::band_fft_default(n), huvk::kFieldsXnbDefault};
   double cur_ms = 1e30;
   const int B = planes[0]->B;
   for (int m : {32, 64, 128, 192, 256}) {
     if (m < 4 * B + 1 || m > n) continue;
-    huv_socs_tuning t{m, 0, 0};
+    huv_socs_tuning t{m, cur.n1, cur.xnb};
     double ms;
     st = time_one(t, ms);
     if (st != HUV_OK) break;

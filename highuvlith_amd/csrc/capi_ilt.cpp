@@ -260,13 +260,48 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     if (e == hipSuccess) e = hipMemcpyAsync(coefB, h_b.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
     return e == hipSuccess;
   };
-  auto eval = [&](const float* th, float* g, std::vector<double>& out) -> huv_status {
-    huv_status r = ilt_eval(p, T, targets, th, g, b);
+  // Targets that are done cost nothing: evaluations gather the rows of the
+  // targets in `which` into compact buffers (all targets: no copies).
+  char* cmp = (char*)c->ws.get(Workspace::ILT8, 3 * vb + T * sizeof(int), e);
+  if (!cmp) return hip_fail(e, "ILT optimizer workspace");
+  float* th_c = (float*)cmp;
+  float* tg_c = (float*)(cmp + vb);
+  float* g_c = (float*)(cmp + 2 * vb);
+  int* idx_d = (int*)(cmp + 3 * vb);
+  std::vector<double> cost_c(T);
+  auto eval_sel = [&](const float* th, float* g, std::vector<double>& out,
+                      const std::vector<int>& which) -> huv_status {
+    const int Tc = (int)which.size();
+    if (Tc == 0) return HUV_OK;
+    if (Tc == T) {
+      huv_status r = ilt_eval(p, T, targets, th, g, b);
+      if (r != HUV_OK) return r;
+      e = hipMemcpyAsync(out.data(), b.cost_d, T * sizeof(double), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      return e == hipSuccess ? HUV_OK : hip_fail(e, "cost copy");
+    }
+    e = hipMemcpyAsync(idx_d, which.data(), Tc * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = huvk::launch_rows_gather(th, idx_d, N, Tc, th_c, s);
+    if (e == hipSuccess) e = huvk::launch_rows_gather(targets, idx_d, N, Tc, tg_c, s);
+    if (e != hipSuccess) return hip_fail(e, "target gather");
+    huv_status r = ilt_eval(p, Tc, tg_c, th_c, g ? g_c : nullptr, b);
     if (r != HUV_OK) return r;
-    e = hipMemcpyAsync(out.data(), b.cost_d, T * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (g) {
+      e = huvk::launch_rows_scatter(g_c, idx_d, N, Tc, g, s);
+      if (e != hipSuccess) return hip_fail(e, "gradient scatter");
+    }
+    e = hipMemcpyAsync(cost_c.data(), b.cost_d, Tc * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    return e == hipSuccess ? HUV_OK : hip_fail(e, "cost copy");
+    if (e != hipSuccess) return hip_fail(e, "cost copy");
+    for (int j = 0; j < Tc; ++j) out[which[j]] = cost_c[j];
+    return HUV_OK;
   };
+  std::vector<int> all_t(T);
+  for (int t = 0; t < T; ++t) all_t[t] = t;
+  auto eval = [&](const float* th, float* g, std::vector<double>& out) -> huv_status {
+    return eval_sel(th, g, out, all_t);
+  };
+  std::vector<int> which;
 
   const int H = max_iterations + 1;
   std::vector<double> cost(T), ctrial(T), cnew(T), step(T, learning_rate), slope(T), dmax(T),
@@ -337,7 +372,10 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         if (e != hipSuccess) return fail_e("coef upload");
         e = huvk::launch_vec_axpy(theta, dir, coefA, N, T, trial, s);
         if (e != hipSuccess) return fail_e("axpy");
-        st = eval(trial, nullptr, ctrial);
+        which.clear();
+        for (int t = 0; t < T; ++t)
+          if (searching[t]) which.push_back(t);
+        st = eval_sel(trial, nullptr, ctrial, which);
         if (st != HUV_OK) return st;
         for (int t = 0; t < T; ++t) h_sel[t] = 0;
         bool take = false;
@@ -391,7 +429,10 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       }
     }
     if (!any_acc) break;
-    st = eval(theta, gnew, cnew);
+    which.clear();
+    for (int t = 0; t < T; ++t)
+      if (active[t] && accepted[t]) which.push_back(t);
+    st = eval_sel(theta, gnew, cnew, which);
     if (st != HUV_OK) return st;
     // CG update: beta = max(0, (g'.g' - g'.g) / g.g)
     std::vector<double> gg(T), gng(T), gngn(T);

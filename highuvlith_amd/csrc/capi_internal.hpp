@@ -19,7 +19,7 @@ huv_status hip_fail(hipError_t e, const char* what);
 // Device scratch owned by a context, grown on demand and reused across calls.
 struct Workspace {
   enum Slot { TMP_ROWS = 0, BAND, COARSE, TMP_COARSE, IBAND, ILT0, ILT1, ILT2, ILT3, ILT4, ILT5,
-              ILT6, ILT7, kSlots };
+              ILT6, ILT7, ILT8, kSlots };
   void* ptr[kSlots] = {};
   size_t size[kSlots] = {};
   uint64_t generation = 0;  // bumped whenever a slot is reallocated

@@ -397,6 +397,21 @@ __global__ void k_vec_select(const float* __restrict__ src, const int* __restric
   }
 }
 
+// Row gather / scatter of per-target images: dst[j] = src[idx[j]] or
+// dst[idx[j]] = src[j], j < nrows (rows of npix floats).
+template <bool SCATTER>
+__global__ void k_rows_move(const float* __restrict__ src, const int* __restrict__ idx,
+                            int64_t npix, int nrows, float* __restrict__ dst) {
+  const int64_t total = npix * nrows;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = i / npix, k = i - j * npix;
+    const int64_t r = idx[j];
+    if (SCATTER) dst[r * npix + k] = src[i];
+    else dst[i] = src[r * npix + k];
+  }
+}
+
 // ===========================================================================
 // Launchers
 // ===========================================================================
@@ -497,6 +512,20 @@ hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int6
                           int ntargets, float* d, hipStream_t s) {
   hipLaunchKernelGGL(k_vec_dir, dim3(grid_for(npix * ntargets, 256)), dim3(256), 0, s, g, a, b,
                      npix, ntargets, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_gather(const float* src, const int* idx, int64_t npix, int nrows,
+                              float* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_rows_move<false>, dim3(grid_for(npix * nrows, 256)), dim3(256), 0, s, src,
+                     idx, npix, nrows, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_scatter(const float* src, const int* idx, int64_t npix, int nrows,
+                               float* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_rows_move<true>, dim3(grid_for(npix * nrows, 256)), dim3(256), 0, s, src,
+                     idx, npix, nrows, dst);
   return hipGetLastError();
 }
 

@@ -33,6 +33,11 @@ hipError_t launch_vec_axpy(const float* x, const float* d, const double* coef, i
                            int ntargets, float* y, hipStream_t s);
 hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int64_t npix,
                           int ntargets, float* d, hipStream_t s);
+// dst[j] = src[idx[j]] (gather) / dst[idx[j]] = src[j] (scatter), rows of npix.
+hipError_t launch_rows_gather(const float* src, const int* idx, int64_t npix, int nrows,
+                              float* dst, hipStream_t s);
+hipError_t launch_rows_scatter(const float* src, const int* idx, int64_t npix, int nrows,
+                               float* dst, hipStream_t s);
 hipError_t launch_vec_select(const float* src, const int* sel, int64_t npix, int ntargets,
                              float* dst, hipStream_t s);
 
