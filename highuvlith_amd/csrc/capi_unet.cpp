@@ -90,11 +90,21 @@ huv_status alloc_t(Acts& a, T** p, size_t count) {
 }
 
 int wgrad_splits(const Conv& c, int N, int H) {
-  const int tiles = ((c.cout + 31) / 32) * ((c.cin + 31) / 32) * 9;
+  const int cob = (c.cout + 31) / 32;
+  const int tiles = (cob >= 2 ? (cob + 1) / 2 : cob) * ((c.cin + 31) / 32) * 3;
   const int rows = N * H;
   int s = (1024 + tiles - 1) / tiles;
   s = std::min(s, std::max(1, rows / 4));
   return std::max(1, s);
+}
+
+// Chunks of the bias-gradient reduction: about 2048 work-groups in total,
+// at least 4096 pixels each.
+int bias_splits(const Conv& c, int N, int H, int W) {
+  const int64_t total = (int64_t)N * H * W;
+  int64_t s = std::max<int64_t>(1, 2048 / c.cout);
+  s = std::min<int64_t>(s, std::max<int64_t>(1, total / 4096));
+  return (int)s;
 }
 
 huv_status ensure_acts(huv_unet* u, int N, int H, int W) {
@@ -119,6 +129,7 @@ huv_status ensure_acts(huv_unet* u, int N, int H, int W) {
   auto note_conv = [&](const Conv& c, int l) {
     const int Hl = H >> l;
     part = std::max(part, (size_t)wgrad_splits(c, N, Hl) * c.cout * 9 * c.cin);
+    part = std::max(part, (size_t)bias_splits(c, N, Hl, W >> l) * c.cout);
   };
   for (int l = 0; l < L; ++l) {
     const int64_t Pl = P >> (2 * l);
@@ -209,7 +220,9 @@ huv_status conv_bwd(huv_unet* u, const Conv& c, const bf16_t* x, const bf16_t* y
        "U-Net weight gradient");
   HCHK(huvk::launch_sum_splits(a.part, splits, (int64_t)c.cout * 9 * c.cin, u->g + c.w_off, S(u)),
        "U-Net split sum");
-  HCHK(huvk::launch_channel_sum(a.dyt, a.N, H * W, c.cout, u->g + c.b_off, S(u)),
+  // bias partials reuse the (already consumed) weight-gradient split buffer
+  const int bsplits = bias_splits(c, a.N, H, W);
+  HCHK(huvk::launch_channel_sum(a.dyt, a.N, H * W, c.cout, a.part, bsplits, u->g + c.b_off, S(u)),
        "U-Net bias gradient");
   if (dx)
     HCHK(huvk::launch_conv3x3_fwd(dy, a.N, H, W, c.cout, wrot(u, c), nullptr, c.cin, 0, dx, S(u)),

@@ -178,10 +178,15 @@ __global__ __launch_bounds__(256) void k_transpose_t(bf16_t* __restrict__ dY,
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient (split K over rows).  Grid x = co blocks * ci blocks * 9
-// taps, grid y = splits; the 4 waves take interleaved rows of the split and
-// are reduced through LDS in a fixed order.
+// Weight gradient (split K over rows).  Grid x = co-block groups (BM blocks
+// of 32) * ci blocks * 3 kernel rows kh, grid y = splits.  Each wave loads a
+// 16-pixel segment of dY (per co block) and of the input row y + kh - 1 once
+// and feeds all three column taps kw from it (the -1 / +1 shifts are 2-byte
+// funnel shifts), i.e. 3 BM products per BM + 1 operand loads.  The 4 waves
+// take interleaved rows of the split and are reduced through LDS in a fixed
+// order, one output tile at a time.
 // ---------------------------------------------------------------------------
+template <int BM>
 __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16_t* __restrict__ dYT,
                                                        const bf16_t* __restrict__ XT, int N,
                                                        int H, int W, int Cin, int Cout,
@@ -192,58 +197,86 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16_t* __restrict_
   const int half = lane >> 5, li = lane & 31;
   const int ciblocks = (Cin + 31) / 32;
   const int tile = blockIdx.x;
-  const int tap = tile % 9, rest = tile / 9;
-  const int cib = rest % ciblocks, cob = rest / ciblocks;
-  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const int kh = tile % 3, rest = tile / 3;
+  const int cib = rest % ciblocks, cog = rest / ciblocks;
+  const int dy = kh - 1;
   const int split = blockIdx.y;
   const int rows = N * H;
   const int r0 = split * rows_per_split;
   const int r1 = min(rows, r0 + rows_per_split);
-  const int coA = cob * 32 + li, ciB = cib * 32 + li;
-  const bool av = coA < Cout, bv = ciB < Cin;
-  f32x16 acc;
+  int coA[BM];
+  bool av[BM];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  for (int bm = 0; bm < BM; ++bm) {
+    coA[bm] = (cog * BM + bm) * 32 + li;
+    av[bm] = coA[bm] < Cout;
+  }
+  const int ciB = cib * 32 + li;
+  const bool bv = ciB < Cin;
+  f32x16 acc[BM][3];
+#pragma unroll
+  for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[bm][k][r] = 0.f;
   for (int row = r0 + wave; row < r1; row += 4) {
     const int n = row / H, y = row - n * H;
     const int yy = y + dy;
     const bool rv = bv && yy >= 0 && yy < H;
-    const bf16_t* arow = dYT + (((int64_t)n * Cout + (av ? coA : 0)) * H + y) * W;
+    const bf16_t* arow[BM];
+#pragma unroll
+    for (int bm = 0; bm < BM; ++bm)
+      arow[bm] = dYT + (((int64_t)n * Cout + (av[bm] ? coA[bm] : 0)) * H + y) * W;
     const bf16_t* brow = XT + (((int64_t)n * Cin + (bv ? ciB : 0)) * H + (rv ? yy : 0)) * W;
     for (int x0 = 0; x0 < W; x0 += 16) {
       const int xs = x0 + 8 * half;
-      const bf16x8 a = av ? ld8(arow + xs) : zero8();
-      bf16x8 b = zero8();
+      bf16x8 a[BM];
+#pragma unroll
+      for (int bm = 0; bm < BM; ++bm) a[bm] = av[bm] ? ld8(arow[bm] + xs) : zero8();
+      uint4 w = make_uint4(0, 0, 0, 0);
+      uint32_t prev = 0u, next = 0u;
       if (rv) {
-        uint4 w = *reinterpret_cast<const uint4*>(brow + xs);
-        if (dx < 0) {
-          const uint32_t prev = xs > 0 ? *reinterpret_cast<const uint32_t*>(brow + xs - 2) : 0u;
-          w = make_uint4(__builtin_amdgcn_alignbit(w.x, prev, 16),
-                         __builtin_amdgcn_alignbit(w.y, w.x, 16),
-                         __builtin_amdgcn_alignbit(w.z, w.y, 16),
-                         __builtin_amdgcn_alignbit(w.w, w.z, 16));
-        } else if (dx > 0) {
-          const uint32_t next = xs + 8 < W ? *reinterpret_cast<const uint32_t*>(brow + xs + 8) : 0u;
-          w = make_uint4(__builtin_amdgcn_alignbit(w.y, w.x, 16),
-                         __builtin_amdgcn_alignbit(w.z, w.y, 16),
-                         __builtin_amdgcn_alignbit(w.w, w.z, 16),
-                         __builtin_amdgcn_alignbit(next, w.w, 16));
-        }
-        b = __builtin_bit_cast(bf16x8, w);
+        w = *reinterpret_cast<const uint4*>(brow + xs);
+        if (xs > 0) prev = *reinterpret_cast<const uint32_t*>(brow + xs - 2);
+        if (xs + 8 < W) next = *reinterpret_cast<const uint32_t*>(brow + xs + 8);
       }
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      // kw = 0: input column x - 1; kw = 1: x; kw = 2: x + 1
+      const bf16x8 b0 = __builtin_bit_cast(
+          bf16x8, make_uint4(__builtin_amdgcn_alignbit(w.x, prev, 16),
+                             __builtin_amdgcn_alignbit(w.y, w.x, 16),
+                             __builtin_amdgcn_alignbit(w.z, w.y, 16),
+                             __builtin_amdgcn_alignbit(w.w, w.z, 16)));
+      const bf16x8 b1 = __builtin_bit_cast(bf16x8, w);
+      const bf16x8 b2 = __builtin_bit_cast(
+          bf16x8, make_uint4(__builtin_amdgcn_alignbit(w.y, w.x, 16),
+                             __builtin_amdgcn_alignbit(w.z, w.y, 16),
+                             __builtin_amdgcn_alignbit(w.w, w.z, 16),
+                             __builtin_amdgcn_alignbit(next, w.w, 16)));
+#pragma unroll
+      for (int bm = 0; bm < BM; ++bm) {
+        acc[bm][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bm], b0, acc[bm][0], 0, 0, 0);
+        acc[bm][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bm], b1, acc[bm][1], 0, 0, 0);
+        acc[bm][2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bm], b2, acc[bm][2], 0, 0, 0);
+      }
     }
   }
 #pragma unroll
-  for (int r = 0; r < 16; ++r) red[wave][8 * (r / 4) + 4 * half + (r % 4)][li] = acc[r];
-  __syncthreads();
-  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
-    const int i = e / 32, j = e % 32;
-    const float v = ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
-    const int co = cob * 32 + i, ci = cib * 32 + j;
-    if (co < Cout && ci < Cin)
-      part[(((int64_t)split * Cout + co) * 9 + tap) * Cin + ci] = v;
-  }
+  for (int bm = 0; bm < BM; ++bm)
+#pragma unroll
+    for (int kw = 0; kw < 3; ++kw) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) red[wave][8 * (r / 4) + 4 * half + (r % 4)][li] = acc[bm][kw][r];
+      __syncthreads();
+      for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+        const int i = e / 32, j = e % 32;
+        const float v = ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
+        const int co = (cog * BM + bm) * 32 + i, ci = cib * 32 + j;
+        if (co < Cout && ci < Cin)
+          part[(((int64_t)split * Cout + co) * 9 + kh * 3 + kw) * Cin + ci] = v;
+      }
+      __syncthreads();
+    }
 }
 
 __global__ void k_sum_splits(const float* __restrict__ part, int splits, int64_t count,
@@ -256,15 +289,19 @@ __global__ void k_sum_splits(const float* __restrict__ part, int splits, int64_t
   }
 }
 
-// db[c] = sum_n sum_hw dYT[n][c][hw]: one work-group per channel, fixed order.
-__global__ __launch_bounds__(256) void k_channel_sum(const bf16_t* __restrict__ dYT, int N,
-                                                     int HW, int C, float* __restrict__ db) {
+// part[s][c] = sum of dYT[n][c][hw] over the s-th of `splits` contiguous
+// chunks of the flat (n, hw) range (fixed order within a chunk).
+__global__ __launch_bounds__(256) void k_channel_partial(const bf16_t* __restrict__ dYT, int N,
+                                                         int HW, int C, int64_t chunk,
+                                                         float* __restrict__ part) {
   __shared__ float sh[256];
-  const int c = blockIdx.x;
+  const int c = blockIdx.x, sp = blockIdx.y;
+  const int64_t total = (int64_t)N * HW;
+  const int64_t i0 = (int64_t)sp * chunk, i1 = min(total, i0 + chunk);
   float s = 0.f;
-  for (int n = 0; n < N; ++n) {
-    const bf16_t* p = dYT + ((int64_t)n * C + c) * HW;
-    for (int i = threadIdx.x; i < HW; i += 256) s += bf2f(p[i]);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    const int64_t n = i / HW, hw = i - n * HW;
+    s += bf2f(dYT[(n * C + c) * HW + hw]);
   }
   sh[threadIdx.x] = s;
   __syncthreads();
@@ -272,7 +309,7 @@ __global__ __launch_bounds__(256) void k_channel_sum(const bf16_t* __restrict__ 
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) db[c] = sh[0];
+  if (threadIdx.x == 0) part[(int64_t)sp * C + c] = sh[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -521,9 +558,16 @@ hipError_t launch_conv3x3_wgrad(const bf16_t* dYT, const bf16_t* XT, int N, int 
   if (W % 16 != 0 || Cin % 8 != 0 || splits < 1) return hipErrorInvalidValue;
   const int rows = N * H;
   const int rps = (rows + splits - 1) / splits;
-  const int tiles = ((Cout + 31) / 32) * ((Cin + 31) / 32) * 9;
-  hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(tiles, splits), dim3(256), 0, s, dYT, XT, N, H, W,
-                     Cin, Cout, rps, part);
+  const int coblocks = (Cout + 31) / 32, ciblocks = (Cin + 31) / 32;
+  if (coblocks >= 2) {
+    const int tiles = ((coblocks + 1) / 2) * ciblocks * 3;
+    hipLaunchKernelGGL(k_conv3x3_wgrad<2>, dim3(tiles, splits), dim3(256), 0, s, dYT, XT, N, H,
+                       W, Cin, Cout, rps, part);
+  } else {
+    const int tiles = coblocks * ciblocks * 3;
+    hipLaunchKernelGGL(k_conv3x3_wgrad<1>, dim3(tiles, splits), dim3(256), 0, s, dYT, XT, N, H,
+                       W, Cin, Cout, rps, part);
+  }
   return hipGetLastError();
 }
 
@@ -533,9 +577,15 @@ hipError_t launch_sum_splits(const float* part, int splits, int64_t count, float
   return hipGetLastError();
 }
 
-hipError_t launch_channel_sum(const bf16_t* dYT, int N, int HW, int C, float* db, hipStream_t s) {
-  hipLaunchKernelGGL(k_channel_sum, dim3(C), dim3(256), 0, s, dYT, N, HW, C, db);
-  return hipGetLastError();
+hipError_t launch_channel_sum(const bf16_t* dYT, int N, int HW, int C, float* part, int splits,
+                              float* db, hipStream_t s) {
+  const int64_t total = (int64_t)N * HW;
+  const int64_t chunk = (total + splits - 1) / splits;
+  hipLaunchKernelGGL(k_channel_partial, dim3(C, splits), dim3(256), 0, s, dYT, N, HW, C, chunk,
+                     part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_sum_splits(part, splits, C, db, s);
 }
 
 hipError_t launch_maxpool_fwd(const bf16_t* X, int N, int H, int W, int C, bf16_t* Y,

@@ -31,8 +31,10 @@ hipError_t launch_conv3x3_wgrad(const bf16_t* dYT, const bf16_t* XT, int N, int 
 // out[i] = sum_s part[s][i] (fixed order), i < count.
 hipError_t launch_sum_splits(const float* part, int splits, int64_t count, float* out,
                              hipStream_t s);
-// db[c] = sum over n, pixels of dYT[n][c][.]  (NCHW, fixed order).
-hipError_t launch_channel_sum(const bf16_t* dYT, int N, int HW, int C, float* db, hipStream_t s);
+// db[c] = sum over n, pixels of dYT[n][c][.]  (NCHW): `splits` chunk
+// partials in part (splits x C floats), then summed in a fixed order.
+hipError_t launch_channel_sum(const bf16_t* dYT, int N, int HW, int C, float* part, int splits,
+                              float* db, hipStream_t s);
 // 2 x 2 max pooling (H, W even) and its gradient (the first maximum takes it).
 hipError_t launch_maxpool_fwd(const bf16_t* X, int N, int H, int W, int C, bf16_t* Y,
                               hipStream_t s);
