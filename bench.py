@@ -293,11 +293,14 @@ def bench_warm(args, H, W, np, torch):
     n = grid.size
     train = WS.contact_line_targets(args.warm_pairs, n, grid.pixel_nm, seed=500)
     held = WS.contact_line_targets(args.warm_eval, n, grid.pixel_nm, seed=900)
-    labels, lab_s = WS.ilt_labels(eng, train, cfg, iterations=200)
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    labels, lab_s = WS.ilt_labels(eng, train, cfg, iterations=200, log=log)
     net = UN.UNet(eng.context(), levels=4, base=32, seed=7)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist = net.train(train, labels, epochs=args.warm_epochs, batch=8, lr=1e-3)
+    hist = net.train(train, labels, epochs=args.warm_epochs, batch=8, lr=1e-3, log=log)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     res = UN.compare_warm_start(eng, net, held.cpu().numpy(), cfg, budget=200)

@@ -61,9 +61,10 @@ def contact_line_targets(count, n, pixel_nm, seed, device="cuda"):
     return out.to(torch.float32).contiguous()
 
 
-def ilt_labels(engine, targets, config, iterations=200, batch=64):
+def ilt_labels(engine, targets, config, iterations=200, batch=64, log=None):
     """Theta after `iterations` CG/Armijo steps from theta = target - 1/2, per
-    target (torch CUDA), in batches; returns (thetas, wall seconds)."""
+    target (torch CUDA), in batches; returns (thetas, wall seconds).  `log`
+    (optional) receives a progress line every 16 batches."""
     import torch
     from .ilt import IltProblem
     prob = IltProblem(engine, config)
@@ -75,5 +76,7 @@ def ilt_labels(engine, targets, config, iterations=200, batch=64):
         th = prob.initial_theta(tg)
         prob.run_stage(tg, th, iterations)
         out[i:i + batch] = th
+        if log and (i // batch) % 16 == 15:
+            log(f"labels: {i + tg.shape[0]} / {targets.shape[0]}")
     torch.cuda.synchronize()
     return out, time.perf_counter() - t0
