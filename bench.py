@@ -303,6 +303,8 @@ def bench_warm(args, H, W, np, torch):
     hist = net.train(train, labels, epochs=args.warm_epochs, batch=8, lr=1e-3, log=log)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    del train, labels
+    torch.cuda.empty_cache()
     res = UN.compare_warm_start(eng, net, held.cpu().numpy(), cfg, budget=200)
     T = args.warm_eval
     steps = args.warm_epochs * (args.warm_pairs // 8)

@@ -596,14 +596,16 @@ huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int**
 }
 
 // Kernel stacks of a plane list packed into one zero-padded buffer
-// [nplanes][Kmax][W][W] (W = 2 BB + 1, BB >= every set's band) with
-// eigenvalues / n^4 in [nplanes][Kmax] (0 for padding), built from the
-// host copies of the sets.
-huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int BB, const float2** kern,
+// [Q][nplanes][Kmax][W][W] for coarse grid m (W = 2 BB + 1, BB = (m - 1) / 4
+// >= every set's band, Q = m / 32; copy q multiplied by e^{2 pi i kx q / m} in
+// FP64) with eigenvalues / n^4 in [nplanes][Kmax] (0 for padding), built from
+// the host copies of the sets.
+huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, const float2** kern,
                       const float** lam, int* Kmax) {
+  const int BB = huvk::coarse_band(m), Q = huvk::coarse_classes(m);
   std::vector<uint64_t> key;
   for (int p = 0; p < nplanes; ++p) key.push_back(planes[p]->id);
-  key.push_back((uint64_t)BB);
+  key.push_back((uint64_t)m);
   auto it = c->plane_packs.find(key);
   if (it == c->plane_packs.end()) {
     Ctx::PlanePack pk;
@@ -613,7 +615,8 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int BB, cons
     }
     const int W = 2 * BB + 1;
     const size_t box = (size_t)W * W;
-    std::vector<float2> hk((size_t)nplanes * pk.Kmax * box, make_float2(0.f, 0.f));
+    const size_t per_class = (size_t)nplanes * pk.Kmax * box;
+    std::vector<float2> hk((size_t)Q * per_class, make_float2(0.f, 0.f));
     std::vector<float> hl((size_t)nplanes * pk.Kmax, 0.f);
     for (int p = 0; p < nplanes; ++p) {
       const huv_kset* ks = planes[p];
@@ -621,12 +624,17 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int BB, cons
       const double n4 = std::pow((double)n, 4.0);
       for (int k = 0; k < ks->K; ++k) {
         hl[(size_t)p * pk.Kmax + k] = (float)(ks->lam_host[k] / n4);
-        float2* dst = hk.data() + ((size_t)p * pk.Kmax + k) * box;
         for (int64_t t = 0; t < ks->S; ++t) {
           const int64_t i = ks->support[t] / n, j = ks->support[t] % n;
           const int64_t ky = i < n / 2 ? i : i - n, kx = j < n / 2 ? j : j - n;
-          dst[(kx + BB) * W + (ky + BB)] = make_float2((float)ks->values[2 * (k * ks->S + t)],
-                                                       (float)ks->values[2 * (k * ks->S + t) + 1]);
+          const double re = ks->values[2 * (k * ks->S + t)], im = ks->values[2 * (k * ks->S + t) + 1];
+          for (int q = 0; q < Q; ++q) {
+            const long e = ((kx * q) % m + m) % m;  // e^{2 pi i kx q / m}
+            const double a = 2.0 * M_PI * (double)e / (double)m;
+            const double cs = std::cos(a), sn = std::sin(a);
+            hk[q * per_class + ((size_t)p * pk.Kmax + k) * box + (kx + BB) * W + (ky + BB)] =
+                make_float2((float)(re * cs - im * sn), (float)(re * sn + im * cs));
+          }
         }
       }
     }
@@ -669,7 +677,7 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   const float2* kern;
   const float* lam;
   int Kmax;
-  st = plane_pack(c, nplanes, planes, BB, &kern, &lam, &Kmax);
+  st = plane_pack(c, nplanes, planes, m, &kern, &lam, &Kmax);
   if (st != HUV_OK) return st;
   float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)nimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");

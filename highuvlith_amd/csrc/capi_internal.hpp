@@ -85,7 +85,7 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* mas
 huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes);
 huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
                        const float2* const** dkern, const float* const** dlam);
-huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int BB, const float2** kern,
+huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, const float2** kern,
                       const float** lam, int* Kmax);
 // Forward images; tune = NULL uses the kernel sets' own coarse grid and the
 // default launch shapes (the ILT adjoint relies on that band layout).

@@ -113,8 +113,10 @@ __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
 // own band).  With P = K . M (untwisted band product), a field sample is
 //   E[y][x] = sum_{kx, ky} P[kx][ky] e^{2 pi i (kx x + ky y) / M}.
 // x group, band row iy (ky = iy - BB), for the columns x = Q j + q:
-//   c[h] = e^{2 pi i h q / M} sum_t P[h + 32 t][ky] e^{2 pi i t q / Q}
+//   c[h] = sum_t P_q[h + 32 t][ky],  P_q[kx] = K[kx] e^{2 pi i kx q / M} . M[kx]
 //   T[iy][j] = IFFT_32(c)[j]
+// (the class twist e^{2 pi i kx q / M} is carried by the kernel stack, one
+// pre-twisted copy per class, so the x pass is a plain multiply-accumulate)
 // y group, column j and residue r, for the rows y = R j' + r:
 //   d[h] = e^{2 pi i h r / M} sum_t T[h + 32 t][j] e^{2 pi i t r / R}
 //   E[R j' + r][Q j + q] = IFFT_32(d)[j'];  acc += lambda |E|^2
@@ -138,7 +140,7 @@ struct CoarseCfg {
   static constexpr int XOFF = ((YT + 63) / 64) * 64;
   static constexpr int XT = ((W + 63) / 64) * 64;
   static constexpr int THREADS = XOFF + XT;
-  static constexpr int LDS_C32 = 2 * W * TP + R * F + F;
+  static constexpr int LDS_C32 = 2 * W * TP + R * F;
 };
 
 constexpr int kXcds = 8;
@@ -164,6 +166,7 @@ __device__ __forceinline__ c32 load_c32(__amdgpu_buffer_rsrc_t r, int voff, int 
 }
 
 // XNB: band columns per load block of the x pass (a tuning parameter).
+// kern: [Q][nplanes * Kmax][W][W], class q's copy twisted by e^{2 pi i kx q / M}.
 template <int M, int XNB>
 __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_socs_fields(
     const float2* __restrict__ Mq, const float2* __restrict__ kern, const float* __restrict__ lam,
@@ -174,7 +177,6 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
   extern __shared__ c32 lds[];
   c32* T = lds;                    // 2 x W x TP
   c32* twy = lds + 2 * W * TP;     // [r][h] = e^{2 pi i h r / M}
-  c32* twx = twy + R * F;          // [h] = e^{2 pi i h q / M}
   const int L = blockIdx.x;
   const int slot = L / kXcds;
   const int q = slot % Q;
@@ -182,19 +184,15 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
   if (clip >= nclips) return;  // whole work-group, before any barrier
   const int tid = threadIdx.x;
   for (int i = tid; i < R * F; i += C::THREADS) twy[i] = tw_pos(tw_m, (i % F) * (i / F), M);
-  for (int i = tid; i < F; i += C::THREADS) twx[i] = tw_pos(tw_m, i * q, M);
   const __amdgpu_buffer_rsrc_t mr = box_rsrc(Mq + (size_t)clip * W * W, W * W);
   const int steps = nplanes * Kmax;
   const bool is_y = tid < C::YT;
   const int j = tid % F, r = tid / F;
   const int iy = tid - C::XOFF;
-  // fold factors e^{2 pi i t q / Q} (x) or e^{2 pi i t r / R} (y): M / Q = 32
+  // y fold factors e^{2 pi i t r / R} (M / R = 32)
   c32 wf[NT];
-  {
-    const int u = is_y ? r : q;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) wf[t] = tw_pos(tw_m, (t + TMIN) * u * F, M);
-  }
+  for (int t = 0; t < NT; ++t) wf[t] = tw_pos(tw_m, (t + TMIN) * (is_y ? r : 0) * F, M);
   __syncthreads();
   // The two groups run separate loops (wave-uniform split) with the same
   // number of barriers, so their register live ranges do not overlap.
@@ -203,7 +201,8 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
     for (int s = 0; s <= steps; ++s) {
       if (s < steps && iy < W) {
         // Buffer loads: one VGPR offset per lane, the column offset in an SGPR.
-        const __amdgpu_buffer_rsrc_t kr = box_rsrc(kern + (size_t)s * W * W, W * W);
+        const __amdgpu_buffer_rsrc_t kr =
+            box_rsrc(kern + ((size_t)q * steps + s) * W * W, W * W);
         c32 c[F];
 #pragma unroll
         for (int h = 0; h < F; ++h) c[h] = c32{0.f, 0.f};
@@ -237,14 +236,12 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
             if (col < W) {
               const int kx = col - BB;
               const int h = ((kx % F) + F) % F;
-              const int t = (kx - h) / F;
-              const c32 p = cmul(ka[blk & 1][i], ma[blk & 1][i]);
-              c[h] = cadd(c[h], t == 0 ? p : cmul(p, wf[t - TMIN]));
+              const c32 a = ka[blk & 1][i], b = ma[blk & 1][i];
+              c[h].re += a.re * b.re - a.im * b.im;
+              c[h].im += a.re * b.im + a.im * b.re;
             }
           }
         }
-#pragma unroll
-        for (int h = 0; h < F; ++h) c[h] = cmul(c[h], twx[h]);
         fft_reg<+1, F>(c);
         c32* Tb = T + (s & 1) * W * TP;
 #pragma unroll
@@ -283,10 +280,15 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
 #pragma unroll
         for (int i = 0; i < F; ++i) acc[i] += l * cnorm(d[i]);
         if (sp % Kmax == Kmax - 1) {
-          float* out = coarse + ((size_t)clip * nplanes + sp / Kmax) * M * M;
+          // Vector buffer stores: one VGPR offset, the row offset in an SGPR.
+          const __amdgpu_buffer_rsrc_t orc = __builtin_amdgcn_make_buffer_rsrc(
+              coarse + ((size_t)clip * nplanes + sp / Kmax) * M * M, (short)0,
+              M * M * (int)sizeof(float), 0x00020000);
+          const int vo = (r * M + Q * j + q) * (int)sizeof(float);
 #pragma unroll
           for (int i = 0; i < F; ++i) {
-            out[(size_t)(R * i + r) * M + Q * j + q] = acc[i];
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, acc[i]), orc, vo,
+                                                  R * i * M * (int)sizeof(float), 0);
             acc[i] = 0.f;
           }
         }
@@ -386,7 +388,10 @@ __global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_cols(
 //       d[h] = e^{2 pi i h c / n} sum_t z[h + 32 t] e^{2 pi i t c / C}
 //   with z[g] = Ua[g] + i Ub[g] (g >= 0), conj Ua[-g] + i conj Ub[-g] (g < 0);
 //   row a takes Re X, row b Im X.  Writes are coalesced across classes.
-// Folding ranges t in [TMIN, TMAX] cover |g| <= G (G <= 32 (TMAX + 1) - 1).
+// Folding ranges t in [TMIN, TMAX] cover |g| <= G with 32 (TMAX) <= G <=
+// 32 (TMAX + 1) - 1 (TMIN = -TMAX - 1), so only the outermost t of either
+// sign can fall outside the band; U rows are zero-padded to |g| <= 32 |TMIN|
+// so phase 2 needs no band test at all.
 // Work-groups of one image are dealt to one XCD so the band stays in its L2.
 // ---------------------------------------------------------------------------
 template <int TMIN, int TMAX>
@@ -395,7 +400,7 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
                                                     float* __restrict__ out) {
   constexpr int F = 32;
   extern __shared__ c32 smem[];
-  const int UP = G + 1;
+  constexpr int UP = 32 * (-TMIN) + 1;  // U row: g = 0 .. 32 |TMIN|, zero beyond G
   c32* U = smem;           // F x UP
   c32* twn = U + F * UP;   // e^{+2 pi i k / n}, k < n
   const int P = n / F, C = n / F;
@@ -406,6 +411,10 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
   if (img >= nimg) return;  // whole work-group, before any barrier
   const int tid = threadIdx.x;
   for (int i = tid; i < n; i += 256) twn[i] = tw_lookup<+1>(tw_n, i);
+  for (int i = tid; i < F * (UP - (G + 1)); i += 256) {
+    const int row = i / (UP - (G + 1)), col = G + 1 + i % (UP - (G + 1));
+    U[row * UP + col] = c32{0.f, 0.f};
+  }
   __syncthreads();
   const int nk = 2 * G + 1;
   // ---- phase 1: two threads per column gx, each folding half of the t range ----
@@ -429,7 +438,9 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
       for (int h = 0; h < F; ++h) {
         const int gy = h + F * t;
         e[h] = c32{0.f, 0.f};
-        if (gy >= -G && gy <= G) e[h] = load_c32(br, vo, (gy + G) * (G + 1) * (int)sizeof(float2));
+        // only the outermost folds can leave the band
+        const bool inb = t == TMIN ? gy >= -G : (t == TMAX ? gy <= G : true);
+        if (inb) e[h] = load_c32(br, vo, (gy + G) * (G + 1) * (int)sizeof(float2));
       }
 #pragma unroll
       for (int h = 0; h < F; ++h) u[h] = cadd(u[h], t == 0 ? e[h] : cmul(e[h], w));
@@ -464,17 +475,16 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
 #pragma unroll
       for (int h = 0; h < F; ++h) {
         const int g = h + F * t;
-        if (g >= -G && g <= G) {
-          c32 z;
-          if (g >= 0) {
-            const c32 a = ua[g], b = ub[g];
-            z = c32{a.re - b.im, a.im + b.re};
-          } else {
-            const c32 a = ua[-g], b = ub[-g];
-            z = c32{a.re + b.im, -a.im + b.re};
-          }
-          d[h] = cadd(d[h], t == 0 ? z : cmul(z, w));
+        // U is zero beyond G, so every fold term can be read unconditionally
+        c32 z;
+        if (g >= 0) {
+          const c32 a = ua[g], b = ub[g];
+          z = c32{a.re - b.im, a.im + b.re};
+        } else {
+          const c32 a = ua[-g], b = ub[-g];
+          z = c32{a.re + b.im, -a.im + b.re};
         }
+        d[h] = cadd(d[h], t == 0 ? z : cmul(z, w));
       }
     }
 #pragma unroll
@@ -660,12 +670,13 @@ hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2*
 template <int TMIN, int TMAX>
 static hipError_t band_image_t(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
                                float* out, hipStream_t s) {
-  const size_t shmem = ((size_t)32 * (G + 1) + n) * sizeof(c32);
+  constexpr int UP = 32 * (-TMIN) + 1;
+  const size_t shmem = ((size_t)32 * UP + n) * sizeof(c32);
   auto kern = k_band_image<TMIN, TMAX>;
-  // Sized for the largest band of this instantiation, once.
+  // Sized for the largest grid (n = 2048), once.
   static const hipError_t attr = hipFuncSetAttribute(
       (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-      (int)(((size_t)32 * (32 * (TMAX + 1)) + 2048) * sizeof(c32)));
+      (int)(((size_t)32 * UP + 2048) * sizeof(c32)));
   if (attr != hipSuccess) return attr;
   const int groups = (nimg + kXcds - 1) / kXcds;
   HUV_LAUNCH(kern, dim3(groups * kXcds * (n / 32)), dim3(256), shmem, s, Ib, n, G, nimg, tw_n,

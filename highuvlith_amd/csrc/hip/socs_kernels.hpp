@@ -28,8 +28,9 @@ hipError_t launch_rows_band(int n, int n1, int B, int nclips, const float* masks
 // Mq: [nclips][2BB+1][2BB+1] band spectrum, zero beyond |k| <= B.
 hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const float2* tmp,
                             const float2* tw_n, const float* sinc_t, float2* Mq, hipStream_t s);
-// kern: [nplanes][Kmax][W][W] (W = 2 coarse_band(m) + 1), lam: [nplanes][Kmax];
-// Mq at the same W; coarse: [nclips][nplanes][m][m].  xnb = 0 selects the default.
+// kern: [Q][nplanes][Kmax][W][W] (W = 2 coarse_band(m) + 1, Q = m / 32; copy q
+// multiplied by e^{2 pi i kx q / m}), lam: [nplanes][Kmax]; Mq at the same W;
+// coarse: [nclips][nplanes][m][m].  xnb = 0 selects the default.
 hipError_t launch_socs_fields(int m, int xnb, int nclips, int nplanes, int Kmax,
                               const float2* kern, const float* lam, const float2* Mq,
                               const float2* tw_m, float* coarse, hipStream_t s);

@@ -53,15 +53,19 @@ class UNet:
         x = x.to(torch.float32).contiguous()
         return x[None] if x.dim() == 2 else x
 
-    def forward(self, x):
-        """theta_0 = net(target) for a batch (torch CUDA or numpy (N, H, W))."""
+    def forward(self, x, chunk=32):
+        """theta_0 = net(target) for a batch (torch CUDA or numpy (N, H, W)),
+        `chunk` samples per call (activations are sized per call)."""
         import torch
         x = self._dev(x)
         out = torch.empty_like(x)
         self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
         N, H, W = x.shape
-        check(self.ctx.lib.huv_unet_forward(self.handle, N, H, W, C.c_void_p(x.data_ptr()),
-                                            C.c_void_p(out.data_ptr())))
+        for i in range(0, N, chunk):
+            xi, oi = x[i:i + chunk], out[i:i + chunk]
+            check(self.ctx.lib.huv_unet_forward(self.handle, xi.shape[0], H, W,
+                                                C.c_void_p(xi.data_ptr()),
+                                                C.c_void_p(oi.data_ptr())))
         return out
 
     def loss_and_grad(self, x, target, want_grad=False):
