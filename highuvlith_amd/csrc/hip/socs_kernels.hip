@@ -459,12 +459,36 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     for (int j = 0; j < F; ++j) U[j * UP + gx] = cadd(U[j * UP + gx], u[j]);
   }
   __syncthreads();
+  // ---- pair packing, in place: Z[pr][g + GP] = z_pr[g] for |g| <= GP ----
+  constexpr int GP = 32 * (-TMIN);
+  constexpr int ZW = 2 * GP + 1;
+  constexpr int NZ = (F / 2) * ZW;  // <= F * UP
+  constexpr int PER = (NZ + 255) / 256;
+  {
+    c32 zv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = tid + 256 * k;
+      zv[k] = c32{0.f, 0.f};
+      if (idx < NZ) {
+        const int pr = idx / ZW, g = idx % ZW - GP, ag = g < 0 ? -g : g;
+        const c32 a = U[(2 * pr) * UP + ag], b = U[(2 * pr + 1) * UP + ag];
+        zv[k] = g >= 0 ? c32{a.re - b.im, a.im + b.re} : c32{a.re + b.im, -a.im + b.re};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = tid + 256 * k;
+      if (idx < NZ) U[idx] = zv[k];
+    }
+    __syncthreads();
+  }
   // ---- phase 2 ----
   float* img_out = out + (size_t)img * n * n;
   for (int task = tid; task < (F / 2) * C; task += 256) {
     const int pr = task / C, c = task % C;
-    const c32* ua = U + (2 * pr) * UP;
-    const c32* ub = ua + UP;
+    const c32* zr = U + pr * ZW + GP;  // zr[g], |g| <= GP
     c32 d[F];
 #pragma unroll
     for (int h = 0; h < F; ++h) d[h] = c32{0.f, 0.f};
@@ -475,15 +499,8 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
 #pragma unroll
       for (int h = 0; h < F; ++h) {
         const int g = h + F * t;
-        // U is zero beyond G, so every fold term can be read unconditionally
-        c32 z;
-        if (g >= 0) {
-          const c32 a = ua[g], b = ub[g];
-          z = c32{a.re - b.im, a.im + b.re};
-        } else {
-          const c32 a = ua[-g], b = ub[-g];
-          z = c32{a.re + b.im, -a.im + b.re};
-        }
+        // Z is zero beyond G, so every fold term is read unconditionally
+        const c32 z = zr[g];
         d[h] = cadd(d[h], t == 0 ? z : cmul(z, w));
       }
     }
