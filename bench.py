@@ -253,7 +253,7 @@ def bench_uneven(args, H, W, np, torch):
     for masks, planes, _, eng in groups:
         e = masks.shape[-1]
         tuned[str(e)] = {"band": planes[0].band, "count": int(masks.shape[0]),
-                         "m_n1_xnb": ctx.tuning(e, planes[0].band, 1, planes[0].host.num_kernels)}
+                         "m_n1_xnb_xlanes": ctx.tuning(e, planes[0].band, 1, planes[0].host.num_kernels)}
     return {"workload": f"{len(edges)} clips, edges 256/512/1024/2048 px at 2 nm, ArF 193i "
                         "annular 0.7/0.9, K=32, one graph launch",
             "mpix_per_batch": round(pix / 1e6, 1), "ms_per_batch": round(dt * 1e3, 3),

@@ -201,7 +201,7 @@ void Ctx::forget_kset(uint64_t id) {
 }
 
 huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* masks, float2* Mq,
-                      bool pixel_transfer, int n1) {
+                      bool pixel_transfer, int n1, float2* Mp) {
   hipError_t e;
   const float2* tw_n = c->twiddles(n, e);
   if (!tw_n) return hip_fail(e, "twiddle table");
@@ -216,7 +216,7 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* mas
   if (e != hipSuccess) return hip_fail(e, "rows_band launch");
   c->stage_end(ST_ROWS_BAND, ev);
   c->stage_begin(ST_COLS_BAND, &ev);
-  e = huvk::launch_cols_band(n, n1, B, BB, nclips, tmp, tw_n, st, Mq, c->stream());
+  e = huvk::launch_cols_band(n, n1, B, BB, nclips, tmp, tw_n, st, Mq, Mp, c->stream());
   if (e != hipSuccess) return hip_fail(e, "cols_band launch");
   c->stage_end(ST_COLS_BAND, ev);
   return HUV_OK;
@@ -600,12 +600,13 @@ huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int**
 // >= every set's band, Q = m / 32; copy q multiplied by e^{2 pi i kx q / m} in
 // FP64) with eigenvalues / n^4 in [nplanes][Kmax] (0 for padding), built from
 // the host copies of the sets.
-huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, const float2** kern,
-                      const float** lam, int* Kmax) {
+huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool paired,
+                      const float2** kern, const float** lam, int* Kmax) {
   const int BB = huvk::coarse_band(m), Q = huvk::coarse_classes(m);
   std::vector<uint64_t> key;
   for (int p = 0; p < nplanes; ++p) key.push_back(planes[p]->id);
   key.push_back((uint64_t)m);
+  key.push_back((uint64_t)paired);
   auto it = c->plane_packs.find(key);
   if (it == c->plane_packs.end()) {
     Ctx::PlanePack pk;
@@ -614,7 +615,12 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, const
       if (planes[p]->B > BB) return fail(HUV_INTERNAL_ERROR, "kernel band exceeds the packed box");
     }
     const int W = 2 * BB + 1;
-    const size_t box = (size_t)W * W;
+    // W x W, or column-paired (W + 1) x W: column col at [(col + 1) / 2][row][(col + 1) % 2]
+    const size_t box = (size_t)(paired ? W + 1 : W) * W;
+    auto at = [&](int col, int row) -> size_t {
+      return paired ? ((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)
+                    : (size_t)col * W + row;
+    };
     const size_t per_class = (size_t)nplanes * pk.Kmax * box;
     std::vector<float2> hk((size_t)Q * per_class, make_float2(0.f, 0.f));
     std::vector<float> hl((size_t)nplanes * pk.Kmax, 0.f);
@@ -632,7 +638,7 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, const
             const long e = ((kx * q) % m + m) % m;  // e^{2 pi i kx q / m}
             const double a = 2.0 * M_PI * (double)e / (double)m;
             const double cs = std::cos(a), sn = std::sin(a);
-            hk[q * per_class + ((size_t)p * pk.Kmax + k) * box + (kx + BB) * W + (ky + BB)] =
+            hk[q * per_class + ((size_t)p * pk.Kmax + k) * box + at(kx + BB, ky + BB)] =
                 make_float2((float)(re * cs - im * sn), (float)(re * sn + im * cs));
           }
         }
@@ -664,7 +670,10 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   const int m = tune && tune->m ? tune->m : planes[0]->m;
   const int BB = huvk::coarse_band(m);
   const int n1 = tune ? tune->n1 : 0, xnb = tune ? tune->xnb : 0;
+  const int xlanes = tune && tune->xlanes ? tune->xlanes : huvk::fields_lanes_default(m);
   if (B > BB || m > n) return fail(HUV_INVALID_PARAMETER, "coarse grid too small for the kernel band");
+  if (!huvk::fields_lanes_supported(m, xlanes))
+    return fail(HUV_INVALID_PARAMETER, "two x lanes per row need a coarse grid of 64 or more");
   const double flare = planes[0]->flare;
   hipError_t e;
   const int W = 2 * BB + 1;
@@ -672,12 +681,18 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   float2* Mq = (float2*)c->ws.get(Workspace::BAND, (size_t)nclips * W * W * sizeof(float2), e);
   if (!Mq) return hip_fail(e, "workspace");
   if (Mq_out) *Mq_out = Mq;
-  huv_status st = band_stage(c, nclips, n, B, BB, masks, Mq, pt, n1);
+  // the two-lane field kernel reads the column-paired copy of the band
+  float2* Mp = nullptr;
+  if (xlanes == 2) {
+    Mp = (float2*)c->ws.get(Workspace::BAND_PAIRED, (size_t)nclips * (W + 1) * W * sizeof(float2), e);
+    if (!Mp) return hip_fail(e, "workspace");
+  }
+  huv_status st = band_stage(c, nclips, n, B, BB, masks, Mq, pt, n1, Mp);
   if (st != HUV_OK) return st;
   const float2* kern;
   const float* lam;
   int Kmax;
-  st = plane_pack(c, nplanes, planes, m, &kern, &lam, &Kmax);
+  st = plane_pack(c, nplanes, planes, m, xlanes == 2, &kern, &lam, &Kmax);
   if (st != HUV_OK) return st;
   float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)nimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");
@@ -685,8 +700,8 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   if (!tw_m) return hip_fail(e, "twiddle table");
   hipEvent_t ev;
   c->stage_begin(ST_FIELDS, &ev);
-  e = huvk::launch_socs_fields(m, xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse,
-                               c->stream());
+  e = huvk::launch_socs_fields(m, xnb, xlanes, nclips, nplanes, Kmax, kern, lam,
+                               xlanes == 2 ? Mp : Mq, tw_m, coarse, c->stream());
   if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
   c->stage_end(ST_FIELDS, ev);
   const float2* tw_n = c->twiddles(n, e);
@@ -754,12 +769,12 @@ huv_status autotune_socs(Ctx* c, int nclips, int n, const float* masks, int npla
     return HUV_OK;
   };
   // explicit values throughout (0 would mean "default" in the table)
-  huv_socs_tuning cur{planes[0]->m, huvk::band_fft_default(n), huvk::kFieldsXnbDefault};
+  huv_socs_tuning cur{planes[0]->m, huvk::band_fft_default(n), huvk::kFieldsXnbDefault, 1};
   double cur_ms = 1e30;
   const int B = planes[0]->B;
   for (int m : {32, 64, 128, 192, 256}) {
     if (m < 4 * B + 1 || m > n) continue;
-    huv_socs_tuning t{m, cur.n1, cur.xnb};
+    huv_socs_tuning t{m, cur.n1, cur.xnb, cur.xlanes};
     double ms;
     st = time_one(t, ms);
     if (st != HUV_OK) break;
@@ -774,13 +789,17 @@ huv_status autotune_socs(Ctx* c, int nclips, int n, const float* masks, int npla
     st = time_one(t, ms);
     if (st == HUV_OK && ms < cur_ms) { cur_ms = ms; cur = t; }
   }
-  for (int xnb : {4, 6, 8}) {
-    if (st != HUV_OK) break;
-    huv_socs_tuning t = cur;
-    t.xnb = xnb;
-    double ms;
-    st = time_one(t, ms);
-    if (st == HUV_OK && ms < cur_ms) { cur_ms = ms; cur = t; }
+  for (int xlanes : {1, 2}) {
+    if (!huvk::fields_lanes_supported(cur.m, xlanes)) continue;
+    for (int xnb : {4, 6, 8}) {
+      if (st != HUV_OK) break;
+      huv_socs_tuning t = cur;
+      t.xlanes = xlanes;
+      t.xnb = xnb;
+      double ms;
+      st = time_one(t, ms);
+      if (st == HUV_OK && ms < cur_ms) { cur_ms = ms; cur = t; }
+    }
   }
   // leave the chosen shape's result in `out`
   if (st == HUV_OK) st = socs_forward(c, nclips, n, masks, nplanes, planes, out, pt, nullptr, &cur);
@@ -802,7 +821,7 @@ huv_status socs_shape(Ctx* c, int nclips, int n, const float* masks, int nplanes
     *t = it->second;
     return HUV_OK;
   }
-  *t = huv_socs_tuning{0, 0, 0};
+  *t = huv_socs_tuning{0, 0, 0, 0};
   if (!c->autotune) return HUV_OK;
   huv_status st = autotune_socs(c, nclips, n, masks, nplanes, planes, out, pt, t);
   if (st != HUV_OK) return st;
@@ -892,7 +911,7 @@ huv_status socs_groups(Ctx* c, int ngroups, const huv_clip_group* groups, const 
   for (int i = 0; i < ngroups; ++i) {
     const huv_clip_group& g = groups[i];
     if (g.nclips == 0) continue;
-    GroupPlan gp{g, {0, 0, 0}};
+    GroupPlan gp{g, {0, 0, 0, 0}};
     bool computed = false;
     huv_status st = socs_shape(c, g.nclips, g.n, g.masks, g.nplanes, g.planes, g.out, pt, &gp.t,
                                &computed);
@@ -900,7 +919,7 @@ huv_status socs_groups(Ctx* c, int ngroups, const huv_clip_group* groups, const 
     plan.push_back(gp);
     key.insert(key.end(), {(uint64_t)g.n, (uint64_t)g.nclips, (uint64_t)(uintptr_t)g.masks,
                            (uint64_t)(uintptr_t)g.out, (uint64_t)g.nplanes, (uint64_t)gp.t.m,
-                           (uint64_t)gp.t.n1, (uint64_t)gp.t.xnb});
+                           (uint64_t)gp.t.n1, (uint64_t)gp.t.xnb, (uint64_t)gp.t.xlanes});
     for (int p = 0; p < g.nplanes; ++p) {
       key.push_back(g.planes[p]->id);
       ksets.push_back(g.planes[p]->id);
@@ -965,7 +984,7 @@ huv_status huv_ctx_tuning_get(huv_ctx* c, int n, int band, int nplanes, int kmax
   std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->tuned.find({n, band, nplanes, kmax});
   *found = it != c->tuned.end();
-  *out = *found ? it->second : huv_socs_tuning{0, 0, 0};
+  *out = *found ? it->second : huv_socs_tuning{0, 0, 0, 0};
   return HUV_OK;
 }
 
@@ -984,6 +1003,9 @@ huv_status huv_ctx_tuning_set(huv_ctx* c, int n, int band, int nplanes, int kmax
   }
   if (t->xnb != 0 && t->xnb != 4 && t->xnb != 6 && t->xnb != 8)
     return fail(HUV_INVALID_PARAMETER, "tuning: xnb must be 4, 6 or 8");
+  if (t->xlanes < 0 || t->xlanes > 2 ||
+      (t->xlanes == 2 && t->m != 0 && !huvk::fields_lanes_supported(t->m, 2)))
+    return fail(HUV_INVALID_PARAMETER, "tuning: xlanes must be 1, or 2 with a coarse grid of 64 or more");
   std::lock_guard<std::mutex> lk(c->mu);
   c->tuned[{n, band, nplanes, kmax}] = *t;
   return HUV_OK;

@@ -18,8 +18,8 @@ huv_status hip_fail(hipError_t e, const char* what);
 
 // Device scratch owned by a context, grown on demand and reused across calls.
 struct Workspace {
-  enum Slot { TMP_ROWS = 0, BAND, COARSE, TMP_COARSE, IBAND, ILT0, ILT1, ILT2, ILT3, ILT4, ILT5,
-              ILT6, ILT7, ILT8, kSlots };
+  enum Slot { TMP_ROWS = 0, BAND, BAND_PAIRED, COARSE, TMP_COARSE, IBAND, ILT0, ILT1, ILT2, ILT3,
+              ILT4, ILT5, ILT6, ILT7, ILT8, kSlots };
   void* ptr[kSlots] = {};
   size_t size[kSlots] = {};
   uint64_t generation = 0;  // bumped whenever a slot is reallocated
@@ -39,7 +39,7 @@ struct Ctx {
   // by the ids of the kernel sets in a plane list (dropped when a set dies).
   std::map<std::vector<uint64_t>, void*> plane_tables;
   struct PlanePack {
-    float2* kern = nullptr;  // [nplanes][Kmax][W][W]
+    float2* kern = nullptr;  // [Q][nplanes][Kmax] boxes (W x W or column-paired)
     float* lam = nullptr;    // [nplanes][Kmax], zero-padded
     int Kmax = 0;
   };
@@ -78,15 +78,18 @@ enum Stage { ST_ROWS_BAND = 0, ST_COLS_BAND, ST_FIELDS, ST_COARSE_ROWS, ST_COARS
              ST_BAND_IMAGE, ST_ILT_POINTWISE, ST_ILT_ADJOINT, ST_ILT_OTHER, ST_COUNT };
 
 // Band spectrum of nclips real n x n masks: bins |k| <= B computed, stored
-// in boxes of half-width BB >= B (zero beyond B).  Layout Mq[clip][kx + BB][ky + BB].
+// in boxes of half-width BB >= B (zero beyond B).  Layout Mq[clip][kx + BB][ky + BB];
+// Mp (optional) receives the column-paired copy (huvk::launch_cols_band).
 huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* masks, float2* Mq,
-                      bool pixel_transfer, int n1 = 0);
+                      bool pixel_transfer, int n1 = 0, float2* Mp = nullptr);
 
 huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes);
 huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
                        const float2* const** dkern, const float* const** dlam);
-huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, const float2** kern,
-                      const float** lam, int* Kmax);
+// Per-class pre-twisted kernel stacks at coarse size m (column-paired boxes
+// when `paired`, for the two-lane field kernel).
+huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool paired,
+                      const float2** kern, const float** lam, int* Kmax);
 // Forward images; tune = NULL uses the kernel sets' own coarse grid and the
 // default launch shapes (the ILT adjoint relies on that band layout).
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,

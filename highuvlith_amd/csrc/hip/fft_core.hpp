@@ -27,6 +27,15 @@ HUV_HD c32 csub(c32 a, c32 b) { return c32{a.re - b.re, a.im - b.im}; }
 HUV_HD c32 cmul(c32 a, c32 b) {
   return c32{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
+// acc + a * b as four fused multiply-adds (a separate product and sum would
+// take six instructions).
+HUV_HD c32 cmac(c32 acc, c32 a, c32 b) {
+  acc.re = __builtin_fmaf(a.re, b.re, acc.re);
+  acc.re = __builtin_fmaf(-a.im, b.im, acc.re);
+  acc.im = __builtin_fmaf(a.re, b.im, acc.im);
+  acc.im = __builtin_fmaf(a.im, b.re, acc.im);
+  return acc;
+}
 HUV_HD c32 cconjmul(c32 a, c32 b) {  // conj(a) * b
   return c32{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
 }

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(M) void k_ilt_adjoint(
         }
         if (h - M >= -B) {
           const float2 a = kk[(size_t)(h - M + B) * W], b = mm[(size_t)(h - M + B) * W];
-          v[h] = cadd(v[h], cmul(c32{a.x, a.y}, c32{b.x, b.y}));
+          v[h] = cmac(v[h], c32{a.x, a.y}, c32{b.x, b.y});
         }
       }
       fft_reg<+1, M>(v);

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(TEAMS * N2) void k_rows_band(const float* __restric
 template <int N1, int N2, int TEAMS>
 __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
     const float2* __restrict__ tmp, int B, int BB, const float2* __restrict__ tw_n,
-    const float* __restrict__ sinc_t, float2* __restrict__ Mq) {
+    const float* __restrict__ sinc_t, float2* __restrict__ Mq, float2* __restrict__ Mp) {
   constexpr int N = N1 * N2;
   __shared__ c32 scratch[TEAMS][(N1 + 1) * N2];
   const int team = threadIdx.x / N2, t = threadIdx.x % N2;
@@ -84,6 +84,11 @@ __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
   const float2* col_pos = tmp + ((size_t)clip * nk + kxa) * npairs;
   const float2* col_neg = tmp + ((size_t)clip * nk + (kxa == 0 ? 0 : B + kxa)) * npairs;
   float2* mq = Mq + (size_t)clip * W * W;
+  float2* mp = Mp ? Mp + (size_t)clip * (W + 1) * W : nullptr;
+  // column-paired copy: (col, row) at [(col + 1) / 2][row][(col + 1) % 2]
+  auto put_p = [&](int col, int row, float2 v) {
+    mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
+  };
   fft_four_step<-1, N1, N2>(
       t, scratch[team], tw_n,
       [&](int i) {
@@ -103,6 +108,11 @@ __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
         else v = c32{0.f, 0.f};
         mq[(size_t)(kx + BB) * W + (ky + BB)] = make_float2(v.re, v.im);
         if (kx > 0) mq[(size_t)(BB - kx) * W + (BB - ky)] = make_float2(v.re, -v.im);
+        if (mp) {
+          put_p(kx + BB, ky + BB, make_float2(v.re, v.im));
+          if (kx > 0) put_p(BB - kx, BB - ky, make_float2(v.re, -v.im));
+          if (kx == 0) put_p(-1, ky + BB, make_float2(0.f, 0.f));  // the zero column
+        }
       });
 }
 
@@ -126,7 +136,19 @@ __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
 // Work-groups are ordered so that the Q classes of one clip share an XCD
 // (consecutive work-group ids are dealt round-robin over the 8 XCDs).
 // ---------------------------------------------------------------------------
-template <int M>
+// floor(a / 32)
+constexpr int fdiv32(int a) { return a >= 0 ? a / 32 : -((-a + 31) / 32); }
+
+// XS = 1: one x lane per band row, all columns.  XS = 2: two lanes per row
+// (lanes l and l + 32 of one wave); lane p takes the columns whose bin
+// h = kx mod 32 lies in [16 p, 16 p + 16), in NR runs of 16,
+//   kx = 16 p + 32 (xt0(p) + r) + i,  r < NR, i < 16,
+// and the pair finishes the 32-point transform with one radix-2 step.  The
+// only column outside the band is kx = -BB - 1 (col -1, the first column of
+// one of the lanes).  This form reads column-paired band spectra and kernel
+// stacks, [(col + 1) / 2][row][(col + 1) % 2] with col in [-1, W) (col -1
+// zero), one 16-byte load per lane for two columns.
+template <int M, int XS = 1>
 struct CoarseCfg {
   static constexpr int F = 32;  // transform length of both passes
   static constexpr int Q = M / F;
@@ -138,10 +160,23 @@ struct CoarseCfg {
   static constexpr int TMAX = BB / F;
   static constexpr int YT = F * R;
   static constexpr int XOFF = ((YT + 63) / 64) * 64;
-  static constexpr int XT = ((W + 63) / 64) * 64;
+  static constexpr int XT = XS == 1 ? ((W + 63) / 64) * 64 : ((W + 31) / 32) * 64;
   static constexpr int THREADS = XOFF + XT;
-  static constexpr int LDS_C32 = 2 * W * TP + R * F;
+  static constexpr int xt0(int p) { return -((BB + 16 * p + 15) >> 5); }
+  static constexpr int xcol0(int p) { return 16 * p + 32 * xt0(p) + BB; }
+  static constexpr int nr() {
+    const int a = fdiv32(BB) - xt0(0) + 1, b = fdiv32(BB - 16) - xt0(1) + 1;
+    return a > b ? a : b;
+  }
+  static constexpr int NR = nr();
+  static constexpr bool SPLIT_OK =
+      xcol0(0) >= -1 && xcol0(1) >= -1 && xcol0(0) + 32 * (NR - 1) + 15 <= W - 1 &&
+      xcol0(1) + 32 * (NR - 1) + 15 <= W - 1;
+  static constexpr int LDS_C32 = 2 * W * TP + R * F + F;
 };
+static_assert(CoarseCfg<64, 2>::SPLIT_OK && CoarseCfg<128, 2>::SPLIT_OK &&
+                  CoarseCfg<192, 2>::SPLIT_OK && CoarseCfg<256, 2>::SPLIT_OK,
+              "two-lane x pass: columns must lie in [-1, W)");
 
 constexpr int kXcds = 8;
 
@@ -165,18 +200,29 @@ __device__ __forceinline__ c32 load_c32(__amdgpu_buffer_rsrc_t r, int voff, int 
   return c32{f.x, f.y};
 }
 
+struct c32x2 {
+  c32 a, b;
+};
+
+__device__ __forceinline__ c32x2 load_c32x2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  const float4 f = __builtin_bit_cast(float4, v);
+  return c32x2{c32{f.x, f.y}, c32{f.z, f.w}};
+}
+
 // XNB: band columns per load block of the x pass (a tuning parameter).
 // kern: [Q][nplanes * Kmax][W][W], class q's copy twisted by e^{2 pi i kx q / M}.
-template <int M, int XNB>
-__global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_socs_fields(
+template <int M, int XNB, int XS>
+__global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_waves_per_eu(4))) void k_socs_fields(
     const float2* __restrict__ Mq, const float2* __restrict__ kern, const float* __restrict__ lam,
     int nclips, int nplanes, int Kmax, const float2* __restrict__ tw_m, float* __restrict__ coarse) {
-  using C = CoarseCfg<M>;
+  using C = CoarseCfg<M, XS>;
   constexpr int F = C::F, Q = C::Q, R = C::R, BB = C::BB, W = C::W, TP = C::TP;
   constexpr int TMIN = C::TMIN, TMAX = C::TMAX, NT = TMAX - TMIN + 1;
   extern __shared__ c32 lds[];
   c32* T = lds;                    // 2 x W x TP
   c32* twy = lds + 2 * W * TP;     // [r][h] = e^{2 pi i h r / M}
+  c32* tw32 = twy + R * F;         // [k] = e^{2 pi i k / 32}
   const int L = blockIdx.x;
   const int slot = L / kXcds;
   const int q = slot % Q;
@@ -184,11 +230,16 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
   if (clip >= nclips) return;  // whole work-group, before any barrier
   const int tid = threadIdx.x;
   for (int i = tid; i < R * F; i += C::THREADS) twy[i] = tw_pos(tw_m, (i % F) * (i / F), M);
-  const __amdgpu_buffer_rsrc_t mr = box_rsrc(Mq + (size_t)clip * W * W, W * W);
+  for (int i = tid; i < F; i += C::THREADS) tw32[i] = tw_pos(tw_m, i * Q, M);
+  // band spectrum box of this clip (column-paired for XS = 2)
+  constexpr int BOX = XS == 1 ? W * W : (W + 1) * W;
+  const __amdgpu_buffer_rsrc_t mr = box_rsrc(Mq + (size_t)clip * BOX, BOX);
   const int steps = nplanes * Kmax;
   const bool is_y = tid < C::YT;
   const int j = tid % F, r = tid / F;
-  const int iy = tid - C::XOFF;
+  const int ix = tid - C::XOFF;  // x lane
+  const int part = XS == 1 ? 0 : (ix >> 5) & 1;
+  const int iy = XS == 1 ? ix : (ix < 0 ? -1 : 32 * (ix >> 6) + (ix & 31));
   // y fold factors e^{2 pi i t r / R} (M / R = 32)
   c32 wf[NT];
 #pragma unroll
@@ -196,56 +247,117 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
   __syncthreads();
   // The two groups run separate loops (wave-uniform split) with the same
   // number of barriers, so their register live ranges do not overlap.
-  if (iy >= 0) {
-    const int vo = iy * (int)sizeof(float2);
+  if (ix >= 0) {
     for (int s = 0; s <= steps; ++s) {
       if (s < steps && iy < W) {
-        // Buffer loads: one VGPR offset per lane, the column offset in an SGPR.
-        const __amdgpu_buffer_rsrc_t kr =
-            box_rsrc(kern + ((size_t)q * steps + s) * W * W, W * W);
-        c32 c[F];
+        c32* Tr = T + (s & 1) * W * TP + iy * TP;
+        if constexpr (XS == 1) {
+          // Buffer loads: one VGPR offset per lane, the column offset in an SGPR.
+          const int vo = iy * (int)sizeof(float2);
+          const __amdgpu_buffer_rsrc_t kr =
+              box_rsrc(kern + ((size_t)q * steps + s) * W * W, W * W);
+          c32 c[F];
 #pragma unroll
-        for (int h = 0; h < F; ++h) c[h] = c32{0.f, 0.f};
-        // Columns kx = -BB..BB in blocks of XNB; the loads of block b + 1 are
-        // issued before block b is consumed.
-        constexpr int NBLK = (W + XNB - 1) / XNB;
-        c32 ka[2][XNB], ma[2][XNB];
+          for (int h = 0; h < F; ++h) c[h] = c32{0.f, 0.f};
+          // Columns kx = -BB..BB in blocks of XNB; the loads of block b + 1
+          // are issued before block b is consumed.
+          constexpr int NBLK = (W + XNB - 1) / XNB;
+          c32 ka[2][XNB], ma[2][XNB];
 #pragma unroll
-        for (int i = 0; i < XNB; ++i) {
-          const int so = i * W * (int)sizeof(float2);
-          ka[0][i] = load_c32(kr, vo, so);
-          ma[0][i] = load_c32(mr, vo, so);
-        }
+          for (int i = 0; i < XNB; ++i) {
+            const int so = i * W * (int)sizeof(float2);
+            ka[0][i] = load_c32(kr, vo, so);
+            ma[0][i] = load_c32(mr, vo, so);
+          }
 #pragma unroll
-        for (int blk = 0; blk < NBLK; ++blk) {
-          __builtin_amdgcn_sched_barrier(0);
-          if (blk + 1 < NBLK) {
+          for (int blk = 0; blk < NBLK; ++blk) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (blk + 1 < NBLK) {
+#pragma unroll
+              for (int i = 0; i < XNB; ++i) {
+                const int col = (blk + 1) * XNB + i;
+                if (col < W) {
+                  const int so = col * W * (int)sizeof(float2);
+                  ka[(blk + 1) & 1][i] = load_c32(kr, vo, so);
+                  ma[(blk + 1) & 1][i] = load_c32(mr, vo, so);
+                }
+              }
+            }
 #pragma unroll
             for (int i = 0; i < XNB; ++i) {
-              const int col = (blk + 1) * XNB + i;
+              const int col = blk * XNB + i;
               if (col < W) {
-                const int so = col * W * (int)sizeof(float2);
-                ka[(blk + 1) & 1][i] = load_c32(kr, vo, so);
-                ma[(blk + 1) & 1][i] = load_c32(mr, vo, so);
+                const int kx = col - BB;
+                const int h = ((kx % F) + F) % F;
+                c[h] = cmac(c[h], ka[blk & 1][i], ma[blk & 1][i]);
               }
             }
           }
+          fft_reg<+1, F>(c);
 #pragma unroll
-          for (int i = 0; i < XNB; ++i) {
-            const int col = blk * XNB + i;
-            if (col < W) {
-              const int kx = col - BB;
-              const int h = ((kx % F) + F) % F;
-              const c32 a = ka[blk & 1][i], b = ma[blk & 1][i];
-              c[h].re += a.re * b.re - a.im * b.im;
-              c[h].im += a.re * b.im + a.im * b.re;
+          for (int jj = 0; jj < F; ++jj) Tr[jj] = c[jj];
+        } else {
+          // Lane `part` streams its NR runs of 16 columns from col0, two
+          // columns (one pair) per 16-byte load.
+          constexpr int NP = 8 * C::NR, XP = XNB / 2;
+          const int col0 = part ? C::xcol0(1) : C::xcol0(0);
+          const int vo = (((col0 + 1) >> 1) * W + iy) * (int)(2 * sizeof(float2));
+          const __amdgpu_buffer_rsrc_t kr =
+              box_rsrc(kern + ((size_t)q * steps + s) * BOX, BOX);
+          c32 c[F / 2];  // bins 16 part + h
+#pragma unroll
+          for (int h = 0; h < F / 2; ++h) c[h] = c32{0.f, 0.f};
+          constexpr int NBLK = (NP + XP - 1) / XP;
+          c32x2 ka[2][XP], ma[2][XP];
+#pragma unroll
+          for (int i = 0; i < XP; ++i) {
+            const int so = ((i / 8) * 16 + i % 8) * W * (int)(2 * sizeof(float2));
+            ka[0][i] = load_c32x2(kr, vo, so);
+            ma[0][i] = load_c32x2(mr, vo, so);
+          }
+#pragma unroll
+          for (int blk = 0; blk < NBLK; ++blk) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (blk + 1 < NBLK) {
+#pragma unroll
+              for (int i = 0; i < XP; ++i) {
+                const int k = (blk + 1) * XP + i;
+                if (k < NP) {
+                  const int so = ((k / 8) * 16 + k % 8) * W * (int)(2 * sizeof(float2));
+                  ka[(blk + 1) & 1][i] = load_c32x2(kr, vo, so);
+                  ma[(blk + 1) & 1][i] = load_c32x2(mr, vo, so);
+                }
+              }
+            }
+#pragma unroll
+            for (int i = 0; i < XP; ++i) {
+              const int k = blk * XP + i;
+              if (k < NP) {
+                const int h = 2 * (k % 8);
+                c[h] = cmac(c[h], ka[blk & 1][i].a, ma[blk & 1][i].a);
+                c[h + 1] = cmac(c[h + 1], ka[blk & 1][i].b, ma[blk & 1][i].b);
+              }
             }
           }
-        }
-        fft_reg<+1, F>(c);
-        c32* Tb = T + (s & 1) * W * TP;
+          // radix-2 step across the lane pair through the tile row:
+          //   lane 0: FFT_16(c[h] + c[h + 16])                        -> outputs 2k
+          //   lane 1: FFT_16((c[h] - c[h + 16]) e^{2 pi i h / 32})    -> outputs 2k + 1
 #pragma unroll
-        for (int jj = 0; jj < F; ++jj) Tb[iy * TP + jj] = c[jj];
+          for (int h = 0; h < F / 2; ++h) Tr[F / 2 * part + h] = c[h];
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          const float sg = part ? -1.f : 1.f;
+#pragma unroll
+          for (int h = 0; h < F / 2; ++h) {
+            const c32 o = Tr[F / 2 * (1 - part) + h];
+            c[h] = c32{__builtin_fmaf(sg, c[h].re, o.re), __builtin_fmaf(sg, c[h].im, o.im)};
+          }
+#pragma unroll
+          for (int h = 1; h < F / 2; ++h) c[h] = cmul(c[h], tw32[h * part]);
+          fft_reg<+1, F / 2>(c);
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+          for (int k = 0; k < F / 2; ++k) Tr[2 * k + part] = c[k];
+        }
       }
       __syncthreads();
     }
@@ -270,7 +382,7 @@ __global__ __launch_bounds__(CoarseCfg<M>::THREADS) __attribute__((amdgpu_waves_
             const int ky = h + F * t;
             if (ky >= -BB && ky <= BB) {
               const c32 e = Tb[(ky + BB) * TP + j];
-              v = cadd(v, t == 0 ? e : cmul(e, wf[t - TMIN]));
+              v = t == 0 ? cadd(v, e) : cmac(v, e, wf[t - TMIN]);
             }
           }
           d[h] = cmul(v, ty[h]);
@@ -398,19 +510,25 @@ template <int TMIN, int TMAX>
 __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ Ib, int n, int G,
                                                     int nimg, const float2* __restrict__ tw_n,
                                                     float* __restrict__ out) {
-  constexpr int F = 32;
+  constexpr int F = 32, NT = TMAX - TMIN + 1;
   extern __shared__ c32 smem[];
   constexpr int UP = 32 * (-TMIN) + 1;  // U row: g = 0 .. 32 |TMIN|, zero beyond G
-  c32* U = smem;           // F x UP
-  c32* twn = U + F * UP;   // e^{+2 pi i k / n}, k < n
   const int P = n / F, C = n / F;
+  c32* U = smem;            // F x UP
+  c32* twh = U + F * UP;    // [h][c] = e^{2 pi i h c / n}, h < 32, c < C
+  c32* twt = twh + F * C;   // [t - TMIN][c] = e^{2 pi i t c / C}
   const int L = blockIdx.x;
   const int slot = L / kXcds;
   const int p = slot % P;
   const int img = (slot / P) * kXcds + L % kXcds;
   if (img >= nimg) return;  // whole work-group, before any barrier
   const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += 256) twn[i] = tw_lookup<+1>(tw_n, i);
+  // Class-indexed tables: lane c of a phase-2 wave reads consecutive entries.
+  for (int i = tid; i < F * C; i += 256) twh[i] = tw_lookup<+1>(tw_n, (i / C) * (i % C));
+  for (int i = tid; i < NT * C; i += 256) {
+    const int t = i / C + TMIN, c = i % C;
+    twt[i] = tw_lookup<+1>(tw_n, (((F * t * c) % n) + n) % n);
+  }
   for (int i = tid; i < F * (UP - (G + 1)); i += 256) {
     const int row = i / (UP - (G + 1)), col = G + 1 + i % (UP - (G + 1));
     U[row * UP + col] = c32{0.f, 0.f};
@@ -432,7 +550,7 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
       // Opaque to the optimiser: keeps each band row block's loads after the
       // previous block instead of hoisting every load to the top.
       asm volatile("" : "+v"(vo));
-      const c32 w = twn[(((F * t * p) % n) + n) % n];  // e^{2 pi i t p / P}
+      const c32 w = twt[(t - TMIN) * C + p];  // e^{2 pi i t p / P}
       c32 e[F];
 #pragma unroll
       for (int h = 0; h < F; ++h) {
@@ -443,10 +561,11 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
         if (inb) e[h] = load_c32(br, vo, (gy + G) * (G + 1) * (int)sizeof(float2));
       }
 #pragma unroll
-      for (int h = 0; h < F; ++h) u[h] = cadd(u[h], t == 0 ? e[h] : cmul(e[h], w));
+      for (int h = 0; h < F; ++h) u[h] = t == 0 ? cadd(u[h], e[h]) : cmac(u[h], e[h], w);
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int h = 0; h < F; ++h) u[h] = cmul(u[h], twn[h * p]);
+    for (int h = 0; h < F; ++h) u[h] = cmul(u[h], twh[h * C + p]);
     fft_reg<+1, F>(u);
     if (half == 0) {
 #pragma unroll
@@ -485,7 +604,8 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     __syncthreads();
   }
   // ---- phase 2 ----
-  float* img_out = out + (size_t)img * n * n;
+  const __amdgpu_buffer_rsrc_t orc = __builtin_amdgcn_make_buffer_rsrc(
+      out + (size_t)img * n * n, (short)0, n * n * (int)sizeof(float), 0x00020000);
   for (int task = tid; task < (F / 2) * C; task += 256) {
     const int pr = task / C, c = task % C;
     const c32* zr = U + pr * ZW + GP;  // zr[g], |g| <= GP
@@ -495,24 +615,28 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
 #pragma unroll
     for (int t = TMIN; t <= TMAX; ++t) {
       __builtin_amdgcn_sched_barrier(0);
-      const c32 w = twn[(((F * t * c) % n) + n) % n];  // e^{2 pi i t c / C}
+      const c32 w = twt[(t - TMIN) * C + c];  // e^{2 pi i t c / C}
 #pragma unroll
       for (int h = 0; h < F; ++h) {
         const int g = h + F * t;
         // Z is zero beyond G, so every fold term is read unconditionally
         const c32 z = zr[g];
-        d[h] = cadd(d[h], t == 0 ? z : cmul(z, w));
+        d[h] = t == 0 ? cadd(d[h], z) : cmac(d[h], z, w);
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int h = 0; h < F; ++h) d[h] = cmul(d[h], twn[h * c]);
+    for (int h = 0; h < F; ++h) d[h] = cmul(d[h], twh[h * C + c]);
     fft_reg<+1, F>(d);
-    float* ra = img_out + (size_t)(P * (2 * pr) + p) * n + c;
-    float* rb = ra + (size_t)P * n;
+    // Rows y = P (2 pr) + p and y + P, columns x = c + C i: vector buffer
+    // stores with the column offset C i in an SGPR.
+    const int va = ((P * (2 * pr) + p) * n + c) * (int)sizeof(float);
+    const int vb = va + P * n * (int)sizeof(float);
 #pragma unroll
     for (int i = 0; i < F; ++i) {
-      ra[(size_t)C * i] = d[i].re;
-      rb[(size_t)C * i] = d[i].im;
+      const int so = C * i * (int)sizeof(float);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].re), orc, va, so, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].im), orc, vb, so, 0);
     }
   }
 }
@@ -591,25 +715,28 @@ hipError_t launch_rows_band(int n, int n1, int B, int nclips, const float* masks
 
 template <int N1, int N2>
 static hipError_t cols_band_n(int B, int BB, int nclips, const float2* tmp, const float2* tw_n,
-                              const float* sinc_t, float2* Mq, hipStream_t s) {
+                              const float* sinc_t, float2* Mq, float2* Mp, hipStream_t s) {
   constexpr int TEAMS = 128 / N2 > 0 ? 128 / N2 : 1;
   dim3 grid((BB + 1 + TEAMS - 1) / TEAMS, nclips);
   HUV_LAUNCH((k_cols_band<N1, N2, TEAMS>), grid, dim3(TEAMS * N2), 0, s, tmp, B, BB, tw_n, sinc_t,
-             Mq);
+             Mq, Mp);
 }
 
 hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const float2* tmp,
-                            const float2* tw_n, const float* sinc_t, float2* Mq, hipStream_t s) {
+                            const float2* tw_n, const float* sinc_t, float2* Mq, float2* Mp,
+                            hipStream_t s) {
   if (BB < B || 2 * BB + 1 > n) return hipErrorInvalidValue;
-  HUV_BAND_FFT_SWITCH(cols_band_n, B, BB, nclips, tmp, tw_n, sinc_t, Mq, s)
+  HUV_BAND_FFT_SWITCH(cols_band_n, B, BB, nclips, tmp, tw_n, sinc_t, Mq, Mp, s)
 }
 
-template <int M, int XNB>
+template <int M, int XNB, int XS = 1>
 static hipError_t fields_m(int nclips, int nplanes, int Kmax, const float2* kern, const float* lam,
                            const float2* Mq, const float2* tw_m, float* coarse, hipStream_t s) {
-  using C = CoarseCfg<M>;
+  using C = CoarseCfg<M, XS>;
+  // the two-lane x pass needs every column in [-1, W)
+  if constexpr (XS == 2 && !C::SPLIT_OK) return hipErrorInvalidValue;
   const size_t shmem = (size_t)C::LDS_C32 * sizeof(c32);
-  auto kernel = k_socs_fields<M, XNB>;
+  auto kernel = k_socs_fields<M, XNB, XS>;
   // Once per instantiation (thread-safe static), never inside a graph capture.
   static const hipError_t attr = hipFuncSetAttribute(
       (const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
@@ -620,26 +747,35 @@ static hipError_t fields_m(int nclips, int nplanes, int Kmax, const float2* kern
 }
 
 template <int M>
-static hipError_t fields_x(int xnb, int nclips, int nplanes, int Kmax, const float2* kern,
-                           const float* lam, const float2* Mq, const float2* tw_m, float* coarse,
-                           hipStream_t s) {
-  switch (xnb ? xnb : kFieldsXnbDefault) {
-    case 4: return fields_m<M, 4>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 6: return fields_m<M, 6>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 8: return fields_m<M, 8>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+static hipError_t fields_x(int xnb, int xlanes, int nclips, int nplanes, int Kmax,
+                           const float2* kern, const float* lam, const float2* Mq,
+                           const float2* tw_m, float* coarse, hipStream_t s) {
+  if (!xlanes) xlanes = fields_lanes_default(M);
+  if (!xnb) xnb = xlanes == 2 ? kFieldsXnbDefault2 : kFieldsXnbDefault;
+  switch (xlanes * 100 + xnb) {
+    case 104: return fields_m<M, 4>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 106: return fields_m<M, 6>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 108: return fields_m<M, 8>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 204: return fields_m<M, 4, 2>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 206: return fields_m<M, 6, 2>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 208: return fields_m<M, 8, 2>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_socs_fields(int m, int xnb, int nclips, int nplanes, int Kmax,
+int fields_lanes_default(int m) { return m >= 64 ? 2 : 1; }
+
+bool fields_lanes_supported(int m, int xlanes) { return xlanes == 1 || (xlanes == 2 && m >= 64); }
+
+hipError_t launch_socs_fields(int m, int xnb, int xlanes, int nclips, int nplanes, int Kmax,
                               const float2* kern, const float* lam, const float2* Mq,
                               const float2* tw_m, float* coarse, hipStream_t s) {
   switch (m) {
-    case 32: return fields_x<32>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 64: return fields_x<64>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 128: return fields_x<128>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 192: return fields_x<192>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 256: return fields_x<256>(xnb, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 32: return fields_x<32>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 64: return fields_x<64>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 128: return fields_x<128>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 192: return fields_x<192>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 256: return fields_x<256>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -687,13 +823,13 @@ hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2*
 template <int TMIN, int TMAX>
 static hipError_t band_image_t(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
                                float* out, hipStream_t s) {
-  constexpr int UP = 32 * (-TMIN) + 1;
-  const size_t shmem = ((size_t)32 * UP + n) * sizeof(c32);
+  constexpr int UP = 32 * (-TMIN) + 1, NT = TMAX - TMIN + 1;
+  auto bytes = [](int nn) { return ((size_t)32 * UP + (size_t)(32 + NT) * (nn / 32)) * sizeof(c32); };
+  const size_t shmem = bytes(n);
   auto kern = k_band_image<TMIN, TMAX>;
   // Sized for the largest grid (n = 2048), once.
   static const hipError_t attr = hipFuncSetAttribute(
-      (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-      (int)(((size_t)32 * UP + 2048) * sizeof(c32)));
+      (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes(2048));
   if (attr != hipSuccess) return attr;
   const int groups = (nimg + kXcds - 1) / kXcds;
   HUV_LAUNCH(kern, dim3(groups * kXcds * (n / 32)), dim3(256), shmem, s, Ib, n, G, nimg, tw_n,

@@ -20,18 +20,29 @@ int coarse_band(int m);
 // count).  n1 = 0 selects the default.
 int band_fft_default(int n);
 int band_fft_options(int n, int* n1s);
-// x-pass load block of the field kernel: kFieldsXnbDefault, or 4 / 6 / 8.
+// x pass of the field kernel: xlanes = 1 (one lane per band row, every
+// column) or 2 (two lanes per row, half of the bins each; m >= 64, reads the
+// column-paired layouts); xnb = 4, 6 or 8 columns per load block.  Zero
+// selects fields_lanes_default(m) and kFieldsXnbDefault / kFieldsXnbDefault2.
 constexpr int kFieldsXnbDefault = 6;
+constexpr int kFieldsXnbDefault2 = 6;
+int fields_lanes_default(int m);
+bool fields_lanes_supported(int m, int xlanes);
 
 hipError_t launch_rows_band(int n, int n1, int B, int nclips, const float* masks,
                             const float2* tw_n, float2* tmp, hipStream_t s);
-// Mq: [nclips][2BB+1][2BB+1] band spectrum, zero beyond |k| <= B.
+// Mq: [nclips][W][W] band spectrum (W = 2BB+1, [kx + BB][ky + BB]), zero
+// beyond |k| <= B.  Mp (optional, may be null): the same values column-paired,
+// [nclips][(W + 1) / 2][W][2] with column kx + BB at [(kx + BB + 1) / 2][.][(kx + BB + 1) % 2]
+// and a zero column in front (the layout of the two-lane field kernel).
 hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const float2* tmp,
-                            const float2* tw_n, const float* sinc_t, float2* Mq, hipStream_t s);
-// kern: [Q][nplanes][Kmax][W][W] (W = 2 coarse_band(m) + 1, Q = m / 32; copy q
-// multiplied by e^{2 pi i kx q / m}), lam: [nplanes][Kmax]; Mq at the same W;
-// coarse: [nclips][nplanes][m][m].  xnb = 0 selects the default.
-hipError_t launch_socs_fields(int m, int xnb, int nclips, int nplanes, int Kmax,
+                            const float2* tw_n, const float* sinc_t, float2* Mq, float2* Mp,
+                            hipStream_t s);
+// kern: [Q][nplanes][Kmax] boxes (W = 2 coarse_band(m) + 1, Q = m / 32; copy q
+// multiplied by e^{2 pi i kx q / m}), lam: [nplanes][Kmax]; Mq: one box per
+// clip; coarse: [nclips][nplanes][m][m].  Boxes are W x W ([kx + BB][ky + BB])
+// for xlanes = 1 and column-paired (W + 1) x W (launch_cols_band's Mp) for 2.
+hipError_t launch_socs_fields(int m, int xnb, int xlanes, int nclips, int nplanes, int Kmax,
                               const float2* kern, const float* lam, const float2* Mq,
                               const float2* tw_m, float* coarse, hipStream_t s);
 hipError_t launch_coarse_rows(int m, int B, int nimg, const float* coarse, const float2* tw_m,

@@ -180,12 +180,15 @@ typedef struct {
 
 /* Launch shape of the SOCS path for one (n, band, planes) combination:
  * coarse grid m (32 Q; sets the number of 32-point transform classes Q),
- * band-FFT factor n1 (four-step n = n1 x n / n1), x-pass load block xnb.
- * Zero fields select the built-in defaults. */
+ * band-FFT factor n1 (four-step n = n1 x n / n1), and the field kernel's
+ * x pass: xlanes = 1 or 2 lanes per band row (2 needs m >= 64), xnb = 4, 6
+ * or 8 band columns per load block.  Zero fields select the built-in
+ * defaults. */
 typedef struct {
   int m;
   int n1;
   int xnb;
+  int xlanes;
 } huv_socs_tuning;
 
 /* Autotuner: when on, the first huv_socs_images / huv_socs_images_batch call

@@ -99,25 +99,60 @@ def test_autotuner_records_a_valid_shape_within_the_gate():
     H.engine.socs_images_device(ctx, masks, planes, out)
     torch.cuda.synchronize()
     t = ctx.tuning(1024, B, 1, planes[0].host.num_kernels)
-    print("band", B, "tuned (m, n1, xnb)", t)
+    print("band", B, "tuned (m, n1, xnb, xlanes)", t)
     assert t is not None and t[0] >= 4 * B + 1 and t[0] in (32, 64, 128, 192, 256)
-    assert t[1] in (16, 32) and t[2] in (4, 6, 8)
+    assert t[1] in (16, 32) and t[2] in (4, 6, 8) and t[3] in (1, 2)
     for c in range(2):
         err = np.abs(out[c, 0].cpu().numpy() - _oracle(eng, masks[c].cpu().numpy())).max()
         assert err <= TOL
     # every forced shape stays within the gate too
     for m in (128, 256):
-        ctx.set_tuning(1024, B, 1, planes[0].host.num_kernels, m=m, n1=16, xnb=4)
-        H.engine.socs_images_device(ctx, masks, planes, out)
-        err = np.abs(out[1, 0].cpu().numpy() - _oracle(eng, masks[1].cpu().numpy())).max()
-        print("forced m", m, "err", err)
-        assert err <= TOL
+        for xlanes in (1, 2):
+            ctx.set_tuning(1024, B, 1, planes[0].host.num_kernels, m=m, n1=16, xnb=4,
+                           xlanes=xlanes)
+            H.engine.socs_images_device(ctx, masks, planes, out)
+            err = np.abs(out[1, 0].cpu().numpy() - _oracle(eng, masks[1].cpu().numpy())).max()
+            print("forced m", m, "xlanes", xlanes, "err", err)
+            assert err <= TOL
     with pytest.raises(H.HuvError):
         ctx.set_tuning(1024, B, 1, 12, m=64)   # below 4B + 1
     with pytest.raises(H.HuvError):
         ctx.set_tuning(1024, B, 1, 12, n1=8)   # no 8 x 128 factorisation
     with pytest.raises(H.HuvError):
         ctx.set_tuning(1024, B, 1, 12, xnb=5)
+    with pytest.raises(H.HuvError):
+        ctx.set_tuning(1024, B, 1, 12, xlanes=3)
+
+
+def test_field_kernel_forms_agree_at_every_coarse_size():
+    """Every x-pass form of the field kernel (one or two lanes per band row,
+    each load block) against the one-lane reference form and the oracle, at
+    each coarse grid the band allows; two lanes are refused below m = 64."""
+    import torch
+    eng = _arf_engine(256, 2.0, 10)
+    ctx = eng.context()
+    planes = [eng.device_kernel_set(0.0), eng.device_kernel_set(40.0)]
+    B, K = planes[0].band, max(p.host.num_kernels for p in planes)
+    masks = torch.from_numpy(_masks(256, 3, 11)).cuda()
+    ref = torch.empty((3, 2, 256, 256), device="cuda")
+    out = torch.empty_like(ref)
+    sizes = [m for m in (32, 64, 128, 192, 256) if m >= 4 * B + 1]
+    assert 32 in sizes or 64 in sizes
+    for m in sizes:
+        ctx.set_tuning(256, B, 2, K, m=m, xnb=6, xlanes=1)
+        H.engine.socs_images_device(ctx, masks, planes, ref)
+        for xlanes, xnb in ((1, 4), (1, 8), (2, 4), (2, 6), (2, 8)):
+            if m == 32 and xlanes == 2:
+                with pytest.raises(H.HuvError):
+                    ctx.set_tuning(256, B, 2, K, m=m, xnb=xnb, xlanes=xlanes)
+                continue
+            ctx.set_tuning(256, B, 2, K, m=m, xnb=xnb, xlanes=xlanes)
+            H.engine.socs_images_device(ctx, masks, planes, out)
+            torch.cuda.synchronize()
+            d = (out - ref).abs().max().item()
+            err = np.abs(out[2, 0].cpu().numpy() - _oracle(eng, masks[2].cpu().numpy())).max()
+            print("m", m, "xlanes", xlanes, "xnb", xnb, "vs one-lane", d, "vs oracle", err)
+            assert d <= TOL and err <= TOL
 
 
 def test_graph_is_dropped_with_its_kernel_sets():
