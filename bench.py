@@ -350,7 +350,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(clips and targets are independent; the sharding design is in DESIGN.md)",
+              "(clips and targets are independent: each GPU would take its own share of the batch)",
               file=sys.stderr)
         sys.exit(2)
 

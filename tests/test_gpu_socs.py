@@ -1,6 +1,6 @@
 """GPU parity of the batched SOCS image path against the FP64 oracle.
 
-Gate (BASELINE north star): max |I_gpu - I_oracle| <= 1e-5 of the clear-field
+Gate: max |I_gpu - I_oracle| <= 1e-5 of the clear-field
 intensity (images are normalised, clear field = 1).  Same kernel sets, same
 seeded masks; the oracle computes the reference's structure in FP64
 (aerial.rs:1682-1759).
