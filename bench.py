@@ -340,7 +340,7 @@ def main():
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--warm-pairs", type=int, default=1024)
     ap.add_argument("--warm-epochs", type=int, default=20)
-    ap.add_argument("--warm-eval", type=int, default=64)
+    ap.add_argument("--warm-eval", type=int, default=512)
     ap.add_argument("--uneven-clips", type=int, default=1000)
     ap.add_argument("--ilt-targets", type=int, default=64)
     ap.add_argument("--ilt-iterations", type=int, default=50)

@@ -694,13 +694,14 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   int Kmax;
   st = plane_pack(c, nplanes, planes, m, xlanes == 2, &kern, &lam, &Kmax);
   if (st != HUV_OK) return st;
-  float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)nimg * m * m * sizeof(float), e);
+  const int S = huvk::fields_splits(m, nclips, Kmax);
+  float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)S * nimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");
   const float2* tw_m = c->twiddles(m, e);
   if (!tw_m) return hip_fail(e, "twiddle table");
   hipEvent_t ev;
   c->stage_begin(ST_FIELDS, &ev);
-  e = huvk::launch_socs_fields(m, xnb, xlanes, nclips, nplanes, Kmax, kern, lam,
+  e = huvk::launch_socs_fields(m, xnb, xlanes, nclips, nplanes, Kmax, S, kern, lam,
                                xlanes == 2 ? Mp : Mq, tw_m, coarse, c->stream());
   if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
   c->stage_end(ST_FIELDS, ev);
@@ -712,7 +713,7 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   float2* Ib = (float2*)c->ws.get(Workspace::IBAND, (size_t)nimg * (G + 1) * (2 * G + 1) * sizeof(float2), e);
   if (!Ib) return hip_fail(e, "workspace");
   c->stage_begin(ST_COARSE_ROWS, &ev);
-  e = huvk::launch_coarse_rows(m, B, nimg, coarse, tw_m, tmp1, c->stream());
+  e = huvk::launch_coarse_rows(m, B, nimg, S, coarse, tw_m, tmp1, c->stream());
   if (e != hipSuccess) return hip_fail(e, "coarse_rows launch");
   c->stage_end(ST_COARSE_ROWS, ev);
   c->stage_begin(ST_COARSE_COLS, &ev);

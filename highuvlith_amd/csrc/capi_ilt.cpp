@@ -37,10 +37,18 @@ namespace {
 constexpr int kBpi = 64;  // reduction blocks per image
 
 struct IltBuffers {
-  float *m, *t, *image, *G, *dcdm, *reg_grad;
+  float *m, *t, *image, *G, *dcdm, *reg_grad, *G0g;
   float2 *Gb, *Rout, *H;
   double *fid_part, *reg_part, *cost_d;
+  int KS = 1;  // adjoint work-groups per image (kernels dealt over them)
 };
+
+// One adjoint work-group per kernel of the largest condition.
+static int ilt_splits(const huv_ilt* p) {
+  int ks = 1;
+  for (const huv_kset* k : p->conds) ks = std::max(ks, k->K);
+  return ks;
+}
 
 huv_status ilt_buffers(Ctx* c, const huv_ilt* p, int T, IltBuffers& b) {
   hipError_t e;
@@ -59,12 +67,15 @@ huv_status ilt_buffers(Ctx* c, const huv_ilt* p, int T, IltBuffers& b) {
   if (!a) return hip_fail(e, "ILT workspace");
   b.dcdm = (float*)a;
   b.reg_grad = b.dcdm + (size_t)T * N;
-  const size_t nGb = TC * W2 * W2, nR = TC * W * W, nH = (size_t)T * (p->B + 1) * W;
-  a = (char*)c->ws.get(Workspace::ILT4, (nGb + nR + nH) * sizeof(float2), e);
+  b.KS = ilt_splits(p);
+  const size_t nGb = TC * W2 * W2, nR = TC * b.KS * W * W, nH = (size_t)T * (p->B + 1) * W;
+  const size_t nG0 = TC * p->m * p->m;  // floats
+  a = (char*)c->ws.get(Workspace::ILT4, (nGb + nR + nH) * sizeof(float2) + nG0 * sizeof(float), e);
   if (!a) return hip_fail(e, "ILT workspace");
   b.Gb = (float2*)a;
   b.Rout = b.Gb + nGb;
   b.H = b.Rout + nR;
+  b.G0g = (float*)(b.H + nH);
   a = (char*)c->ws.get(Workspace::ILT5, (TC * kBpi + (size_t)T * kBpi + T) * sizeof(double), e);
   if (!a) return hip_fail(e, "ILT workspace");
   b.fid_part = (double*)a;
@@ -118,13 +129,13 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const float* theta,
   const double n2 = (double)n * n;
   const float g0_scale = 1.0f / ((float)m * (float)m);
   const float lam_scale = (float)n2;  // device lambda carries 1/n^4
-  e = huvk::launch_ilt_adjoint(m, T * C, Mq, b.Gb, B, C, dK, dkern, dlam, lam_scale, g0_scale,
-                               (float)p->flare, b.Rout, s);
+  e = huvk::launch_ilt_adjoint(m, T * C, b.KS, Mq, b.Gb, B, C, dK, dkern, dlam, lam_scale,
+                               g0_scale, (float)p->flare, b.G0g, b.Rout, s);
   if (e != hipSuccess) return hip_fail(e, "ilt adjoint launch");
   c->stage_end(ST_ILT_ADJOINT, ev);
   c->stage_begin(ST_ILT_OTHER, &ev);
-  e = huvk::launch_ilt_hband(b.Rout, B, T, C, p->weight_d, (float)((1.0 - p->cfg.absorber_re) / n2),
-                             b.H, s);
+  e = huvk::launch_ilt_hband(b.Rout, B, T, C, b.KS, p->weight_d,
+                             (float)((1.0 - p->cfg.absorber_re) / n2), b.H, s);
   if (e != hipSuccess) return hip_fail(e, "ilt hband launch");
   const float2* tw_n = c->twiddles(n, e);
   if (!tw_n) return hip_fail(e, "twiddle table");
@@ -236,21 +247,44 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   float* dir = (float*)(vec + 2 * vb);
   float* trial = (float*)(vec + 3 * vb);
   const int nblk = 64;
-  char* sc = (char*)c->ws.get(Workspace::ILT7, ((size_t)T * nblk + 4 * (size_t)T) * sizeof(double) + T * sizeof(int), e);
+  constexpr int kRed = 3;  // reductions fetched together
+  constexpr int kLists = 4;
+  char* sc = (char*)c->ws.get(
+      Workspace::ILT7,
+      ((size_t)kRed * T * nblk + (size_t)(kRed + 2) * T) * sizeof(double) + (size_t)kLists * T * sizeof(int), e);
   if (!sc) return hip_fail(e, "ILT optimizer workspace");
   double* part = (double*)sc;
-  double* red = part + (size_t)T * nblk;
-  double* coefA = red + T;
+  double* red = part + (size_t)kRed * T * nblk;
+  double* coefA = red + (size_t)kRed * T;
   double* coefB = coefA + T;
-  int* sel = (int*)(coefB + T);
+  int* rows_d = (int*)(coefB + T);
 
   auto fail_e = [&](const char* what) { return hip_fail(e, what); };
-  std::vector<double> h_red(T), h_a(T), h_b(T);
-  std::vector<int> h_sel(T);
-  auto reduce = [&](const float* x, const float* y, int mode) -> bool {
-    e = huvk::launch_vec_reduce(x, y, N, T, nblk, mode, part, red, s);
+  std::vector<double> h_red((size_t)kRed * T), h_a(T), h_b(T);
+  // Vector operations touch only the listed target rows: list slot k is
+  // uploaded before the operations that read it (stream order).
+  auto upload_rows = [&](int k, const std::vector<int>& rows) -> const int* {
+    if ((int)rows.size() == T) return nullptr;  // every row, in order
+    if (rows.empty()) return rows_d;
+    e = hipMemcpyAsync(rows_d + (size_t)k * T, rows.data(), rows.size() * sizeof(int),
+                       hipMemcpyHostToDevice, s);
+    return e == hipSuccess ? rows_d + (size_t)k * T : nullptr;
+  };
+  // Up to kRed per-target reductions over `rows`, one synchronisation;
+  // h_red[r * T + t] holds reduction r for each listed t.
+  struct Red { const float* x; const float* y; int mode; };
+  auto reduce = [&](std::initializer_list<Red> ops, const std::vector<int>& rows) -> bool {
+    if (rows.empty()) return true;
+    const int* rd = upload_rows(0, rows);
     if (e != hipSuccess) return false;
-    e = hipMemcpyAsync(h_red.data(), red, T * sizeof(double), hipMemcpyDeviceToHost, s);
+    int r = 0;
+    for (const Red& op : ops) {
+      e = huvk::launch_vec_reduce(op.x, op.y, N, (int)rows.size(), rd, nblk, op.mode,
+                                  part + (size_t)r * T * nblk, red + (size_t)r * T, s);
+      if (e != hipSuccess) return false;
+      ++r;
+    }
+    e = hipMemcpyAsync(h_red.data(), red, (size_t)r * T * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return false;
     e = hipStreamSynchronize(s);
     return e == hipSuccess;
@@ -258,6 +292,22 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   auto upload_coefs = [&]() -> bool {
     e = hipMemcpyAsync(coefA, h_a.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(coefB, h_b.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
+    return e == hipSuccess;
+  };
+  // d = a g + b d on the listed rows
+  auto dir_rows = [&](const float* g, const std::vector<int>& rows) -> bool {
+    if (rows.empty()) return true;
+    if (!upload_coefs()) return false;
+    const int* rd = upload_rows(1, rows);
+    if (e != hipSuccess) return false;
+    e = huvk::launch_vec_dir(g, coefA, coefB, N, (int)rows.size(), rd, dir, s);
+    return e == hipSuccess;
+  };
+  auto copy_rows = [&](const float* src, float* dst, const std::vector<int>& rows) -> bool {
+    if (rows.empty()) return true;
+    const int* rd = upload_rows(2, rows);
+    if (e != hipSuccess) return false;
+    e = huvk::launch_vec_copy_rows(src, N, (int)rows.size(), rd, dst, s);
     return e == hipSuccess;
   };
   // Targets that are done cost nothing: evaluations gather the rows of the
@@ -319,13 +369,12 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       term[t] = 3;
     }
   }
-  if (!reduce(grad, grad, 0)) return fail_e("dot");
+  if (!reduce({{grad, grad, 0}}, all_t)) return fail_e("dot");
   for (int t = 0; t < T; ++t) gnorm[t] = std::sqrt(h_red[t]);
   for (int t = 0; t < T; ++t) { h_a[t] = -1.0; h_b[t] = 0.0; }
-  if (!upload_coefs()) return fail_e("coef upload");
-  e = huvk::launch_vec_dir(grad, coefA, coefB, N, T, dir, s);
-  if (e != hipSuccess) return fail_e("dir");
+  if (!dir_rows(grad, all_t)) return fail_e("dir");
   const bool cg = use_cg != 0;
+  std::vector<int> lst, rst;
 
   for (int iter = 0; iter < max_iterations; ++iter) {
     bool any = false;
@@ -334,32 +383,27 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     for (int t = 0; t < T; ++t) { attempt[t] = 0; accepted[t] = 0; searching[t] = 0; }
     // Up to two attempts (CG direction, then steepest descent).
     for (int att = 0; att < 2; ++att) {
-      if (!reduce(dir, nullptr, 1)) return fail_e("maxabs");
-      for (int t = 0; t < T; ++t) dmax[t] = h_red[t];
-      if (!reduce(grad, dir, 0)) return fail_e("dot");
-      bool restart = false;
-      for (int t = 0; t < T; ++t) {
-        h_a[t] = 0.0;
-        h_b[t] = 1.0;
-        if (!active[t] || accepted[t] || attempt[t] != att) continue;
+      lst.clear();
+      for (int t = 0; t < T; ++t)
+        if (active[t] && !accepted[t] && attempt[t] == att) lst.push_back(t);
+      if (!reduce({{dir, nullptr, 1}, {grad, dir, 0}}, lst)) return fail_e("maxabs / dot");
+      rst.clear();
+      for (int t : lst) {
+        dmax[t] = h_red[t];
         if (dmax[t] == 0.0) { searching[t] = 0; continue; }
-        slope[t] = h_red[t] / dmax[t];
+        slope[t] = h_red[(size_t)T + t] / dmax[t];
         if (slope[t] >= 0.0) {
           // not a descent direction: restart along -grad (consumes this attempt)
           h_a[t] = -1.0;
           h_b[t] = 0.0;
           attempt[t] = att + 1;
-          restart = true;
+          rst.push_back(t);
           continue;
         }
         searching[t] = 1;
         s_try[t] = step[t];
       }
-      if (restart) {
-        if (!upload_coefs()) return fail_e("coef upload");
-        e = huvk::launch_vec_dir(grad, coefA, coefB, N, T, dir, s);
-        if (e != hipSuccess) return fail_e("dir");
-      }
+      if (!dir_rows(grad, rst)) return fail_e("dir");
       // Batched Armijo backtracking for the searching targets.
       for (int bt = 0; bt < 25; ++bt) {
         bool any_s = false;
@@ -370,53 +414,43 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         if (!any_s) break;
         e = hipMemcpyAsync(coefA, h_a.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return fail_e("coef upload");
-        e = huvk::launch_vec_axpy(theta, dir, coefA, N, T, trial, s);
-        if (e != hipSuccess) return fail_e("axpy");
         which.clear();
         for (int t = 0; t < T; ++t)
           if (searching[t]) which.push_back(t);
+        {
+          const int* rd = upload_rows(3, which);
+          if (e != hipSuccess) return fail_e("row upload");
+          e = huvk::launch_vec_axpy(theta, dir, coefA, N, (int)which.size(), rd, trial, s);
+          if (e != hipSuccess) return fail_e("axpy");
+        }
         st = eval_sel(trial, nullptr, ctrial, which);
         if (st != HUV_OK) return st;
-        for (int t = 0; t < T; ++t) h_sel[t] = 0;
-        bool take = false;
-        for (int t = 0; t < T; ++t) {
-          if (!searching[t]) continue;
+        lst.clear();
+        for (int t : which) {
           if (ctrial[t] <= cost[t] + 1e-4 * s_try[t] * slope[t] && std::isfinite(ctrial[t])) {
             searching[t] = 0;
             accepted[t] = 1;
             s_used[t] = s_try[t];
-            h_sel[t] = 1;
-            take = true;
+            lst.push_back(t);
           } else {
             s_try[t] *= 0.5;
           }
         }
-        if (take) {
-          e = hipMemcpyAsync(sel, h_sel.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
-          if (e != hipSuccess) return fail_e("sel upload");
-          e = huvk::launch_vec_select(trial, sel, N, T, theta, s);
-          if (e != hipSuccess) return fail_e("select");
-        }
+        if (!copy_rows(trial, theta, lst)) return fail_e("select");
       }
       // Targets whose search failed: retry once along -grad with CG, else stop.
-      bool retry = false;
+      rst.clear();
       for (int t = 0; t < T; ++t) {
-        h_a[t] = 0.0;
-        h_b[t] = 1.0;
         if (!active[t] || accepted[t]) continue;
         if (searching[t]) searching[t] = 0;  // exhausted backtracks
         if (attempt[t] == att && att == 0 && cg && dmax[t] != 0.0) {
           attempt[t] = 1;
           h_a[t] = -1.0;
           h_b[t] = 0.0;
-          retry = true;
+          rst.push_back(t);
         }
       }
-      if (att == 0 && retry) {
-        if (!upload_coefs()) return fail_e("coef upload");
-        e = huvk::launch_vec_dir(grad, coefA, coefB, N, T, dir, s);
-        if (e != hipSuccess) return fail_e("dir");
-      }
+      if (att == 0 && !dir_rows(grad, rst)) return fail_e("dir");
     }
     bool any_acc = false;
     for (int t = 0; t < T; ++t) {
@@ -435,18 +469,11 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     st = eval_sel(theta, gnew, cnew, which);
     if (st != HUV_OK) return st;
     // CG update: beta = max(0, (g'.g' - g'.g) / g.g)
-    std::vector<double> gg(T), gng(T), gngn(T);
-    if (!reduce(grad, grad, 0)) return fail_e("dot");
-    gg = h_red;
-    if (!reduce(gnew, grad, 0)) return fail_e("dot");
-    gng = h_red;
-    if (!reduce(gnew, gnew, 0)) return fail_e("dot");
-    gngn = h_red;
-    for (int t = 0; t < T; ++t) {
-      h_sel[t] = 0;
-      h_a[t] = 0.0;
-      h_b[t] = 1.0;
-      if (!active[t] || !accepted[t]) continue;
+    if (!reduce({{grad, grad, 0}, {gnew, grad, 0}, {gnew, gnew, 0}}, which)) return fail_e("dot");
+    const double* gg = h_red.data();
+    const double* gng = gg + T;
+    const double* gngn = gng + T;
+    for (int t : which) {
       const double prev = cost[t];
       cost[t] = cnew[t];
       it[t] += 1;
@@ -455,7 +482,6 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       const double beta = cg ? (gg[t] > 0.0 ? std::max(0.0, (gngn[t] - gng[t]) / gg[t]) : 0.0) : 0.0;
       h_a[t] = -1.0;
       h_b[t] = beta;
-      h_sel[t] = 1;
       step[t] = std::min(s_used[t] * 1.5, 10.0 * learning_rate);
       const double rel = prev > 0.0 ? (prev - cost[t]) / prev : 0.0;
       small[t] = rel < tol ? small[t] + 1 : 0;
@@ -468,13 +494,8 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         term[t] = 3;
       }
     }
-    if (!upload_coefs()) return fail_e("coef upload");
-    e = huvk::launch_vec_dir(gnew, coefA, coefB, N, T, dir, s);
-    if (e != hipSuccess) return fail_e("dir");
-    e = hipMemcpyAsync(sel, h_sel.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return fail_e("sel upload");
-    e = huvk::launch_vec_select(gnew, sel, N, T, grad, s);
-    if (e != hipSuccess) return fail_e("select");
+    if (!dir_rows(gnew, which)) return fail_e("dir");
+    if (!copy_rows(gnew, grad, which)) return fail_e("gradient copy");
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return fail_e("synchronize");

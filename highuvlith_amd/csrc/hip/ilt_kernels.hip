@@ -146,30 +146,27 @@ __global__ void k_ilt_cost_reduce(const double* __restrict__ fid, const double* 
 }
 
 // ---------------------------------------------------------------------------
-// Fused adjoint, one work-group per (target, condition), coarse grid M <= 64
-// (a single x class).  Gb: |g| <= 2B band of F(G) on the n grid, layout
-// [gx + 2B][gy + 2B].  Rout: accumulated band [fx + B][fy + B] of
-// sum_k lambda_k conj(K_k) F_n(G0_b . A_k) / n^2 (complex).
+// Adjoint, coarse grid M <= 64 (a single x class), in two launches:
+//   k_ilt_g0:      one work-group per image (target, condition): G0_b on the
+//                  coarse grid, G0[img][y][x] (real), from the |g| <= 2B band
+//                  Gb of F(G), layout [gx + 2B][gy + 2B], flare adjoint applied.
+//   k_ilt_adjoint: work-group (image, split s) takes the kernels k = s, s + KS,
+//                  ...: field A_k, product with G0_b, back to the |f| <= B band,
+//                  accumulating lambda_k conj(K_k) F(G0_b . A_k) / n^2 into its
+//                  own partial band Rpart[img][s][fx + B][fy + B]; k_ilt_hband
+//                  sums the KS partials in a fixed order.
 // ---------------------------------------------------------------------------
 template <int M>
-__global__ __launch_bounds__(M) void k_ilt_adjoint(
-    const float2* __restrict__ Mq, const float2* __restrict__ Gb, int B, int ncond,
-    const int* __restrict__ cond_K, const float2* const* __restrict__ cond_kern,
-    const float* const* __restrict__ cond_lam, float lam_scale, float g0_scale, float flare,
-    float2* __restrict__ Rout) {
+__global__ __launch_bounds__(M) void k_ilt_g0(const float2* __restrict__ Gb, int B, float g0_scale,
+                                              float flare, float* __restrict__ G0g) {
   constexpr int TP = M + 1;
   extern __shared__ unsigned char smem_raw[];
-  c32* T = reinterpret_cast<c32*>(smem_raw);             // (4B+1) x TP
-  const int W = 2 * B + 1, G = 2 * B, W2 = 4 * B + 1;
-  c32* P1 = T + (size_t)W2 * TP;                         // W x TP
-  float* G0 = reinterpret_cast<float*>(P1 + (size_t)W * TP);  // M x TP
+  c32* T = reinterpret_cast<c32*>(smem_raw);  // (4B+1) x TP
+  const int G = 2 * B, W2 = 4 * B + 1;
   const int img = blockIdx.x;
-  const int tgt = img / ncond, c = img % ncond;
   const int tid = threadIdx.x;
-  const float2* mq = Mq + (size_t)tgt * W * W;
   const float2* gb = Gb + (size_t)img * W2 * W2;
-
-  // ---- G0_b on the coarse grid: x pass over the 4B+1 band rows gy ----
+  // x pass over the 4B+1 band rows gy
   for (int iy = tid; iy < W2; iy += M) {
     c32 v[M];
 #pragma unroll
@@ -195,18 +192,38 @@ __global__ __launch_bounds__(M) void k_ilt_adjoint(
     for (int j = 0; j < M; ++j) T[iy * TP + j] = v[j];
   }
   __syncthreads();
+  const int j = tid;
+  c32 d[M];
+#pragma unroll
+  for (int h = 0; h < M; ++h) {
+    d[h] = c32{0.f, 0.f};
+    if (h <= G) d[h] = T[(h + G) * TP + j];
+    if (h - M >= -G) d[h] = cadd(d[h], T[(h - M + G) * TP + j]);
+  }
+  fft_reg<+1, M>(d);
+  float* g0 = G0g + (size_t)img * M * M;
+#pragma unroll
+  for (int y = 0; y < M; ++y) g0[y * M + j] = d[y].re * g0_scale;
+}
+
+template <int M>
+__global__ __launch_bounds__(M) void k_ilt_adjoint(
+    const float2* __restrict__ Mq, const float* __restrict__ G0g, int B, int ncond,
+    const int* __restrict__ cond_K, const float2* const* __restrict__ cond_kern,
+    const float* const* __restrict__ cond_lam, float lam_scale, float2* __restrict__ Rpart) {
+  constexpr int TP = M + 1;
+  extern __shared__ unsigned char smem_raw[];
+  const int W = 2 * B + 1;
+  c32* T = reinterpret_cast<c32*>(smem_raw);                  // W x TP
+  c32* P1 = T + (size_t)W * TP;                               // W x TP
+  float* G0 = reinterpret_cast<float*>(P1 + (size_t)W * TP);  // M x TP
+  const int img = blockIdx.x, split = blockIdx.y, KS = gridDim.y;
+  const int tgt = img / ncond, c = img % ncond;
+  const int tid = threadIdx.x;
+  const float2* mq = Mq + (size_t)tgt * W * W;
   {
-    const int j = tid;
-    c32 d[M];
-#pragma unroll
-    for (int h = 0; h < M; ++h) {
-      d[h] = c32{0.f, 0.f};
-      if (h <= G) d[h] = T[(h + G) * TP + j];
-      if (h - M >= -G) d[h] = cadd(d[h], T[(h - M + G) * TP + j]);
-    }
-    fft_reg<+1, M>(d);
-#pragma unroll
-    for (int y = 0; y < M; ++y) G0[y * TP + j] = d[y].re * g0_scale;
+    const float* g0 = G0g + (size_t)img * M * M;
+    for (int y = 0; y < M; ++y) G0[y * TP + tid] = g0[y * M + tid];
   }
   __syncthreads();
 
@@ -217,7 +234,7 @@ __global__ __launch_bounds__(M) void k_ilt_adjoint(
   c32 R[2 * 15 + 1];  // accumulator row fy = tid - B (B <= 15)
 #pragma unroll
   for (int i = 0; i < 31; ++i) R[i] = c32{0.f, 0.f};
-  for (int k = 0; k < K; ++k) {
+  for (int k = split; k < K; k += KS) {
     // x pass (rows iy in [0, 2B]): E row = IFFT_M(K . M)
     for (int iy = tid; iy < W; iy += M) {
       const float2* kk = kern + (size_t)k * W * W + iy;
@@ -282,7 +299,7 @@ __global__ __launch_bounds__(M) void k_ilt_adjoint(
     __syncthreads();
   }
   if (tid < W) {
-    float2* ro = Rout + (size_t)img * W * W + tid;
+    float2* ro = Rpart + ((size_t)img * KS + split) * W * W + tid;
 #pragma unroll
     for (int i = 0; i < 31; ++i) {
       const int fx = i - 15;
@@ -293,7 +310,8 @@ __global__ __launch_bounds__(M) void k_ilt_adjoint(
 
 // H[fy][fx] (fx in [0, B], fy in [-B, B]) = sum_c w_c (1-a) (R_c[f] + conj R_c[-f]) / n^2,
 // layout [fy + B][fx] (the Hermitian half band read by band_image, half-width B).
-__global__ void k_ilt_hband(const float2* __restrict__ Rin, int B, int ncond,
+// R_c = sum over the KS partial bands of image (tgt, c), in split order.
+__global__ void k_ilt_hband(const float2* __restrict__ Rin, int B, int ncond, int KS,
                             const float* __restrict__ weight, float scale,
                             float2* __restrict__ H) {
   const int tgt = blockIdx.y;
@@ -303,10 +321,15 @@ __global__ void k_ilt_hband(const float2* __restrict__ Rin, int B, int ncond,
   const int fx = idx % (B + 1), fy = idx / (B + 1) - B;
   c32 acc{0.f, 0.f};
   for (int c = 0; c < ncond; ++c) {
-    const float2* r = Rin + ((size_t)tgt * ncond + c) * W * W;
-    const float2 p = r[(size_t)(fx + B) * W + (fy + B)];
-    const float2 q = r[(size_t)(B - fx) * W + (B - fy)];
-    acc = cadd(acc, cscale(c32{p.x + q.x, p.y - q.y}, weight[c]));
+    c32 p{0.f, 0.f}, q{0.f, 0.f};
+    for (int sp = 0; sp < KS; ++sp) {
+      const float2* r = Rin + (((size_t)tgt * ncond + c) * KS + sp) * W * W;
+      const float2 a = r[(size_t)(fx + B) * W + (fy + B)];
+      const float2 b = r[(size_t)(B - fx) * W + (B - fy)];
+      p = cadd(p, c32{a.x, a.y});
+      q = cadd(q, c32{b.x, b.y});
+    }
+    acc = cadd(acc, cscale(c32{p.re + q.re, p.im - q.im}, weight[c]));
   }
   H[(size_t)tgt * (B + 1) * W + idx] = make_float2(acc.re * scale, acc.im * scale);
 }
@@ -328,12 +351,19 @@ __global__ void k_ilt_chain(const float* __restrict__ dcdm, const float* __restr
 // Per-target FP64 partial sums, reduced in a fixed order.
 // ---------------------------------------------------------------------------
 // out[t * nblk + b] = partial of sum_i x_i y_i  (mode 0) or max |x_i| (mode 1)
+// Per-target vector operations over a list of target rows (rows = null: all
+// rows 0..count-1).  Grid: x over pixel blocks, y over the listed rows.
+__device__ __forceinline__ int row_of(const int* __restrict__ rows, int j) {
+  return rows ? rows[j] : j;
+}
+
 __global__ __launch_bounds__(PW_THREADS) void k_vec_reduce(const float* __restrict__ x,
                                                            const float* __restrict__ y,
                                                            int64_t npix, int nblk, int mode,
+                                                           const int* __restrict__ rows,
                                                            double* __restrict__ out) {
   __shared__ double sh[PW_THREADS];
-  const int t = blockIdx.y;
+  const int t = row_of(rows, blockIdx.y);
   const float* xx = x + (size_t)t * npix;
   const float* yy = y ? y + (size_t)t * npix : nullptr;
   const int64_t chunk = (npix + nblk - 1) / nblk;
@@ -354,47 +384,47 @@ __global__ __launch_bounds__(PW_THREADS) void k_vec_reduce(const float* __restri
   if (threadIdx.x == 0) out[(size_t)t * nblk + blockIdx.x] = sh[0];
 }
 
-__global__ void k_vec_finish(const double* __restrict__ part, int ntargets, int nblk, int mode,
-                             double* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntargets) return;
+__global__ void k_vec_finish(const double* __restrict__ part, int count, int nblk, int mode,
+                             const int* __restrict__ rows, double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int t = row_of(rows, j);
   double a = 0.0;
   for (int b = 0; b < nblk; ++b) a = mode == 0 ? a + part[(size_t)t * nblk + b] : fmax(a, part[(size_t)t * nblk + b]);
   out[t] = a;
 }
 
-// y = x + coef[t] * d   (per-target coefficient; coef 0 copies x)
+// y = x + coef[t] * d   (per-target coefficient)
 __global__ void k_vec_axpy(const float* __restrict__ x, const float* __restrict__ d,
-                           const double* __restrict__ coef, int64_t npix, int ntargets,
-                           float* __restrict__ y) {
-  const int64_t total = npix * ntargets;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int t = (int)(i / npix);
-    y[i] = (float)((double)x[i] + coef[t] * (double)d[i]);
-  }
+                           const double* __restrict__ coef, int64_t npix,
+                           const int* __restrict__ rows, float* __restrict__ y) {
+  const int t = row_of(rows, blockIdx.y);
+  const size_t base = (size_t)t * npix;
+  const double c = coef[t];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[base + i] = (float)((double)x[base + i] + c * (double)d[base + i]);
 }
 
 // d = a[t] * g + b[t] * d
 __global__ void k_vec_dir(const float* __restrict__ g, const double* __restrict__ a,
-                          const double* __restrict__ b, int64_t npix, int ntargets,
+                          const double* __restrict__ b, int64_t npix, const int* __restrict__ rows,
                           float* __restrict__ d) {
-  const int64_t total = npix * ntargets;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int t = (int)(i / npix);
-    d[i] = (float)(a[t] * (double)g[i] + b[t] * (double)d[i]);
-  }
+  const int t = row_of(rows, blockIdx.y);
+  const size_t base = (size_t)t * npix;
+  const double ca = a[t], cb = b[t];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[base + i] = (float)(ca * (double)g[base + i] + cb * (double)d[base + i]);
 }
 
-// dst[t] = src[t] where sel[t] != 0
-__global__ void k_vec_select(const float* __restrict__ src, const int* __restrict__ sel,
-                             int64_t npix, int ntargets, float* __restrict__ dst) {
-  const int64_t total = npix * ntargets;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (sel[i / npix]) dst[i] = src[i];
-  }
+// dst[t] = src[t] for the listed rows
+__global__ void k_vec_copy_rows(const float* __restrict__ src, int64_t npix,
+                                const int* __restrict__ rows, float* __restrict__ dst) {
+  const size_t base = (size_t)row_of(rows, blockIdx.y) * npix;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[base + i] = src[base + i];
 }
 
 // Row gather / scatter of per-target images: dst[j] = src[idx[j]] or
@@ -452,36 +482,46 @@ hipError_t launch_ilt_cost_reduce(const double* fid, const double* reg, int ntar
 }
 
 template <int M>
-static hipError_t adjoint_m(int nimg, const float2* Mq, const float2* Gb, int B, int ncond,
+static hipError_t adjoint_m(int nimg, int KS, const float2* Mq, const float2* Gb, int B, int ncond,
                             const int* cond_K, const float2* const* cond_kern,
                             const float* const* cond_lam, float lam_scale, float g0_scale,
-                            float flare, float2* Rout, hipStream_t s) {
+                            float flare, float* G0g, float2* Rpart, hipStream_t s) {
   const int W = 2 * B + 1, W2 = 4 * B + 1;
-  const size_t shmem = (size_t)(W2 + W) * (M + 1) * sizeof(c32) + (size_t)M * (M + 1) * sizeof(float);
-  auto kern = k_ilt_adjoint<M>;
-  hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  hipLaunchKernelGGL(kern, dim3(nimg), dim3(M), shmem, s, Mq, Gb, B, ncond, cond_K, cond_kern,
-                     cond_lam, lam_scale, g0_scale, flare, Rout);
+  const size_t sh0 = (size_t)W2 * (M + 1) * sizeof(c32);
+  const size_t sh1 = (size_t)2 * W * (M + 1) * sizeof(c32) + (size_t)M * (M + 1) * sizeof(float);
+  auto k0 = k_ilt_g0<M>;
+  auto k1 = k_ilt_adjoint<M>;
+  // sized for the largest band (B = 15), once per instantiation
+  static const hipError_t a0 = hipFuncSetAttribute(
+      (const void*)k0, hipFuncAttributeMaxDynamicSharedMemorySize, (int)((61 * (M + 1)) * sizeof(c32)));
+  static const hipError_t a1 = hipFuncSetAttribute(
+      (const void*)k1, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)(2 * 31 * (M + 1) * sizeof(c32) + M * (M + 1) * sizeof(float)));
+  if (a0 != hipSuccess) return a0;
+  if (a1 != hipSuccess) return a1;
+  hipLaunchKernelGGL(k0, dim3(nimg), dim3(M), sh0, s, Gb, B, g0_scale, flare, G0g);
+  hipLaunchKernelGGL(k1, dim3(nimg, KS), dim3(M), sh1, s, Mq, G0g, B, ncond, cond_K, cond_kern,
+                     cond_lam, lam_scale, Rpart);
   return hipGetLastError();
 }
 
-hipError_t launch_ilt_adjoint(int m, int nimg, const float2* Mq, const float2* Gb, int B,
+hipError_t launch_ilt_adjoint(int m, int nimg, int KS, const float2* Mq, const float2* Gb, int B,
                               int ncond, const int* cond_K, const float2* const* cond_kern,
                               const float* const* cond_lam, float lam_scale, float g0_scale,
-                              float flare, float2* Rout, hipStream_t s) {
-  if (B > 15) return hipErrorInvalidValue;
+                              float flare, float* G0g, float2* Rpart, hipStream_t s) {
+  if (B > 15 || KS < 1) return hipErrorInvalidValue;
   switch (m) {
-    case 32: return adjoint_m<32>(nimg, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, Rout, s);
-    case 64: return adjoint_m<64>(nimg, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, Rout, s);
+    case 32: return adjoint_m<32>(nimg, KS, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, G0g, Rpart, s);
+    case 64: return adjoint_m<64>(nimg, KS, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, G0g, Rpart, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, const float* weight,
-                            float scale, float2* H, hipStream_t s) {
+hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, int KS,
+                            const float* weight, float scale, float2* H, hipStream_t s) {
   const int cnt = (B + 1) * (2 * B + 1);
   hipLaunchKernelGGL(k_ilt_hband, dim3((cnt + 127) / 128, ntargets), dim3(128), 0, s, Rin, B,
-                     ncond, weight, scale, H);
+                     ncond, KS, weight, scale, H);
   return hipGetLastError();
 }
 
@@ -492,26 +532,34 @@ hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m,
   return hipGetLastError();
 }
 
-hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int ntargets, int nblk,
-                             int mode, double* part, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_vec_reduce, dim3(nblk, ntargets), dim3(PW_THREADS), 0, s, x, y, npix, nblk,
-                     mode, part);
-  hipLaunchKernelGGL(k_vec_finish, dim3((ntargets + 63) / 64), dim3(64), 0, s, part, ntargets,
-                     nblk, mode, out);
+static dim3 rows_grid(int64_t npix, int count) {
+  const int64_t bx = std::min<int64_t>((npix + 255) / 256, 1024);
+  return dim3((unsigned)bx, (unsigned)count);
+}
+
+hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int count,
+                             const int* rows, int nblk, int mode, double* part, double* out,
+                             hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vec_reduce, dim3(nblk, count), dim3(PW_THREADS), 0, s, x, y, npix, nblk,
+                     mode, rows, part);
+  hipLaunchKernelGGL(k_vec_finish, dim3((count + 63) / 64), dim3(64), 0, s, part, count, nblk,
+                     mode, rows, out);
   return hipGetLastError();
 }
 
 hipError_t launch_vec_axpy(const float* x, const float* d, const double* coef, int64_t npix,
-                           int ntargets, float* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_vec_axpy, dim3(grid_for(npix * ntargets, 256)), dim3(256), 0, s, x, d, coef,
-                     npix, ntargets, y);
+                           int count, const int* rows, float* y, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vec_axpy, rows_grid(npix, count), dim3(256), 0, s, x, d, coef, npix, rows,
+                     y);
   return hipGetLastError();
 }
 
 hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int64_t npix,
-                          int ntargets, float* d, hipStream_t s) {
-  hipLaunchKernelGGL(k_vec_dir, dim3(grid_for(npix * ntargets, 256)), dim3(256), 0, s, g, a, b,
-                     npix, ntargets, d);
+                          int count, const int* rows, float* d, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vec_dir, rows_grid(npix, count), dim3(256), 0, s, g, a, b, npix, rows, d);
   return hipGetLastError();
 }
 
@@ -529,10 +577,11 @@ hipError_t launch_rows_scatter(const float* src, const int* idx, int64_t npix, i
   return hipGetLastError();
 }
 
-hipError_t launch_vec_select(const float* src, const int* sel, int64_t npix, int ntargets,
-                             float* dst, hipStream_t s) {
-  hipLaunchKernelGGL(k_vec_select, dim3(grid_for(npix * ntargets, 256)), dim3(256), 0, s, src, sel,
-                     npix, ntargets, dst);
+hipError_t launch_vec_copy_rows(const float* src, int64_t npix, int count, const int* rows,
+                                float* dst, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vec_copy_rows, rows_grid(npix, count), dim3(256), 0, s, src, npix, rows,
+                     dst);
   return hipGetLastError();
 }
 

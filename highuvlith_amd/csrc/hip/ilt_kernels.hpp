@@ -18,27 +18,37 @@ hipError_t launch_ilt_reg(const float* m, int n, int ntargets, int bpi, float tv
 hipError_t launch_ilt_cost_reduce(const double* fid, const double* reg, int ntargets, int ncond,
                                   int bpi, int reg_bpi, const float* weight, double inv_n,
                                   double* cost, hipStream_t s);
-hipError_t launch_ilt_adjoint(int m, int nimg, const float2* Mq, const float2* Gb, int B,
+// Adjoint band of nimg = ntargets x ncond images on coarse grid m (32 or 64):
+// G0g scratch [nimg][m][m] floats, Rpart [nimg][KS][2B+1][2B+1] partial bands
+// (the kernels of each condition dealt over KS work-groups per image).
+hipError_t launch_ilt_adjoint(int m, int nimg, int KS, const float2* Mq, const float2* Gb, int B,
                               int ncond, const int* cond_K, const float2* const* cond_kern,
                               const float* const* cond_lam, float lam_scale, float g0_scale,
-                              float flare, float2* Rout, hipStream_t s);
-hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, const float* weight,
-                            float scale, float2* H, hipStream_t s);
+                              float flare, float* G0g, float2* Rpart, hipStream_t s);
+hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, int KS,
+                            const float* weight, float scale, float2* H, hipStream_t s);
 hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m, float beta,
                             int64_t total, float* grad, hipStream_t s);
-// mode 0: per-target sum x.y ; mode 1: per-target max |x|.  part: ntargets x nblk.
-hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int ntargets, int nblk,
-                             int mode, double* part, double* out, hipStream_t s);
+// Per-target vector operations on `count` target rows listed in `rows`
+// (device; null = rows 0..count-1); other rows are left untouched.
+// mode 0: per-target sum x.y ; mode 1: per-target max |x|.  part: ntargets x nblk,
+// out[t] for the listed t.
+hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int count,
+                             const int* rows, int nblk, int mode, double* part, double* out,
+                             hipStream_t s);
+// y = x + coef[t] d
 hipError_t launch_vec_axpy(const float* x, const float* d, const double* coef, int64_t npix,
-                           int ntargets, float* y, hipStream_t s);
+                           int count, const int* rows, float* y, hipStream_t s);
+// d = a[t] g + b[t] d
 hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int64_t npix,
-                          int ntargets, float* d, hipStream_t s);
+                          int count, const int* rows, float* d, hipStream_t s);
 // dst[j] = src[idx[j]] (gather) / dst[idx[j]] = src[j] (scatter), rows of npix.
 hipError_t launch_rows_gather(const float* src, const int* idx, int64_t npix, int nrows,
                               float* dst, hipStream_t s);
 hipError_t launch_rows_scatter(const float* src, const int* idx, int64_t npix, int nrows,
                                float* dst, hipStream_t s);
-hipError_t launch_vec_select(const float* src, const int* sel, int64_t npix, int ntargets,
-                             float* dst, hipStream_t s);
+// dst[t] = src[t] for the listed rows
+hipError_t launch_vec_copy_rows(const float* src, int64_t npix, int count, const int* rows,
+                                float* dst, hipStream_t s);
 
 }  // namespace huvk

@@ -212,10 +212,15 @@ __device__ __forceinline__ c32x2 load_c32x2(__amdgpu_buffer_rsrc_t r, int voff, 
 
 // XNB: band columns per load block of the x pass (a tuning parameter).
 // kern: [Q][nplanes * Kmax][W][W], class q's copy twisted by e^{2 pi i kx q / M}.
+// S > 1 (small batches): work-group split j takes the kernels [j ks, j ks + ks)
+// of every plane (ks = ceil(Kmax / S), every split non-empty) and writes its
+// partial coarse images at coarse + j * nclips * nplanes * M^2; coarse_rows
+// sums the S partials in split order.
 template <int M, int XNB, int XS>
 __global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_waves_per_eu(4))) void k_socs_fields(
     const float2* __restrict__ Mq, const float2* __restrict__ kern, const float* __restrict__ lam,
-    int nclips, int nplanes, int Kmax, const float2* __restrict__ tw_m, float* __restrict__ coarse) {
+    int nclips, int nplanes, int Kmax, int S, const float2* __restrict__ tw_m,
+    float* __restrict__ coarse) {
   using C = CoarseCfg<M, XS>;
   constexpr int F = C::F, Q = C::Q, R = C::R, BB = C::BB, W = C::W, TP = C::TP;
   constexpr int TMIN = C::TMIN, TMAX = C::TMAX, NT = TMAX - TMIN + 1;
@@ -226,7 +231,8 @@ __global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_
   const int L = blockIdx.x;
   const int slot = L / kXcds;
   const int q = slot % Q;
-  const int clip = (slot / Q) * kXcds + L % kXcds;
+  const int sj = (slot / Q) % S;
+  const int clip = (slot / (Q * S)) * kXcds + L % kXcds;
   if (clip >= nclips) return;  // whole work-group, before any barrier
   const int tid = threadIdx.x;
   for (int i = tid; i < R * F; i += C::THREADS) twy[i] = tw_pos(tw_m, (i % F) * (i / F), M);
@@ -235,6 +241,11 @@ __global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_
   constexpr int BOX = XS == 1 ? W * W : (W + 1) * W;
   const __amdgpu_buffer_rsrc_t mr = box_rsrc(Mq + (size_t)clip * BOX, BOX);
   const int steps = nplanes * Kmax;
+  // this split's kernels k0 .. k0 + kc - 1 of each plane; local step l is
+  // plane l / kc, kernel k0 + l % kc
+  const int ks = (Kmax + S - 1) / S, k0 = sj * ks, kc = min(ks, Kmax - k0);
+  const int lsteps = nplanes * kc;
+  auto gstep = [&](int l) { return (l / kc) * Kmax + k0 + l % kc; };
   const bool is_y = tid < C::YT;
   const int j = tid % F, r = tid / F;
   const int ix = tid - C::XOFF;  // x lane
@@ -248,14 +259,15 @@ __global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_
   // The two groups run separate loops (wave-uniform split) with the same
   // number of barriers, so their register live ranges do not overlap.
   if (ix >= 0) {
-    for (int s = 0; s <= steps; ++s) {
-      if (s < steps && iy < W) {
+    for (int s = 0; s <= lsteps; ++s) {
+      if (s < lsteps && iy < W) {
+        const int sg = gstep(s);
         c32* Tr = T + (s & 1) * W * TP + iy * TP;
         if constexpr (XS == 1) {
           // Buffer loads: one VGPR offset per lane, the column offset in an SGPR.
           const int vo = iy * (int)sizeof(float2);
           const __amdgpu_buffer_rsrc_t kr =
-              box_rsrc(kern + ((size_t)q * steps + s) * W * W, W * W);
+              box_rsrc(kern + ((size_t)q * steps + sg) * W * W, W * W);
           c32 c[F];
 #pragma unroll
           for (int h = 0; h < F; ++h) c[h] = c32{0.f, 0.f};
@@ -303,7 +315,7 @@ __global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_
           const int col0 = part ? C::xcol0(1) : C::xcol0(0);
           const int vo = (((col0 + 1) >> 1) * W + iy) * (int)(2 * sizeof(float2));
           const __amdgpu_buffer_rsrc_t kr =
-              box_rsrc(kern + ((size_t)q * steps + s) * BOX, BOX);
+              box_rsrc(kern + ((size_t)q * steps + sg) * BOX, BOX);
           c32 c[F / 2];  // bins 16 part + h
 #pragma unroll
           for (int h = 0; h < F / 2; ++h) c[h] = c32{0.f, 0.f};
@@ -365,7 +377,7 @@ __global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_
     float acc[F];
 #pragma unroll
     for (int i = 0; i < F; ++i) acc[i] = 0.f;
-    for (int s = 0; s <= steps; ++s) {
+    for (int s = 0; s <= lsteps; ++s) {
       if (is_y && s >= 1) {
         const int sp = s - 1;
         // Re-read the twiddles every step rather than holding them in registers.
@@ -388,13 +400,13 @@ __global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_
           d[h] = cmul(v, ty[h]);
         }
         fft_reg<+1, F>(d);
-        const float l = lam[sp];
+        const float l = lam[gstep(sp)];
 #pragma unroll
         for (int i = 0; i < F; ++i) acc[i] += l * cnorm(d[i]);
-        if (sp % Kmax == Kmax - 1) {
+        if (sp % kc == kc - 1) {
           // Vector buffer stores: one VGPR offset, the row offset in an SGPR.
           const __amdgpu_buffer_rsrc_t orc = __builtin_amdgcn_make_buffer_rsrc(
-              coarse + ((size_t)clip * nplanes + sp / Kmax) * M * M, (short)0,
+              coarse + (((size_t)sj * nclips + clip) * nplanes + sp / kc) * M * M, (short)0,
               M * M * (int)sizeof(float), 0x00020000);
           const int vo = (r * M + Q * j + q) * (int)sizeof(float);
 #pragma unroll
@@ -421,6 +433,23 @@ struct CoarseFft {
 // coarse_rows: coarse row pairs -> FFT_M forward, keep gx in [0, G] and
 // [M - G, M) (G = 2B).  tmp1 layout [img][kk][pair] with kk in [0, 2G].
 // ---------------------------------------------------------------------------
+// coarse[i] = sum_j coarse[j * count + i], j = 0 .. S-1 in order (float4 lanes).
+__global__ void k_sum_splits(float* __restrict__ coarse, int S, int64_t count4) {
+  float4* c4 = reinterpret_cast<float4*>(coarse);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count4;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float4 v = c4[i];
+    for (int j = 1; j < S; ++j) {
+      const float4 w = c4[j * count4 + i];
+      v.x += w.x;
+      v.y += w.y;
+      v.z += w.z;
+      v.w += w.w;
+    }
+    c4[i] = v;
+  }
+}
+
 template <int M, int TEAMS>
 __global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_rows(
     const float* __restrict__ coarse, int B, const float2* __restrict__ tw_m,
@@ -730,8 +759,9 @@ hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const floa
 }
 
 template <int M, int XNB, int XS = 1>
-static hipError_t fields_m(int nclips, int nplanes, int Kmax, const float2* kern, const float* lam,
-                           const float2* Mq, const float2* tw_m, float* coarse, hipStream_t s) {
+static hipError_t fields_m(int nclips, int nplanes, int Kmax, int S, const float2* kern,
+                           const float* lam, const float2* Mq, const float2* tw_m, float* coarse,
+                           hipStream_t s) {
   using C = CoarseCfg<M, XS>;
   // the two-lane x pass needs every column in [-1, W)
   if constexpr (XS == 2 && !C::SPLIT_OK) return hipErrorInvalidValue;
@@ -742,23 +772,23 @@ static hipError_t fields_m(int nclips, int nplanes, int Kmax, const float2* kern
       (const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
   if (attr != hipSuccess) return attr;
   const int groups = (nclips + kXcds - 1) / kXcds;
-  HUV_LAUNCH(kernel, dim3(groups * kXcds * C::Q), dim3(C::THREADS), shmem, s, Mq, kern, lam,
-             nclips, nplanes, Kmax, tw_m, coarse);
+  HUV_LAUNCH(kernel, dim3(groups * kXcds * C::Q * S), dim3(C::THREADS), shmem, s, Mq, kern, lam,
+             nclips, nplanes, Kmax, S, tw_m, coarse);
 }
 
 template <int M>
-static hipError_t fields_x(int xnb, int xlanes, int nclips, int nplanes, int Kmax,
+static hipError_t fields_x(int xnb, int xlanes, int nclips, int nplanes, int Kmax, int S,
                            const float2* kern, const float* lam, const float2* Mq,
                            const float2* tw_m, float* coarse, hipStream_t s) {
   if (!xlanes) xlanes = fields_lanes_default(M);
   if (!xnb) xnb = xlanes == 2 ? kFieldsXnbDefault2 : kFieldsXnbDefault;
   switch (xlanes * 100 + xnb) {
-    case 104: return fields_m<M, 4>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 106: return fields_m<M, 6>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 108: return fields_m<M, 8>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 204: return fields_m<M, 4, 2>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 206: return fields_m<M, 6, 2>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 208: return fields_m<M, 8, 2>(nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 104: return fields_m<M, 4>(nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 106: return fields_m<M, 6>(nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 108: return fields_m<M, 8>(nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 204: return fields_m<M, 4, 2>(nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 206: return fields_m<M, 6, 2>(nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 208: return fields_m<M, 8, 2>(nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -767,35 +797,58 @@ int fields_lanes_default(int m) { return m >= 64 ? 2 : 1; }
 
 bool fields_lanes_supported(int m, int xlanes) { return xlanes == 1 || (xlanes == 2 && m >= 64); }
 
-hipError_t launch_socs_fields(int m, int xnb, int xlanes, int nclips, int nplanes, int Kmax,
+int fields_splits(int m, int nclips, int Kmax) {
+  const int Q = coarse_classes(m);
+  if (Q <= 0 || Kmax <= 1) return 1;
+  // small batches: deal the kernels over more work-groups (about 512 busy ones)
+  const int wgs = nclips * Q;
+  int S = (512 + wgs - 1) / wgs;
+  if (S <= 1) return 1;
+  if (S > Kmax) S = Kmax;
+  const int ks = (Kmax + S - 1) / S;
+  return (Kmax + ks - 1) / ks;  // every split non-empty
+}
+
+hipError_t launch_socs_fields(int m, int xnb, int xlanes, int nclips, int nplanes, int Kmax, int S,
                               const float2* kern, const float* lam, const float2* Mq,
                               const float2* tw_m, float* coarse, hipStream_t s) {
+  if (S < 1 || S > Kmax) return hipErrorInvalidValue;
   switch (m) {
-    case 32: return fields_x<32>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 64: return fields_x<64>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 128: return fields_x<128>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 192: return fields_x<192>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
-    case 256: return fields_x<256>(xnb, xlanes, nclips, nplanes, Kmax, kern, lam, Mq, tw_m, coarse, s);
+    case 32: return fields_x<32>(xnb, xlanes, nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 64: return fields_x<64>(xnb, xlanes, nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 128: return fields_x<128>(xnb, xlanes, nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 192: return fields_x<192>(xnb, xlanes, nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
+    case 256: return fields_x<256>(xnb, xlanes, nclips, nplanes, Kmax, S, kern, lam, Mq, tw_m, coarse, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <int M>
-static hipError_t coarse_rows_m(int B, int nimg, const float* coarse, const float2* tw_m,
+static hipError_t coarse_rows_m(int B, int nimg, int S, const float* coarse, const float2* tw_m,
                                 float2* tmp1, hipStream_t s) {
   constexpr int TEAMS = 256 / CoarseFft<M>::N2;
   dim3 grid((M / 2 + TEAMS - 1) / TEAMS, nimg);
+  if (S > 1) {
+    // fold the kernel-split partials into the first set (M^2 is a multiple of 4)
+    const int64_t count4 = (int64_t)nimg * M * M / 4;
+    const int blocks = (int)std::min<int64_t>((count4 + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_sum_splits, dim3(blocks), dim3(256), 0, s, const_cast<float*>(coarse), S,
+                       count4);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   HUV_LAUNCH((k_coarse_rows<M, TEAMS>), grid, dim3(256), 0, s, coarse, B, tw_m, tmp1);
 }
 
-hipError_t launch_coarse_rows(int m, int B, int nimg, const float* coarse, const float2* tw_m,
-                              float2* tmp1, hipStream_t s) {
+hipError_t launch_coarse_rows(int m, int B, int nimg, int S, const float* coarse,
+                              const float2* tw_m, float2* tmp1, hipStream_t s) {
+  if (S < 1) return hipErrorInvalidValue;
   switch (m) {
-    case 32: return coarse_rows_m<32>(B, nimg, coarse, tw_m, tmp1, s);
-    case 64: return coarse_rows_m<64>(B, nimg, coarse, tw_m, tmp1, s);
-    case 128: return coarse_rows_m<128>(B, nimg, coarse, tw_m, tmp1, s);
-    case 192: return coarse_rows_m<192>(B, nimg, coarse, tw_m, tmp1, s);
-    case 256: return coarse_rows_m<256>(B, nimg, coarse, tw_m, tmp1, s);
+    case 32: return coarse_rows_m<32>(B, nimg, S, coarse, tw_m, tmp1, s);
+    case 64: return coarse_rows_m<64>(B, nimg, S, coarse, tw_m, tmp1, s);
+    case 128: return coarse_rows_m<128>(B, nimg, S, coarse, tw_m, tmp1, s);
+    case 192: return coarse_rows_m<192>(B, nimg, S, coarse, tw_m, tmp1, s);
+    case 256: return coarse_rows_m<256>(B, nimg, S, coarse, tw_m, tmp1, s);
     default: return hipErrorInvalidValue;
   }
 }

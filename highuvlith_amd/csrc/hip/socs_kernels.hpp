@@ -42,11 +42,14 @@ hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const floa
 // multiplied by e^{2 pi i kx q / m}), lam: [nplanes][Kmax]; Mq: one box per
 // clip; coarse: [nclips][nplanes][m][m].  Boxes are W x W ([kx + BB][ky + BB])
 // for xlanes = 1 and column-paired (W + 1) x W (launch_cols_band's Mp) for 2.
-hipError_t launch_socs_fields(int m, int xnb, int xlanes, int nclips, int nplanes, int Kmax,
+// S kernel splits (fields_splits): coarse then holds S partial image sets,
+// [S][nclips][nplanes][m][m], summed by launch_coarse_rows.
+int fields_splits(int m, int nclips, int Kmax);
+hipError_t launch_socs_fields(int m, int xnb, int xlanes, int nclips, int nplanes, int Kmax, int S,
                               const float2* kern, const float* lam, const float2* Mq,
                               const float2* tw_m, float* coarse, hipStream_t s);
-hipError_t launch_coarse_rows(int m, int B, int nimg, const float* coarse, const float2* tw_m,
-                              float2* tmp1, hipStream_t s);
+hipError_t launch_coarse_rows(int m, int B, int nimg, int S, const float* coarse,
+                              const float2* tw_m, float2* tmp1, hipStream_t s);
 hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2* tmp1,
                               const float2* tw_m, float2* Ib, hipStream_t s);
 // Ib: [nimg][2G+1][G+1] Hermitian half band (gy + G, gx).
