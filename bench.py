@@ -297,7 +297,7 @@ def bench_warm(args, H, W, np, torch):
         print(msg, file=sys.stderr, flush=True)
 
     labels, lab_s = WS.ilt_labels(eng, train, cfg, iterations=200, log=log)
-    net = UN.UNet(eng.context(), levels=4, base=32, seed=7)
+    net = UN.UNet(eng.context(), levels=4, base=16, seed=7)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = net.train(train, labels, epochs=args.warm_epochs, batch=8, lr=1e-3, log=log)
@@ -308,7 +308,7 @@ def bench_warm(args, H, W, np, torch):
     res = UN.compare_warm_start(eng, net, held.cpu().numpy(), cfg, budget=200)
     T = args.warm_eval
     steps = args.warm_epochs * (args.warm_pairs // 8)
-    return {"workload": f"U-Net 4 levels 32/64/128/256 ch, {args.warm_pairs} training pairs "
+    return {"workload": f"U-Net 4 levels 16/32/64/128 ch, {args.warm_pairs} training pairs "
                         f"(512x512 @ 2 nm, 200-iteration ILT labels), {args.warm_epochs} epochs "
                         f"Adam lr 1e-3 batch 8; {T} held-out targets",
             "plain_clips_per_s": round(T / res.plain_wall_s, 3),

@@ -39,10 +39,11 @@ def ctx():
     return H.DeviceContext(0)
 
 
-def test_forward_matches_oracle(ctx):
+@pytest.mark.parametrize("base", [32, 16])
+def test_forward_matches_oracle(ctx, base):
     import torch
-    net = UN.UNet(ctx, levels=4, base=32, seed=1)
-    lay = U.Layout(4, 32)
+    net = UN.UNet(ctx, levels=4, base=base, seed=1)
+    lay = U.Layout(4, base)
     assert net.num_params == lay.n
     p = net.get_params().astype(np.float64)
     x = _binary(128, 2, 3)
@@ -53,10 +54,11 @@ def test_forward_matches_oracle(ctx):
     assert err <= 1e-2
 
 
-def test_gradient_matches_oracle(ctx):
+@pytest.mark.parametrize("base", [32, 16])
+def test_gradient_matches_oracle(ctx, base):
     import torch
-    net = UN.UNet(ctx, levels=4, base=32, seed=2)
-    lay = U.Layout(4, 32)
+    net = UN.UNet(ctx, levels=4, base=base, seed=2)
+    lay = U.Layout(4, base)
     p = net.get_params().astype(np.float64)
     x = _binary(128, 2, 5)
     t = (x * 1.5 - 0.7).astype(np.float32)
