@@ -154,7 +154,30 @@ Ctx::~Ctx() {
     hipFree(kv.second.lam);
   }
   for (auto& kv : graphs) drop_graph(kv.second);
+  if (pin_h) hipHostFree(pin_h);
+  if (pin_d) hipFree(pin_d);
   if (own_stream) hipStreamDestroy(own_stream);
+}
+
+bool Ctx::staging(size_t bytes, hipError_t& e) {
+  e = hipSuccess;
+  if (bytes <= pin_cap) return true;
+  e = hipStreamSynchronize(stream());
+  if (e != hipSuccess) return false;
+  if (pin_h) hipHostFree(pin_h);
+  if (pin_d) hipFree(pin_d);
+  pin_h = pin_d = nullptr;
+  pin_cap = 0;
+  e = hipHostMalloc((void**)&pin_h, bytes, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&pin_d, bytes);
+  if (e != hipSuccess) {
+    if (pin_h) hipHostFree(pin_h);
+    pin_h = nullptr;
+    pin_d = nullptr;
+    return false;
+  }
+  pin_cap = bytes;
+  return true;
 }
 
 void Ctx::drop_graph(GraphEntry& g) {

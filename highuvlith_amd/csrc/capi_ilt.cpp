@@ -248,35 +248,52 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   float* trial = (float*)(vec + 3 * vb);
   const int nblk = 64;
   constexpr int kRed = 3;  // reductions fetched together
-  constexpr int kLists = 4;
   char* sc = (char*)c->ws.get(
-      Workspace::ILT7,
-      ((size_t)kRed * T * nblk + (size_t)(kRed + 2) * T) * sizeof(double) + (size_t)kLists * T * sizeof(int), e);
+      Workspace::ILT7, ((size_t)kRed * T * nblk + (size_t)kRed * T) * sizeof(double), e);
   if (!sc) return hip_fail(e, "ILT optimizer workspace");
   double* part = (double*)sc;
   double* red = part + (size_t)kRed * T * nblk;
-  double* coefA = red + (size_t)kRed * T;
-  double* coefB = coefA + T;
-  int* rows_d = (int*)(coefB + T);
 
   auto fail_e = [&](const char* what) { return hip_fail(e, what); };
   std::vector<double> h_red((size_t)kRed * T), h_a(T), h_b(T);
-  // Vector operations touch only the listed target rows: list slot k is
-  // uploaded before the operations that read it (stream order).
-  auto upload_rows = [&](int k, const std::vector<int>& rows) -> const int* {
-    if ((int)rows.size() == T) return nullptr;  // every row, in order
-    if (rows.empty()) return rows_d;
-    e = hipMemcpyAsync(rows_d + (size_t)k * T, rows.data(), rows.size() * sizeof(int),
-                       hipMemcpyHostToDevice, s);
-    return e == hipSuccess ? rows_d + (size_t)k * T : nullptr;
+  // Small uploads (coefficients, row lists) are appended to the pinned
+  // staging area and copied with one transfer before the launches that read
+  // them; the area is reused after every stream synchronisation.
+  const size_t stage_cap = 96 * (size_t)T + 4096;
+  if (!c->staging(stage_cap, e)) return fail_e("staging buffer");
+  size_t so = 0, sf = 0;
+  auto flush = [&]() -> bool {
+    if (so > sf) {
+      e = hipMemcpyAsync(c->pin_d + sf, c->pin_h + sf, so - sf, hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) return false;
+      sf = so;
+    }
+    return true;
+  };
+  auto synced = [&]() { so = sf = 0; };  // after a stream synchronisation
+  // One synchronisation epoch stages at most ~32 T bytes plus alignment.
+  auto put = [&](const void* src, size_t bytes) -> void* {
+    const size_t a = (so + 15) & ~(size_t)15;
+    if (a + bytes > stage_cap) {
+      e = hipErrorOutOfMemory;
+      return nullptr;
+    }
+    std::memcpy(c->pin_h + a, src, bytes);
+    so = a + bytes;
+    return c->pin_d + a;
+  };
+  // Vector operations touch only the listed target rows (null: all rows).
+  auto stage_rows = [&](const std::vector<int>& rows) -> const int* {
+    if ((int)rows.size() == T || rows.empty()) return nullptr;
+    return (const int*)put(rows.data(), rows.size() * sizeof(int));
   };
   // Up to kRed per-target reductions over `rows`, one synchronisation;
   // h_red[r * T + t] holds reduction r for each listed t.
   struct Red { const float* x; const float* y; int mode; };
   auto reduce = [&](std::initializer_list<Red> ops, const std::vector<int>& rows) -> bool {
     if (rows.empty()) return true;
-    const int* rd = upload_rows(0, rows);
-    if (e != hipSuccess) return false;
+    const int* rd = stage_rows(rows);
+    if (e != hipSuccess || !flush()) return false;
     int r = 0;
     for (const Red& op : ops) {
       e = huvk::launch_vec_reduce(op.x, op.y, N, (int)rows.size(), rd, nblk, op.mode,
@@ -287,61 +304,62 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     e = hipMemcpyAsync(h_red.data(), red, (size_t)r * T * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return false;
     e = hipStreamSynchronize(s);
+    synced();
     return e == hipSuccess;
   };
-  auto upload_coefs = [&]() -> bool {
-    e = hipMemcpyAsync(coefA, h_a.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(coefB, h_b.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
-    return e == hipSuccess;
-  };
-  // d = a g + b d on the listed rows
+  // d = a g + b d on the listed rows (a, b from h_a, h_b)
   auto dir_rows = [&](const float* g, const std::vector<int>& rows) -> bool {
     if (rows.empty()) return true;
-    if (!upload_coefs()) return false;
-    const int* rd = upload_rows(1, rows);
-    if (e != hipSuccess) return false;
-    e = huvk::launch_vec_dir(g, coefA, coefB, N, (int)rows.size(), rd, dir, s);
+    const double* ca = (const double*)put(h_a.data(), T * sizeof(double));
+    const double* cb = (const double*)put(h_b.data(), T * sizeof(double));
+    const int* rd = stage_rows(rows);
+    if (e != hipSuccess || !ca || !cb || !flush()) return false;
+    e = huvk::launch_vec_dir(g, ca, cb, N, (int)rows.size(), rd, dir, s);
     return e == hipSuccess;
   };
   auto copy_rows = [&](const float* src, float* dst, const std::vector<int>& rows) -> bool {
     if (rows.empty()) return true;
-    const int* rd = upload_rows(2, rows);
-    if (e != hipSuccess) return false;
+    const int* rd = stage_rows(rows);
+    if (e != hipSuccess || !flush()) return false;
     e = huvk::launch_vec_copy_rows(src, N, (int)rows.size(), rd, dst, s);
     return e == hipSuccess;
   };
   // Targets that are done cost nothing: evaluations gather the rows of the
   // targets in `which` into compact buffers (all targets: no copies).
-  char* cmp = (char*)c->ws.get(Workspace::ILT8, 3 * vb + T * sizeof(int), e);
+  char* cmp = (char*)c->ws.get(Workspace::ILT8, 3 * vb, e);
   if (!cmp) return hip_fail(e, "ILT optimizer workspace");
   float* th_c = (float*)cmp;
   float* tg_c = (float*)(cmp + vb);
   float* g_c = (float*)(cmp + 2 * vb);
-  int* idx_d = (int*)(cmp + 3 * vb);
   std::vector<double> cost_c(T);
+  // rows_d: the staged device copy of `which` when the caller staged it
   auto eval_sel = [&](const float* th, float* g, std::vector<double>& out,
-                      const std::vector<int>& which) -> huv_status {
+                      const std::vector<int>& which, const int* rows_d = nullptr) -> huv_status {
     const int Tc = (int)which.size();
     if (Tc == 0) return HUV_OK;
     if (Tc == T) {
+      if (!flush()) return hip_fail(e, "staging copy");
       huv_status r = ilt_eval(p, T, targets, th, g, b);
       if (r != HUV_OK) return r;
       e = hipMemcpyAsync(out.data(), b.cost_d, T * sizeof(double), hipMemcpyDeviceToHost, s);
       if (e == hipSuccess) e = hipStreamSynchronize(s);
+      synced();
       return e == hipSuccess ? HUV_OK : hip_fail(e, "cost copy");
     }
-    e = hipMemcpyAsync(idx_d, which.data(), Tc * sizeof(int), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = huvk::launch_rows_gather(th, idx_d, N, Tc, th_c, s);
-    if (e == hipSuccess) e = huvk::launch_rows_gather(targets, idx_d, N, Tc, tg_c, s);
+    const int* idx = rows_d ? rows_d : stage_rows(which);
+    if (!idx || !flush()) return hip_fail(e, "row staging");
+    e = huvk::launch_rows_gather(th, idx, N, Tc, th_c, s);
+    if (e == hipSuccess) e = huvk::launch_rows_gather(targets, idx, N, Tc, tg_c, s);
     if (e != hipSuccess) return hip_fail(e, "target gather");
     huv_status r = ilt_eval(p, Tc, tg_c, th_c, g ? g_c : nullptr, b);
     if (r != HUV_OK) return r;
     if (g) {
-      e = huvk::launch_rows_scatter(g_c, idx_d, N, Tc, g, s);
+      e = huvk::launch_rows_scatter(g_c, idx, N, Tc, g, s);
       if (e != hipSuccess) return hip_fail(e, "gradient scatter");
     }
     e = hipMemcpyAsync(cost_c.data(), b.cost_d, Tc * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    synced();
     if (e != hipSuccess) return hip_fail(e, "cost copy");
     for (int j = 0; j < Tc; ++j) out[which[j]] = cost_c[j];
     return HUV_OK;
@@ -412,18 +430,15 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
           any_s = any_s || searching[t];
         }
         if (!any_s) break;
-        e = hipMemcpyAsync(coefA, h_a.data(), T * sizeof(double), hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return fail_e("coef upload");
         which.clear();
         for (int t = 0; t < T; ++t)
           if (searching[t]) which.push_back(t);
-        {
-          const int* rd = upload_rows(3, which);
-          if (e != hipSuccess) return fail_e("row upload");
-          e = huvk::launch_vec_axpy(theta, dir, coefA, N, (int)which.size(), rd, trial, s);
-          if (e != hipSuccess) return fail_e("axpy");
-        }
-        st = eval_sel(trial, nullptr, ctrial, which);
+        const double* ca = (const double*)put(h_a.data(), T * sizeof(double));
+        const int* rd = stage_rows(which);
+        if (e != hipSuccess || !ca || !flush()) return fail_e("coef upload");
+        e = huvk::launch_vec_axpy(theta, dir, ca, N, (int)which.size(), rd, trial, s);
+        if (e != hipSuccess) return fail_e("axpy");
+        st = eval_sel(trial, nullptr, ctrial, which, rd);
         if (st != HUV_OK) return st;
         lst.clear();
         for (int t : which) {

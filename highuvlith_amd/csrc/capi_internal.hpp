@@ -57,6 +57,12 @@ struct Ctx {
     std::vector<uint64_t> ksets;
   };
   std::map<std::vector<uint64_t>, GraphEntry> graphs;
+  // Pinned host staging area and its device mirror for small per-step
+  // uploads (the ILT driver's coefficients and row lists), grown on demand.
+  char* pin_h = nullptr;
+  char* pin_d = nullptr;
+  size_t pin_cap = 0;
+  bool staging(size_t bytes, hipError_t& e);
   void drop_graph(GraphEntry& g);
   void forget_kset(uint64_t id);
   // Optional per-stage timing with HIP events on the active stream.

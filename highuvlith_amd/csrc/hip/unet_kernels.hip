@@ -315,88 +315,134 @@ __global__ __launch_bounds__(256) void k_channel_partial(const bf16_t* __restric
 // ---------------------------------------------------------------------------
 // Pooling, upsampling + concatenation, head, loss, optimiser.
 // ---------------------------------------------------------------------------
+// Element-wise kernels on NHWC tensors with channel counts that are multiples
+// of 8: one thread per (pixel, 8-channel group), 16-byte loads and stores.
+union B8 {
+  uint4 u;
+  bf16_t h[8];
+};
+
+__device__ __forceinline__ B8 ld_b8(const bf16_t* p) {
+  B8 r;
+  r.u = *reinterpret_cast<const uint4*>(p);
+  return r;
+}
+__device__ __forceinline__ void st_b8(bf16_t* p, const B8& v) {
+  *reinterpret_cast<uint4*>(p) = v.u;
+}
+
 __global__ void k_maxpool_fwd(const bf16_t* __restrict__ X, int N, int H, int W, int C,
                               bf16_t* __restrict__ Y) {
-  const int Ho = H / 2, Wo = W / 2;
-  const int64_t total = (int64_t)N * Ho * Wo * C;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const int64_t q = i / C;
-    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((int64_t)Wo * Ho));
-    const bf16_t* b = X + (((int64_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
-    float m = bf2f(b[0]);
-    m = fmaxf(m, bf2f(b[C]));
-    m = fmaxf(m, bf2f(b[(int64_t)W * C]));
-    m = fmaxf(m, bf2f(b[(int64_t)W * C + C]));
-    Y[i] = f2bf(m);
+  const int Ho = H / 2, Wo = W / 2, G = C / 8;
+  const int total = N * Ho * Wo * G;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int g = i % G, q = i / G;
+    const int xo = q % Wo, t = q / Wo, yo = t % Ho, n = t / Ho;
+    const bf16_t* b = X + (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + 8 * g;
+    const B8 v0 = ld_b8(b), v1 = ld_b8(b + C), v2 = ld_b8(b + (size_t)W * C),
+             v3 = ld_b8(b + (size_t)W * C + C);
+    B8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      // the maximum is one of the inputs: keep its bits (no rounding)
+      bf16_t m = v0.h[e];
+      if (bf2f(v1.h[e]) > bf2f(m)) m = v1.h[e];
+      if (bf2f(v2.h[e]) > bf2f(m)) m = v2.h[e];
+      if (bf2f(v3.h[e]) > bf2f(m)) m = v3.h[e];
+      o.h[e] = m;
+    }
+    st_b8(Y + (size_t)q * C + 8 * g, o);
   }
 }
 
+// dX: dY routed to the first maximum of each 2 x 2 window, zero elsewhere.
 __global__ void k_maxpool_bwd(const bf16_t* __restrict__ X, const bf16_t* __restrict__ dY, int N,
                               int H, int W, int C, bf16_t* __restrict__ dX) {
-  const int Ho = H / 2, Wo = W / 2;
-  const int64_t total = (int64_t)N * Ho * Wo * C;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const int64_t q = i / C;
-    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((int64_t)Wo * Ho));
-    const int64_t o[4] = {0, C, (int64_t)W * C, (int64_t)W * C + C};
-    const int64_t base = (((int64_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
-    int arg = 0;
-    float m = bf2f(X[base]);
-    for (int k = 1; k < 4; ++k) {
-      const float v = bf2f(X[base + o[k]]);
-      if (v > m) { m = v; arg = k; }
+  const int Ho = H / 2, Wo = W / 2, G = C / 8;
+  const int total = N * Ho * Wo * G;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int g = i % G, q = i / G;
+    const int xo = q % Wo, t = q / Wo, yo = t % Ho, n = t / Ho;
+    const size_t base = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + 8 * g;
+    const size_t o[4] = {0, (size_t)C, (size_t)W * C, (size_t)W * C + C};
+    B8 v[4], d[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ld_b8(X + base + o[k]);
+    const B8 dy = ld_b8(dY + (size_t)q * C + 8 * g);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      int arg = 0;
+      float m = bf2f(v[0].h[e]);
+#pragma unroll
+      for (int k = 1; k < 4; ++k) {
+        const float x = bf2f(v[k].h[e]);
+        if (x > m) { m = x; arg = k; }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d[k].h[e] = k == arg ? dy.h[e] : (bf16_t)0;
     }
-    for (int k = 0; k < 4; ++k) dX[base + o[k]] = k == arg ? dY[i] : (bf16_t)0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) st_b8(dX + base + o[k], d[k]);
   }
 }
 
+// cat = [nearest 2x upsample of U (Cu channels), S (Cs channels)]
 __global__ void k_upcat_fwd(const bf16_t* __restrict__ U, int Cu, const bf16_t* __restrict__ S,
                             int Cs, int N, int H, int W, bf16_t* __restrict__ cat) {
-  const int Ct = Cu + Cs;
-  const int64_t total = (int64_t)N * H * W * Ct;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % Ct);
-    const int64_t p = i / Ct;
-    if (c < Cu) {
-      const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((int64_t)W * H));
-      cat[i] = U[(((int64_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Cu + c];
+  const int Ct = Cu + Cs, G = Ct / 8, Gu = Cu / 8;
+  const int total = N * H * W * G;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int g = i % G, p = i / G;
+    B8 v;
+    if (g < Gu) {
+      const int x = p % W, t = p / W, y = t % H, n = t / H;
+      v = ld_b8(U + (((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Cu + 8 * g);
     } else {
-      cat[i] = S[p * Cs + (c - Cu)];
+      v = ld_b8(S + (size_t)p * Cs + 8 * (g - Gu));
     }
+    st_b8(cat + (size_t)p * Ct + 8 * g, v);
   }
 }
 
+// dU = sum of dcat's first Cu channels over each 2 x 2 block
 __global__ void k_upcat_bwd_u(const bf16_t* __restrict__ dcat, int Cu, int Cs, int N, int H,
                               int W, bf16_t* __restrict__ dU) {
-  const int Ct = Cu + Cs, Hh = H / 2, Wh = W / 2;
-  const int64_t total = (int64_t)N * Hh * Wh * Cu;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % Cu);
-    const int64_t q = i / Cu;
-    const int x = (int)(q % Wh), y = (int)((q / Wh) % Hh), n = (int)(q / ((int64_t)Wh * Hh));
-    const int64_t b = (((int64_t)n * H + 2 * y) * W + 2 * x) * Ct + c;
-    const float s = (bf2f(dcat[b]) + bf2f(dcat[b + Ct])) +
-                    (bf2f(dcat[b + (int64_t)W * Ct]) + bf2f(dcat[b + (int64_t)W * Ct + Ct]));
-    dU[i] = f2bf(s);
+  const int Ct = Cu + Cs, Hh = H / 2, Wh = W / 2, G = Cu / 8;
+  const int total = N * Hh * Wh * G;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int g = i % G, q = i / G;
+    const int x = q % Wh, t = q / Wh, y = t % Hh, n = t / Hh;
+    const size_t b = (((size_t)n * H + 2 * y) * W + 2 * x) * Ct + 8 * g;
+    const B8 a0 = ld_b8(dcat + b), a1 = ld_b8(dcat + b + Ct), a2 = ld_b8(dcat + b + (size_t)W * Ct),
+             a3 = ld_b8(dcat + b + (size_t)W * Ct + Ct);
+    B8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      o.h[e] = f2bf((bf2f(a0.h[e]) + bf2f(a1.h[e])) + (bf2f(a2.h[e]) + bf2f(a3.h[e])));
+    st_b8(dU + (size_t)q * Cu + 8 * g, o);
   }
 }
 
+// dS (+)= dcat's last Cs channels
 __global__ void k_upcat_bwd_s(const bf16_t* __restrict__ dcat, int Cu, int Cs, int64_t P,
                               bf16_t* __restrict__ dS, int accumulate) {
-  const int Ct = Cu + Cs;
-  const int64_t total = P * Cs;
+  const int Ct = Cu + Cs, G = Cs / 8;
+  const int64_t total = P * G;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % Cs);
-    const int64_t p = i / Cs;
-    const bf16_t g = dcat[p * Ct + Cu + c];
-    dS[i] = accumulate ? f2bf(bf2f(dS[i]) + bf2f(g)) : g;
+    const int g = (int)(i % G);
+    const int64_t p = i / G;
+    const B8 v = ld_b8(dcat + (size_t)p * Ct + Cu + 8 * g);
+    bf16_t* d = dS + (size_t)p * Cs + 8 * g;
+    if (accumulate) {
+      const B8 w = ld_b8(d);
+      B8 o;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) o.h[e] = f2bf(bf2f(w.h[e]) + bf2f(v.h[e]));
+      st_b8(d, o);
+    } else {
+      st_b8(d, v);
+    }
   }
 }
 
@@ -590,32 +636,38 @@ hipError_t launch_channel_sum(const bf16_t* dYT, int N, int HW, int C, float* pa
 
 hipError_t launch_maxpool_fwd(const bf16_t* X, int N, int H, int W, int C, bf16_t* Y,
                               hipStream_t s) {
-  hipLaunchKernelGGL(k_maxpool_fwd, grid1d((int64_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s,
-                     X, N, H, W, C, Y);
+  if (C % 8 != 0 || (int64_t)N * H * W * C / 8 >= (1ll << 30)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_maxpool_fwd, grid1d((int64_t)N * (H / 2) * (W / 2) * C / 8), dim3(256), 0,
+                     s, X, N, H, W, C, Y);
   return hipGetLastError();
 }
 
 hipError_t launch_maxpool_bwd(const bf16_t* X, const bf16_t* dY, int N, int H, int W, int C,
                               bf16_t* dX, hipStream_t s) {
-  hipLaunchKernelGGL(k_maxpool_bwd, grid1d((int64_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s,
-                     X, dY, N, H, W, C, dX);
+  if (C % 8 != 0 || (int64_t)N * H * W * C / 8 >= (1ll << 30)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_maxpool_bwd, grid1d((int64_t)N * (H / 2) * (W / 2) * C / 8), dim3(256), 0,
+                     s, X, dY, N, H, W, C, dX);
   return hipGetLastError();
 }
 
 hipError_t launch_upcat_fwd(const bf16_t* U, int Cu, const bf16_t* S, int Cs, int N, int H,
                             int W, bf16_t* cat, hipStream_t s) {
-  hipLaunchKernelGGL(k_upcat_fwd, grid1d((int64_t)N * H * W * (Cu + Cs)), dim3(256), 0, s, U, Cu,
+  if (Cu % 8 != 0 || Cs % 8 != 0 || (int64_t)N * H * W * (Cu + Cs) / 8 >= (1ll << 30))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_upcat_fwd, grid1d((int64_t)N * H * W * (Cu + Cs) / 8), dim3(256), 0, s, U, Cu,
                      S, Cs, N, H, W, cat);
   return hipGetLastError();
 }
 
 hipError_t launch_upcat_bwd(const bf16_t* dcat, int Cu, int Cs, int N, int H, int W, bf16_t* dU,
                             bf16_t* dS, int accumulate, hipStream_t s) {
-  hipLaunchKernelGGL(k_upcat_bwd_u, grid1d((int64_t)N * (H / 2) * (W / 2) * Cu), dim3(256), 0, s,
-                     dcat, Cu, Cs, N, H, W, dU);
+  if (Cu % 8 != 0 || Cs % 8 != 0 || (int64_t)N * H * W * (Cu + Cs) / 8 >= (1ll << 30))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_upcat_bwd_u, grid1d((int64_t)N * (H / 2) * (W / 2) * Cu / 8), dim3(256), 0,
+                     s, dcat, Cu, Cs, N, H, W, dU);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_upcat_bwd_s, grid1d((int64_t)N * H * W * Cs), dim3(256), 0, s, dcat, Cu,
+  hipLaunchKernelGGL(k_upcat_bwd_s, grid1d((int64_t)N * H * W * Cs / 8), dim3(256), 0, s, dcat, Cu,
                      Cs, (int64_t)N * H * W, dS, accumulate);
   return hipGetLastError();
 }
