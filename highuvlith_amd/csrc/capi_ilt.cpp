@@ -240,12 +240,11 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   const int64_t N = (int64_t)p->n * p->n;
   const size_t vb = (size_t)T * N * sizeof(float);
   hipError_t e;
-  char* vec = (char*)c->ws.get(Workspace::ILT6, 4 * vb, e);
+  char* vec = (char*)c->ws.get(Workspace::ILT6, 3 * vb, e);
   if (!vec) return hip_fail(e, "ILT optimizer workspace");
   float* grad = (float*)vec;
   float* gnew = (float*)(vec + vb);
   float* dir = (float*)(vec + 2 * vb);
-  float* trial = (float*)(vec + 3 * vb);
   const int nblk = 64;
   constexpr int kRed = 3;  // reductions fetched together
   char* sc = (char*)c->ws.get(
@@ -370,9 +369,12 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     return eval_sel(th, g, out, all_t);
   };
   std::vector<int> which;
+  // line-search trial rows: target, coefficient, step; per-target first row and count
+  std::vector<int> prow, pfirst(T), pcount(T), used(T), acc_src, acc_dst;
+  std::vector<double> pcoef, pstep;
 
   const int H = max_iterations + 1;
-  std::vector<double> cost(T), ctrial(T), cnew(T), step(T, learning_rate), slope(T), dmax(T),
+  std::vector<double> cost(T), cnew(T), step(T, learning_rate), slope(T), dmax(T),
       s_try(T), s_used(T), gnorm(T);
   std::vector<int> active(T, 1), small(T, 0), it(T, 0), term(T, 1), attempt(T, 0), searching(T, 0),
       accepted(T, 0);
@@ -422,36 +424,74 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         s_try[t] = step[t];
       }
       if (!dir_rows(grad, rst)) return fail_e("dir");
-      // Batched Armijo backtracking for the searching targets.
-      for (int bt = 0; bt < 25; ++bt) {
-        bool any_s = false;
-        for (int t = 0; t < T; ++t) {
-          h_a[t] = searching[t] ? s_try[t] / dmax[t] : 0.0;
-          any_s = any_s || searching[t];
-        }
-        if (!any_s) break;
+      // Batched Armijo backtracking (c1 = 1e-4, at most 25 trial steps per
+      // target, halving after each failure).  One round evaluates the next
+      // trial of every searching target; when few targets are left, each also
+      // evaluates its following halvings in the same round (up to 4, never
+      // more trial rows than targets) and takes the first step that passes --
+      // the step sequential halving takes -- so stragglers cost fewer rounds.
+      std::fill(used.begin(), used.end(), 0);
+      for (;;) {
         which.clear();
         for (int t = 0; t < T; ++t)
           if (searching[t]) which.push_back(t);
-        const double* ca = (const double*)put(h_a.data(), T * sizeof(double));
-        const int* rd = stage_rows(which);
-        if (e != hipSuccess || !ca || !flush()) return fail_e("coef upload");
-        e = huvk::launch_vec_axpy(theta, dir, ca, N, (int)which.size(), rd, trial, s);
-        if (e != hipSuccess) return fail_e("axpy");
-        st = eval_sel(trial, nullptr, ctrial, which, rd);
-        if (st != HUV_OK) return st;
-        lst.clear();
+        if (which.empty()) break;
+        const int nb = std::max(1, std::min(4, T / (int)which.size()));
+        prow.clear();
+        pcoef.clear();
+        pstep.clear();
         for (int t : which) {
-          if (ctrial[t] <= cost[t] + 1e-4 * s_try[t] * slope[t] && std::isfinite(ctrial[t])) {
-            searching[t] = 0;
-            accepted[t] = 1;
-            s_used[t] = s_try[t];
-            lst.push_back(t);
-          } else {
-            s_try[t] *= 0.5;
+          const int k = std::min(nb, 25 - used[t]);
+          pfirst[t] = (int)prow.size();
+          pcount[t] = k;
+          double sj = s_try[t];
+          for (int j = 0; j < k; ++j, sj *= 0.5) {
+            prow.push_back(t);
+            pcoef.push_back(sj / dmax[t]);
+            pstep.push_back(sj);
           }
         }
-        if (!copy_rows(trial, theta, lst)) return fail_e("select");
+        const int np = (int)prow.size();  // <= T
+        const int* prow_d = (const int*)put(prow.data(), np * sizeof(int));
+        const double* pcoef_d = (const double*)put(pcoef.data(), np * sizeof(double));
+        if (!prow_d || !pcoef_d || !flush()) return fail_e("trial staging");
+        e = huvk::launch_vec_axpy_gather(theta, dir, pcoef_d, N, np, prow_d, th_c, s);
+        if (e == hipSuccess) e = huvk::launch_rows_gather(targets, prow_d, N, np, tg_c, s);
+        if (e != hipSuccess) return fail_e("trial build");
+        st = ilt_eval(p, np, tg_c, th_c, nullptr, b);
+        if (st != HUV_OK) return st;
+        e = hipMemcpyAsync(cost_c.data(), b.cost_d, np * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        synced();
+        if (e != hipSuccess) return fail_e("cost copy");
+        acc_src.clear();
+        acc_dst.clear();
+        for (int t : which) {
+          bool ok = false;
+          for (int j = pfirst[t]; j < pfirst[t] + pcount[t]; ++j) {
+            if (cost_c[j] <= cost[t] + 1e-4 * pstep[j] * slope[t] && std::isfinite(cost_c[j])) {
+              searching[t] = 0;
+              accepted[t] = 1;
+              s_used[t] = pstep[j];
+              acc_src.push_back(j);
+              acc_dst.push_back(t);
+              ok = true;
+              break;
+            }
+          }
+          if (!ok) {
+            used[t] += pcount[t];
+            s_try[t] = pstep[pfirst[t] + pcount[t] - 1] * 0.5;
+            if (used[t] >= 25) searching[t] = 0;  // exhausted
+          }
+        }
+        if (!acc_src.empty()) {
+          const int* sr = (const int*)put(acc_src.data(), acc_src.size() * sizeof(int));
+          const int* dr = (const int*)put(acc_dst.data(), acc_dst.size() * sizeof(int));
+          if (!sr || !dr || !flush()) return fail_e("accept staging");
+          e = huvk::launch_rows_copy_map(th_c, sr, theta, dr, N, (int)acc_src.size(), s);
+          if (e != hipSuccess) return fail_e("accept copy");
+        }
       }
       // Targets whose search failed: retry once along -grad with CG, else stop.
       rst.clear();

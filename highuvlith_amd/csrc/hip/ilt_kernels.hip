@@ -427,6 +427,29 @@ __global__ void k_vec_copy_rows(const float* __restrict__ src, int64_t npix,
     dst[base + i] = src[base + i];
 }
 
+// Line-search trials: out[j] = x[rows[j]] + coef[j] d[rows[j]] (compact rows).
+__global__ void k_vec_axpy_gather(const float* __restrict__ x, const float* __restrict__ d,
+                                  const double* __restrict__ coef, int64_t npix,
+                                  const int* __restrict__ rows, float* __restrict__ out) {
+  const int j = blockIdx.y;
+  const size_t src = (size_t)rows[j] * npix, dst = (size_t)j * npix;
+  const double c = coef[j];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[dst + i] = (float)((double)x[src + i] + c * (double)d[src + i]);
+}
+
+// dst[drows[j]] = src[srows[j]] for j < count
+__global__ void k_rows_copy_map(const float* __restrict__ src, const int* __restrict__ srows,
+                                float* __restrict__ dst, const int* __restrict__ drows,
+                                int64_t npix) {
+  const int j = blockIdx.y;
+  const size_t a = (size_t)srows[j] * npix, b = (size_t)drows[j] * npix;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[b + i] = src[a + i];
+}
+
 // Row gather / scatter of per-target images: dst[j] = src[idx[j]] or
 // dst[idx[j]] = src[j], j < nrows (rows of npix floats).
 template <bool SCATTER>
@@ -574,6 +597,22 @@ hipError_t launch_rows_scatter(const float* src, const int* idx, int64_t npix, i
                                float* dst, hipStream_t s) {
   hipLaunchKernelGGL(k_rows_move<true>, dim3(grid_for(npix * nrows, 256)), dim3(256), 0, s, src,
                      idx, npix, nrows, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_axpy_gather(const float* x, const float* d, const double* coef, int64_t npix,
+                                  int count, const int* rows, float* out, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vec_axpy_gather, rows_grid(npix, count), dim3(256), 0, s, x, d, coef, npix,
+                     rows, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_copy_map(const float* src, const int* srows, float* dst, const int* drows,
+                                int64_t npix, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows_copy_map, rows_grid(npix, count), dim3(256), 0, s, src, srows, dst,
+                     drows, npix);
   return hipGetLastError();
 }
 

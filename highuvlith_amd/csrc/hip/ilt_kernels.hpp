@@ -47,6 +47,12 @@ hipError_t launch_rows_gather(const float* src, const int* idx, int64_t npix, in
                               float* dst, hipStream_t s);
 hipError_t launch_rows_scatter(const float* src, const int* idx, int64_t npix, int nrows,
                                float* dst, hipStream_t s);
+// out[j] = x[rows[j]] + coef[j] d[rows[j]], j < count (compact line-search trials)
+hipError_t launch_vec_axpy_gather(const float* x, const float* d, const double* coef, int64_t npix,
+                                  int count, const int* rows, float* out, hipStream_t s);
+// dst[drows[j]] = src[srows[j]], j < count
+hipError_t launch_rows_copy_map(const float* src, const int* srows, float* dst, const int* drows,
+                                int64_t npix, int count, hipStream_t s);
 // dst[t] = src[t] for the listed rows
 hipError_t launch_vec_copy_rows(const float* src, int64_t npix, int count, const int* rows,
                                 float* dst, hipStream_t s);
