@@ -331,9 +331,8 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   float* tg_c = (float*)(cmp + vb);
   float* g_c = (float*)(cmp + 2 * vb);
   std::vector<double> cost_c(T);
-  // rows_d: the staged device copy of `which` when the caller staged it
   auto eval_sel = [&](const float* th, float* g, std::vector<double>& out,
-                      const std::vector<int>& which, const int* rows_d = nullptr) -> huv_status {
+                      const std::vector<int>& which) -> huv_status {
     const int Tc = (int)which.size();
     if (Tc == 0) return HUV_OK;
     if (Tc == T) {
@@ -345,7 +344,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       synced();
       return e == hipSuccess ? HUV_OK : hip_fail(e, "cost copy");
     }
-    const int* idx = rows_d ? rows_d : stage_rows(which);
+    const int* idx = stage_rows(which);
     if (!idx || !flush()) return hip_fail(e, "row staging");
     e = huvk::launch_rows_gather(th, idx, N, Tc, th_c, s);
     if (e == hipSuccess) e = huvk::launch_rows_gather(targets, idx, N, Tc, tg_c, s);

@@ -394,18 +394,6 @@ __global__ void k_vec_finish(const double* __restrict__ part, int count, int nbl
   out[t] = a;
 }
 
-// y = x + coef[t] * d   (per-target coefficient)
-__global__ void k_vec_axpy(const float* __restrict__ x, const float* __restrict__ d,
-                           const double* __restrict__ coef, int64_t npix,
-                           const int* __restrict__ rows, float* __restrict__ y) {
-  const int t = row_of(rows, blockIdx.y);
-  const size_t base = (size_t)t * npix;
-  const double c = coef[t];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
-       i += (int64_t)gridDim.x * blockDim.x)
-    y[base + i] = (float)((double)x[base + i] + c * (double)d[base + i]);
-}
-
 // d = a[t] * g + b[t] * d
 __global__ void k_vec_dir(const float* __restrict__ g, const double* __restrict__ a,
                           const double* __restrict__ b, int64_t npix, const int* __restrict__ rows,
@@ -568,14 +556,6 @@ hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int c
                      mode, rows, part);
   hipLaunchKernelGGL(k_vec_finish, dim3((count + 63) / 64), dim3(64), 0, s, part, count, nblk,
                      mode, rows, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_vec_axpy(const float* x, const float* d, const double* coef, int64_t npix,
-                           int count, const int* rows, float* y, hipStream_t s) {
-  if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vec_axpy, rows_grid(npix, count), dim3(256), 0, s, x, d, coef, npix, rows,
-                     y);
   return hipGetLastError();
 }
 

@@ -36,9 +36,6 @@ hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m,
 hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int count,
                              const int* rows, int nblk, int mode, double* part, double* out,
                              hipStream_t s);
-// y = x + coef[t] d
-hipError_t launch_vec_axpy(const float* x, const float* d, const double* coef, int64_t npix,
-                           int count, const int* rows, float* y, hipStream_t s);
 // d = a[t] g + b[t] d
 hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int64_t npix,
                           int count, const int* rows, float* d, hipStream_t s);
