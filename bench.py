@@ -338,8 +338,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-ilt", action="store_true")
     ap.add_argument("--no-warm", action="store_true")
-    ap.add_argument("--warm-pairs", type=int, default=1024)
-    ap.add_argument("--warm-epochs", type=int, default=20)
+    ap.add_argument("--warm-pairs", type=int, default=4096)
+    ap.add_argument("--warm-epochs", type=int, default=16)
     ap.add_argument("--warm-eval", type=int, default=512)
     ap.add_argument("--uneven-clips", type=int, default=1000)
     ap.add_argument("--ilt-targets", type=int, default=64)
