@@ -129,6 +129,87 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd(const bf16_t* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Forward 3x3 convolution with the input staged in LDS.  Work-group = 4 waves
+// over an output tile of 2 rows x 64 columns of one image; wave w takes row
+// y0 + w / 2, columns x0 + 32 (w % 2) .. + 31, and BN blocks of 32 output
+// channels.  Input channels run in chunks of CC: the (2 + 2) x (64 + 2) x CC
+// halo is loaded once per chunk (zero outside the image) and read by all nine
+// taps from LDS; A = 32 pixels x 16 k per MFMA step (two 8-channel blocks of
+// (tap, channel) index 2s and 2s + 1), B = weights from global memory.
+// Needs W % 64 == 0 and H % 2 == 0.
+// ---------------------------------------------------------------------------
+template <int BN, int CC>
+__global__ __launch_bounds__(256) void k_conv3x3_fwd_lds(const bf16_t* __restrict__ X, int N,
+                                                         int H, int W, int Cin,
+                                                         const bf16_t* __restrict__ Wt,
+                                                         const float* __restrict__ bias, int Cout,
+                                                         int relu, bf16_t* __restrict__ Y) {
+  constexpr int TH = 2, TW = 64, HR = TH + 2, HC = TW + 2;
+  constexpr int PS = CC + 8;  // pixel stride in the tile (bf16), spreads the LDS banks
+  constexpr int CB = CC / 8, KBC = 9 * CB, STEPS = (KBC + 1) / 2;
+  __shared__ __attribute__((aligned(16))) bf16_t tile[HR * HC * PS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int half = lane >> 5, li = lane & 31;
+  const int tilesX = W / TW, tilesY = H / TH;
+  int blk = blockIdx.x;
+  const int tx = blk % tilesX;
+  blk /= tilesX;
+  const int ty = blk % tilesY, n = blk / tilesY;
+  const int x0 = tx * TW, y0 = ty * TH;
+  const int cobase = blockIdx.y * 32 * BN;
+  const int py = wave >> 1, px = 32 * (wave & 1) + li;  // this lane's A row (pixel) in the tile
+  int co[BN];
+#pragma unroll
+  for (int bn = 0; bn < BN; ++bn) co[bn] = cobase + 32 * bn + li;
+  f32x16 acc[BN];
+#pragma unroll
+  for (int bn = 0; bn < BN; ++bn)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[bn][r] = 0.f;
+
+  for (int c0 = 0; c0 < Cin; c0 += CC) {
+    __syncthreads();  // the previous chunk has been consumed
+    for (int i = threadIdx.x; i < HR * HC * CB; i += 256) {
+      const int cb = i % CB, pix = i / CB;
+      const int yy = y0 - 1 + pix / HC, xx = x0 - 1 + pix % HC;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+        v = *reinterpret_cast<const uint4*>(X + (((size_t)n * H + yy) * W + xx) * Cin + c0 + 8 * cb);
+      *reinterpret_cast<uint4*>(tile + pix * PS + 8 * cb) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+      const int kb = 2 * s + half;
+      const bool kv = kb < KBC;
+      const int tap = kv ? kb / CB : 0, cb = kv ? kb % CB : 0;
+      const int dy = tap / 3, dx = tap % 3;
+      const bf16x8 a = kv ? ld8(tile + ((py + dy) * HC + px + dx) * PS + 8 * cb) : zero8();
+#pragma unroll
+      for (int bn = 0; bn < BN; ++bn) {
+        const bf16x8 b = (kv && co[bn] < Cout)
+                             ? ld8(Wt + ((size_t)co[bn] * 9 + tap) * Cin + c0 + 8 * cb)
+                             : zero8();
+        acc[bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[bn], 0, 0, 0);
+      }
+    }
+  }
+  const size_t prow = ((size_t)n * H + y0 + py) * W + x0 + 32 * (wave & 1);
+#pragma unroll
+  for (int bn = 0; bn < BN; ++bn) {
+    if (co[bn] >= Cout) continue;
+    const float bv = bias ? bias[co[bn]] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = 8 * (r / 4) + 4 * half + (r % 4);
+      float v = acc[bn][r] + bv;
+      if (relu) v = fmaxf(v, 0.f);
+      Y[(prow + i) * Cout + co[bn]] = f2bf(v);
+    }
+  }
+}
+
 // Wr[ci][t][co] = W[co][8 - t][ci].
 __global__ void k_conv_weight_rot(const bf16_t* __restrict__ Wt, int Cout, int Cin,
                                   bf16_t* __restrict__ Wr) {
@@ -562,6 +643,24 @@ hipError_t launch_conv3x3_fwd(const bf16_t* X, int N, int H, int W, int Cin, con
                               const float* bias, int Cout, int relu, bf16_t* Y, hipStream_t s) {
   if (Cin % 8 != 0 || Cout < 1 || N < 1 || H < 1 || W < 1) return hipErrorInvalidValue;
   const int64_t NP = (int64_t)N * H * W;
+  if (W % 64 == 0 && H % 2 == 0 && (Cin == 8 || Cin == 16 || Cin % 32 == 0) &&
+      (int64_t)N * (H / 2) * (W / 64) < (1ll << 31)) {
+    dim3 grid((unsigned)((int64_t)N * (H / 2) * (W / 64)), (unsigned)((Cout + (Cout <= 32 ? 31 : 63)) / (Cout <= 32 ? 32 : 64)));
+#define HUV_CONV_LDS(BN_, CC_)                                                                 \
+  hipLaunchKernelGGL((k_conv3x3_fwd_lds<BN_, CC_>), grid, dim3(256), 0, s, X, N, H, W, Cin, Wt, \
+                     bias, Cout, relu, Y)
+    if (Cout <= 32) {
+      if (Cin == 8) HUV_CONV_LDS(1, 8);
+      else if (Cin == 16) HUV_CONV_LDS(1, 16);
+      else HUV_CONV_LDS(1, 32);
+    } else {
+      if (Cin == 8) HUV_CONV_LDS(2, 8);
+      else if (Cin == 16) HUV_CONV_LDS(2, 16);
+      else HUV_CONV_LDS(2, 32);
+    }
+#undef HUV_CONV_LDS
+    return hipGetLastError();
+  }
   constexpr int BM = 2;
   if (Cout <= 32) {
     dim3 grid((unsigned)((NP + 4 * 32 * BM - 1) / (4 * 32 * BM)), (unsigned)((Cout + 31) / 32));
