@@ -17,13 +17,13 @@ Also reported in the same JSON line (each skipped with --no-uneven,
            eager pass, groups issued smallest first).
   ilt      64 contact-array targets 512^2 at 2 nm, ArF 193i quasar, K = 24,
            resist cost (t 0.25, alpha 50), 50 CG/Armijo iterations; clips/s.
-  warm     U-Net warm start (4 levels, 32..256 channels, BF16 on the matrix
+  warm     U-Net warm start (4 levels, 16..128 channels, BF16 on the matrix
            cores): trained on (target, 200-iteration ILT theta) pairs from
            the GPU ILT (seeded contact and line/space targets, 512^2 at 2 nm),
            then on held-out targets plain ILT (200 iterations from the
            target) vs warm start refined to the same per-target cost;
            clips/s to that fixed cost for both.  The default sizes are a
-           reduced version of the 20,000-pair / 20-epoch training
+           reduced version of the 20,000-pair / 16-epoch training
            (--warm-pairs, --warm-epochs, --warm-eval).
 
 `roofline`: the dominant kernel (socs_fields) against the FP32 peak with

@@ -360,13 +360,25 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16_t* __restrict_
     }
 }
 
-__global__ void k_sum_splits(const float* __restrict__ part, int splits, int64_t count,
-                             float* __restrict__ out) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    float s = 0.f;
-    for (int k = 0; k < splits; ++k) s += part[(int64_t)k * count + i];
-    out[i] = s;
+// out[i] = sum over k of part[k][i]: 32 outputs per work-group, the splits
+// dealt round-robin over 32 groups of 32 lanes (each summed in increasing k),
+// then the 32 group sums added in group order, so the result does not depend
+// on scheduling.
+__global__ __launch_bounds__(1024) void k_sum_splits(const float* __restrict__ part, int splits,
+                                                     int64_t count, float* __restrict__ out) {
+  __shared__ float sh[32][33];
+  const int j = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int64_t i = (int64_t)blockIdx.x * 32 + j;
+  float s = 0.f;
+  if (i < count)
+    for (int k = g; k < splits; k += 32) s += part[(int64_t)k * count + i];
+  sh[g][j] = s;
+  __syncthreads();
+  if (g == 0 && i < count) {
+    float t = 0.f;
+#pragma unroll
+    for (int q = 0; q < 32; ++q) t += sh[q][j];
+    out[i] = t;
   }
 }
 
@@ -643,7 +655,7 @@ hipError_t launch_conv3x3_fwd(const bf16_t* X, int N, int H, int W, int Cin, con
                               const float* bias, int Cout, int relu, bf16_t* Y, hipStream_t s) {
   if (Cin % 8 != 0 || Cout < 1 || N < 1 || H < 1 || W < 1) return hipErrorInvalidValue;
   const int64_t NP = (int64_t)N * H * W;
-  if (W % 64 == 0 && H % 2 == 0 && (Cin == 8 || Cin == 16 || Cin % 32 == 0) &&
+  if (W % 64 == 0 && H % 2 == 0 && Cin % 8 == 0 && (Cin <= 16 || Cin % 16 == 0) &&
       (int64_t)N * (H / 2) * (W / 64) < (1ll << 31)) {
     dim3 grid((unsigned)((int64_t)N * (H / 2) * (W / 64)), (unsigned)((Cout + (Cout <= 32 ? 31 : 63)) / (Cout <= 32 ? 32 : 64)));
 #define HUV_CONV_LDS(BN_, CC_)                                                                 \
@@ -651,11 +663,11 @@ hipError_t launch_conv3x3_fwd(const bf16_t* X, int N, int H, int W, int Cin, con
                      bias, Cout, relu, Y)
     if (Cout <= 32) {
       if (Cin == 8) HUV_CONV_LDS(1, 8);
-      else if (Cin == 16) HUV_CONV_LDS(1, 16);
+      else if (Cin % 32 != 0) HUV_CONV_LDS(1, 16);
       else HUV_CONV_LDS(1, 32);
     } else {
       if (Cin == 8) HUV_CONV_LDS(2, 8);
-      else if (Cin == 16) HUV_CONV_LDS(2, 16);
+      else if (Cin % 32 != 0) HUV_CONV_LDS(2, 16);
       else HUV_CONV_LDS(2, 32);
     }
 #undef HUV_CONV_LDS
@@ -718,7 +730,9 @@ hipError_t launch_conv3x3_wgrad(const bf16_t* dYT, const bf16_t* XT, int N, int 
 
 hipError_t launch_sum_splits(const float* part, int splits, int64_t count, float* out,
                              hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_splits, grid1d(count), dim3(256), 0, s, part, splits, count, out);
+  if (count <= 0 || splits < 1 || (count + 31) / 32 >= (1ll << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((count + 31) / 32)), dim3(1024), 0, s, part,
+                     splits, count, out);
   return hipGetLastError();
 }
 
