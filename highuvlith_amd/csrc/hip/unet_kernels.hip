@@ -131,21 +131,22 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd(const bf16_t* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // Forward 3x3 convolution with the input staged in LDS.  Work-group = 4 waves
-// over an output tile of 2 rows x 64 columns of one image; wave w takes row
-// y0 + w / 2, columns x0 + 32 (w % 2) .. + 31, and BN blocks of 32 output
-// channels.  Input channels run in chunks of CC: the (2 + 2) x (64 + 2) x CC
-// halo is loaded once per chunk (zero outside the image) and read by all nine
-// taps from LDS; A = 32 pixels x 16 k per MFMA step (two 8-channel blocks of
+// over an output tile of TH = 2 BM rows x 64 columns of one image; wave w takes
+// columns x0 + 32 (w % 2) .. + 31 of rows y0 + w / 2 + 2 b (b < BM), and BN
+// blocks of 32 output channels, so one weight fragment feeds BM products.
+// Input channels run in chunks of CC: the (TH + 2) x (64 + 2) x CC halo is
+// loaded once per chunk (zero outside the image) and read by all nine taps
+// from LDS; A = 32 pixels x 16 k per MFMA step (two 8-channel blocks of
 // (tap, channel) index 2s and 2s + 1), B = weights from global memory.
-// Needs W % 64 == 0 and H % 2 == 0.
+// Needs W % 64 == 0 and H % TH == 0.
 // ---------------------------------------------------------------------------
-template <int BN, int CC>
+template <int BM, int BN, int CC>
 __global__ __launch_bounds__(256) void k_conv3x3_fwd_lds(const bf16_t* __restrict__ X, int N,
                                                          int H, int W, int Cin,
                                                          const bf16_t* __restrict__ Wt,
                                                          const float* __restrict__ bias, int Cout,
                                                          int relu, bf16_t* __restrict__ Y) {
-  constexpr int TH = 2, TW = 64, HR = TH + 2, HC = TW + 2;
+  constexpr int TH = 2 * BM, TW = 64, HR = TH + 2, HC = TW + 2;
   constexpr int PS = CC + 8;  // pixel stride in the tile (bf16), spreads the LDS banks
   constexpr int CB = CC / 8, KBC = 9 * CB, STEPS = (KBC + 1) / 2;
   __shared__ __attribute__((aligned(16))) bf16_t tile[HR * HC * PS];
@@ -158,15 +159,17 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd_lds(const bf16_t* __restric
   const int ty = blk % tilesY, n = blk / tilesY;
   const int x0 = tx * TW, y0 = ty * TH;
   const int cobase = blockIdx.y * 32 * BN;
-  const int py = wave >> 1, px = 32 * (wave & 1) + li;  // this lane's A row (pixel) in the tile
+  const int px = 32 * (wave & 1) + li;  // this lane's A row (pixel column) in the tile
   int co[BN];
 #pragma unroll
   for (int bn = 0; bn < BN; ++bn) co[bn] = cobase + 32 * bn + li;
-  f32x16 acc[BN];
+  f32x16 acc[BM][BN];
 #pragma unroll
-  for (int bn = 0; bn < BN; ++bn)
+  for (int bm = 0; bm < BM; ++bm)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[bn][r] = 0.f;
+    for (int bn = 0; bn < BN; ++bn)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[bm][bn][r] = 0.f;
 
   for (int c0 = 0; c0 < Cin; c0 += CC) {
     __syncthreads();  // the previous chunk has been consumed
@@ -185,27 +188,35 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd_lds(const bf16_t* __restric
       const bool kv = kb < KBC;
       const int tap = kv ? kb / CB : 0, cb = kv ? kb % CB : 0;
       const int dy = tap / 3, dx = tap % 3;
-      const bf16x8 a = kv ? ld8(tile + ((py + dy) * HC + px + dx) * PS + 8 * cb) : zero8();
+      bf16x8 bw[BN];
 #pragma unroll
-      for (int bn = 0; bn < BN; ++bn) {
-        const bf16x8 b = (kv && co[bn] < Cout)
-                             ? ld8(Wt + ((size_t)co[bn] * 9 + tap) * Cin + c0 + 8 * cb)
-                             : zero8();
-        acc[bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[bn], 0, 0, 0);
+      for (int bn = 0; bn < BN; ++bn)
+        bw[bn] = (kv && co[bn] < Cout) ? ld8(Wt + ((size_t)co[bn] * 9 + tap) * Cin + c0 + 8 * cb)
+                                       : zero8();
+#pragma unroll
+      for (int bm = 0; bm < BM; ++bm) {
+        const int py = (wave >> 1) + 2 * bm;
+        const bf16x8 a = kv ? ld8(tile + ((py + dy) * HC + px + dx) * PS + 8 * cb) : zero8();
+#pragma unroll
+        for (int bn = 0; bn < BN; ++bn)
+          acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[bn], acc[bm][bn], 0, 0, 0);
       }
     }
   }
-  const size_t prow = ((size_t)n * H + y0 + py) * W + x0 + 32 * (wave & 1);
 #pragma unroll
-  for (int bn = 0; bn < BN; ++bn) {
-    if (co[bn] >= Cout) continue;
-    const float bv = bias ? bias[co[bn]] : 0.f;
+  for (int bm = 0; bm < BM; ++bm) {
+    const size_t prow = ((size_t)n * H + y0 + (wave >> 1) + 2 * bm) * W + x0 + 32 * (wave & 1);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = 8 * (r / 4) + 4 * half + (r % 4);
-      float v = acc[bn][r] + bv;
-      if (relu) v = fmaxf(v, 0.f);
-      Y[(prow + i) * Cout + co[bn]] = f2bf(v);
+    for (int bn = 0; bn < BN; ++bn) {
+      if (co[bn] >= Cout) continue;
+      const float bv = bias ? bias[co[bn]] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = 8 * (r / 4) + 4 * half + (r % 4);
+        float v = acc[bm][bn][r] + bv;
+        if (relu) v = fmaxf(v, 0.f);
+        Y[(prow + i) * Cout + co[bn]] = f2bf(v);
+      }
     }
   }
 }
@@ -655,11 +666,13 @@ hipError_t launch_conv3x3_fwd(const bf16_t* X, int N, int H, int W, int Cin, con
                               const float* bias, int Cout, int relu, bf16_t* Y, hipStream_t s) {
   if (Cin % 8 != 0 || Cout < 1 || N < 1 || H < 1 || W < 1) return hipErrorInvalidValue;
   const int64_t NP = (int64_t)N * H * W;
-  if (W % 64 == 0 && H % 2 == 0 && Cin % 8 == 0 && (Cin <= 16 || Cin % 16 == 0) &&
-      (int64_t)N * (H / 2) * (W / 64) < (1ll << 31)) {
-    dim3 grid((unsigned)((int64_t)N * (H / 2) * (W / 64)), (unsigned)((Cout + (Cout <= 32 ? 31 : 63)) / (Cout <= 32 ? 32 : 64)));
-#define HUV_CONV_LDS(BN_, CC_)                                                                 \
-  hipLaunchKernelGGL((k_conv3x3_fwd_lds<BN_, CC_>), grid, dim3(256), 0, s, X, N, H, W, Cin, Wt, \
+  if (W % 64 == 0 && H % 4 == 0 && Cin % 8 == 0 && (Cin <= 16 || Cin % 16 == 0) &&
+      (int64_t)N * (H / 4) * (W / 64) < (1ll << 31)) {
+    // tiles of 4 rows x 64 columns: BM = 2 pixel blocks per wave
+    dim3 grid((unsigned)((int64_t)N * (H / 4) * (W / 64)),
+              (unsigned)((Cout + (Cout <= 32 ? 31 : 63)) / (Cout <= 32 ? 32 : 64)));
+#define HUV_CONV_LDS(BN_, CC_)                                                                    \
+  hipLaunchKernelGGL((k_conv3x3_fwd_lds<2, BN_, CC_>), grid, dim3(256), 0, s, X, N, H, W, Cin, Wt, \
                      bias, Cout, relu, Y)
     if (Cout <= 32) {
       if (Cin == 8) HUV_CONV_LDS(1, 8);
