@@ -139,6 +139,10 @@ SIGNATURES = {
     "huv_socs_images_host": (C.c_int, [P, C.c_int, C.c_int, PF, C.c_int, C.POINTER(P), PF,
                                        C.POINTER(huv_socs_opts)]),
     "huv_band_spectrum": (C.c_int, [P, C.c_int, C.c_int, C.c_int, PF, PF, C.c_int]),
+    "huv_band_spectrum_rects": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_int, PI64, PD, PD,
+                                          PF]),
+    "huv_socs_images_rects": (C.c_int, [P, C.c_int, C.c_int, C.c_double, PI64, PD, PD, C.c_int,
+                                        C.POINTER(P), PF, C.POINTER(huv_socs_opts)]),
     "huv_socs_images_batch": (C.c_int, [P, C.c_int, C.POINTER(huv_clip_group),
                                         C.POINTER(huv_socs_opts)]),
     "huv_ctx_set_autotune": (C.c_int, [P, C.c_int]),

@@ -25,7 +25,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 
 HOST_SRC = ["host/optics_source.cpp", "host/linalg.cpp", "host/kernels_build.cpp"]
-HIP_SRC = ["hip/socs_kernels.hip", "hip/ilt_kernels.hip", "hip/unet_kernels.hip", "capi.cpp",
+HIP_SRC = ["hip/socs_kernels.hip", "hip/ilt_kernels.hip", "hip/unet_kernels.hip",
+           "hip/mask_kernels.hip", "capi.cpp",
            "capi_ilt.cpp", "capi_unet.cpp"]
 
 

@@ -14,6 +14,7 @@
 
 #include "../../include/huv.h"
 #include "capi_internal.hpp"
+#include "hip/mask_kernels.hpp"
 #include "hip/socs_kernels.hpp"
 #include "host/engine.hpp"
 
@@ -245,6 +246,91 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* mas
   return HUV_OK;
 }
 
+
+// Bounds on one call's rectangle geometry: the edge grid of one clip (as in
+// mask.rs:95, MAX_CELLS) and the host staging of all clips.
+constexpr int64_t kRectMaxGrid = 1 << 20;
+constexpr int64_t kRectMaxCells = 1ll << 26;
+
+huv_status validate_rects(int nclips, int n, const RectBatch& rb) {
+  if (!rb.offsets || !rb.background) return fail(HUV_INVALID_PARAMETER, "null rectangle argument");
+  if (!(rb.pixel_nm > 0.0) || !std::isfinite(rb.pixel_nm) || n < 1)
+    return fail(HUV_INVALID_PARAMETER, "need pixel_nm > 0 and n >= 1");
+  if (rb.offsets[0] != 0) return fail(HUV_INVALID_PARAMETER, "rectangle offsets must start at 0");
+  for (int i = 0; i < nclips; ++i)
+    if (rb.offsets[i + 1] < rb.offsets[i])
+      return fail(HUV_INVALID_PARAMETER, "rectangle offsets must be non-decreasing");
+  if (rb.offsets[nclips] > 0 && !rb.rects) return fail(HUV_INVALID_PARAMETER, "null rectangles");
+  for (int64_t r = 0; r < rb.offsets[nclips]; ++r)
+    for (int k = 0; k < 6; ++k)
+      if (!std::isfinite(rb.rects[6 * r + k]))
+        return fail(HUV_INVALID_PARAMETER, "rectangle values must be finite");
+  for (int i = 0; i < 2 * nclips; ++i)
+    if (!std::isfinite(rb.background[i]))
+      return fail(HUV_INVALID_PARAMETER, "background values must be finite");
+  return HUV_OK;
+}
+
+huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectBatch& rb,
+                           float2* Mq, float2* Mp) {
+  // Host plan: disjoint cells per clip (painter's rule), staged with the cell
+  // offsets and backgrounds in one pinned upload.
+  std::vector<double> cells;
+  std::vector<int64_t> coff((size_t)nclips + 1, 0);
+  std::vector<huv::RectFeature> fs;
+  std::vector<huv::RectCell> cl;
+  for (int i = 0; i < nclips; ++i) {
+    fs.clear();
+    for (int64_t r = rb.offsets[i]; r < rb.offsets[i + 1]; ++r) {
+      const double* q = rb.rects + 6 * r;
+      fs.push_back({q[0], q[1], q[2], q[3], huv::cd(q[4], q[5])});
+    }
+    const huv::cd bg(rb.background[2 * i], rb.background[2 * i + 1]);
+    if (!huv::resolve_rect_cells(fs, bg, n, rb.pixel_nm, kRectMaxGrid, cl))
+      return fail(HUV_INVALID_PARAMETER,
+                  "clip " + std::to_string(i) + ": more than 2^20 edge-grid cells; rasterise it instead");
+    for (const auto& e : cl) cells.insert(cells.end(), {e.x0, e.x1, e.y0, e.y1, e.w.real(), e.w.imag()});
+    coff[i + 1] = (int64_t)(cells.size() / 6);
+    if (coff[i + 1] > kRectMaxCells) return fail(HUV_INVALID_PARAMETER, "too many rectangle cells in one call");
+  }
+  const int64_t ncells = coff[nclips];
+  const size_t cell_b = cells.size() * sizeof(double);
+  const size_t off_b = coff.size() * sizeof(int64_t);
+  const size_t bg_b = (size_t)nclips * 2 * sizeof(double);
+  const size_t bytes = cell_b + off_b + bg_b;
+  hipError_t e;
+  // the staging area may still feed an earlier copy on the stream
+  e = hipStreamSynchronize(c->stream());
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  if (!c->staging(bytes, e)) return hip_fail(e, "staging buffer");
+  if (cell_b) std::memcpy(c->pin_h, cells.data(), cell_b);
+  std::memcpy(c->pin_h + cell_b, coff.data(), off_b);
+  std::memcpy(c->pin_h + cell_b + off_b, rb.background, bg_b);
+  e = hipMemcpyAsync(c->pin_d, c->pin_h, bytes, hipMemcpyHostToDevice, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "rectangle upload");
+  const double* d_cells = (const double*)c->pin_d;
+  const int64_t* d_coff = (const int64_t*)(c->pin_d + cell_b);
+  const double* d_bg = (const double*)(c->pin_d + cell_b + off_b);
+  const int nk = 2 * B + 1;
+  double2* X = nullptr;
+  double2* Y = nullptr;
+  if (ncells > 0) {
+    X = (double2*)c->ws.get(Workspace::RECT_TAB, (size_t)ncells * nk * 2 * sizeof(double2), e);
+    if (!X) return hip_fail(e, "workspace");
+    Y = X + (size_t)ncells * nk;
+  }
+  const double L = (double)n * rb.pixel_nm;
+  hipEvent_t ev;
+  c->stage_begin(ST_ROWS_BAND, &ev);
+  e = huvk::launch_rect_tables(d_cells, ncells, n, L, B, X, Y, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "rect_tables launch");
+  c->stage_end(ST_ROWS_BAND, ev);
+  c->stage_begin(ST_COLS_BAND, &ev);
+  e = huvk::launch_rect_band(X, Y, d_coff, d_bg, nclips, n, B, BB, Mq, Mp, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "rect_band launch");
+  c->stage_end(ST_COLS_BAND, ev);
+  return HUV_OK;
+}
 }  // namespace huvi
 
 using namespace huvi;
@@ -688,7 +774,7 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
 // Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
                         huv_kset* const* planes, float* out, bool pt, float2** Mq_out,
-                        const huv_socs_tuning* tune) {
+                        const huv_socs_tuning* tune, const RectBatch* rects) {
   const int B = planes[0]->B;
   const int m = tune && tune->m ? tune->m : planes[0]->m;
   const int BB = huvk::coarse_band(m);
@@ -710,7 +796,8 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
     Mp = (float2*)c->ws.get(Workspace::BAND_PAIRED, (size_t)nclips * (W + 1) * W * sizeof(float2), e);
     if (!Mp) return hip_fail(e, "workspace");
   }
-  huv_status st = band_stage(c, nclips, n, B, BB, masks, Mq, pt, n1, Mp);
+  huv_status st = rects ? rect_band_stage(c, nclips, n, B, BB, *rects, Mq, Mp)
+                        : band_stage(c, nclips, n, B, BB, masks, Mq, pt, n1, Mp);
   if (st != HUV_OK) return st;
   const float2* kern;
   const float* lam;
@@ -978,6 +1065,28 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
   return socs_groups(c, 1, &g, opts, 1);
 }
 
+huv_status huv_socs_images_rects(huv_ctx* c, int nclips, int n, double pixel_nm,
+                                 const int64_t* offsets, const double* rects,
+                                 const double* background, int nplanes, huv_kset* const* planes,
+                                 float* out, const huv_socs_opts* opts) {
+  (void)opts;  // the spectrum is exact (no pixel transfer); no graph capture
+  if (!c || !planes || !out || !offsets || !background)
+    return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (nclips < 0 || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0 and nplanes >= 1");
+  huv_status st = validate_planes(c, n, nplanes, planes);
+  if (st != HUV_OK) return st;
+  const RectBatch rb{pixel_nm, offsets, rects, background};
+  st = validate_rects(nclips, n, rb);
+  if (st != HUV_OK) return st;
+  if (nclips == 0) return HUV_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  // a launch shape tuned for these planes applies (its band-FFT factor is unused)
+  auto it = c->tuned.find(tuning_key(n, planes, nplanes));
+  return socs_forward(c, nclips, n, nullptr, nplanes, planes, out, false, nullptr,
+                      it != c->tuned.end() ? &it->second : nullptr, &rb);
+}
+
 huv_status huv_socs_images_batch(huv_ctx* c, int ngroups, const huv_clip_group* groups,
                                  const huv_socs_opts* opts) {
   if (!c || (ngroups > 0 && !groups)) return fail(HUV_INVALID_PARAMETER, "null argument");
@@ -1074,6 +1183,21 @@ huv_status huv_band_spectrum(huv_ctx* c, int nclips, int n, int band, const floa
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
   return band_stage(c, nclips, n, band, band, masks, (float2*)out, pixel_transfer != 0);
+}
+
+huv_status huv_band_spectrum_rects(huv_ctx* c, int nclips, int n, double pixel_nm, int band,
+                                   const int64_t* offsets, const double* rects,
+                                   const double* background, float* out) {
+  if (!c || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (nclips < 0) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0");
+  if (band < 0 || 2 * band + 1 > n) return fail(HUV_INVALID_PARAMETER, "band out of range");
+  const RectBatch rb{pixel_nm, offsets, rects, background};
+  huv_status st = validate_rects(nclips, n, rb);
+  if (st != HUV_OK) return st;
+  if (nclips == 0) return HUV_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return rect_band_stage(c, nclips, n, band, band, rb, (float2*)out, nullptr);
 }
 
 }  // extern "C"

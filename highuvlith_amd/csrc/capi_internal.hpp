@@ -19,7 +19,7 @@ huv_status hip_fail(hipError_t e, const char* what);
 // Device scratch owned by a context, grown on demand and reused across calls.
 struct Workspace {
   enum Slot { TMP_ROWS = 0, BAND, BAND_PAIRED, COARSE, TMP_COARSE, IBAND, ILT0, ILT1, ILT2, ILT3,
-              ILT4, ILT5, ILT6, ILT7, ILT8, kSlots };
+              ILT4, ILT5, ILT6, ILT7, ILT8, RECT_TAB, kSlots };
   void* ptr[kSlots] = {};
   size_t size[kSlots] = {};
   uint64_t generation = 0;  // bumped whenever a slot is reallocated
@@ -89,6 +89,20 @@ enum Stage { ST_ROWS_BAND = 0, ST_COLS_BAND, ST_FIELDS, ST_COARSE_ROWS, ST_COARS
 huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* masks, float2* Mq,
                       bool pixel_transfer, int n1 = 0, float2* Mp = nullptr);
 
+// Rectangle masks of a batch (host arrays): clip i holds rectangles
+// [offsets[i], offsets[i + 1]) of rects (x, y, w, h, re, im; centre and size
+// in nm, later ones win) over background (re, im) background[2 i ..].
+struct RectBatch {
+  double pixel_nm;
+  const int64_t* offsets;
+  const double* rects;
+  const double* background;
+};
+// Exact band spectrum of rectangle masks into Mq (and Mp), same layout as band_stage.
+huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectBatch& rb,
+                           float2* Mq, float2* Mp);
+huv_status validate_rects(int nclips, int n, const RectBatch& rb);
+
 huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes);
 huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
                        const float2* const** dkern, const float* const** dlam);
@@ -97,10 +111,12 @@ huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int**
 huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool paired,
                       const float2** kern, const float** lam, int* Kmax);
 // Forward images; tune = NULL uses the kernel sets' own coarse grid and the
-// default launch shapes (the ILT adjoint relies on that band layout).
+// default launch shapes (the ILT adjoint relies on that band layout).  With
+// `rects` the band spectrum is the exact one of those rectangle masks and
+// `masks` is not read.
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
                         huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out,
-                        const huv_socs_tuning* tune = nullptr);
+                        const huv_socs_tuning* tune = nullptr, const RectBatch* rects = nullptr);
 
 }  // namespace huvi
 

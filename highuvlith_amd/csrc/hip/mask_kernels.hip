@@ -1,0 +1,115 @@
+// Exact mask spectrum of rectangle masks, evaluated on the device.
+//
+// The spectrum convention is the one of mask.rs:31-45 (crates/highuvlith-core):
+// for the field [-L/2, L/2)^2, L = N p, and the signed order m,
+//   S[m] = N^2 c(m) e^{-i pi (mx + my)(N - 1)/N},
+//   c(m) = (1/L^2) \iint t(x, y) e^{-2 pi i (mx x + my y)/L} dx dy,
+// which equals FFT_N(raster) in the fine-pixel limit.  A rectangle cell
+// [x0, x1) x [y0, y1) of weight w contributes w X(mx) Y(my) with
+//   X(m) = N (wx/L) sinc(pi m wx/L) e^{-i pi m ((x0 + x1)/L + (N - 1)/N)}
+// (mask.rs:1426-1477, axis_coefficients for a finite interval), and the
+// background b adds b N^2 at m = 0 (mask.rs:633).  Axis tables are computed
+// once per cell in FP64; the band box sums w X Y over the cells of a clip in
+// cell order (FP64 accumulation, one FP32 rounding at the store).
+#include <hip/hip_runtime.h>
+
+#include "mask_kernels.hpp"
+
+namespace huvk {
+
+// N (w/L) sinc(pi m w/L) e^{-i pi m ((a + b)/L + (N - 1)/N)}
+__device__ __forceinline__ double2 axis_coeff(double a, double b, int m, int n, double L) {
+  const double w = b - a;
+  const double u = (double)m * w / L;
+  const double sinc = m == 0 ? 1.0 : sinpi(u) / (M_PI * u);
+  const double amp = (double)n * (w / L) * sinc;
+  // phase / (-pi), reduced to [-1, 1] before the sine and cosine
+  double t = (double)m * ((a + b) / L + (double)(n - 1) / (double)n);
+  t -= 2.0 * rint(0.5 * t);
+  double sv, cv;
+  sincospi(-t, &sv, &cv);
+  return make_double2(amp * cv, amp * sv);
+}
+
+__global__ __launch_bounds__(256) void k_rect_tables(const double* __restrict__ cells,
+                                                     int64_t ncells, int n, double L, int B,
+                                                     double2* __restrict__ X,
+                                                     double2* __restrict__ Y) {
+  const int nk = 2 * B + 1;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncells * nk) return;
+  const int64_t c = idx / nk;
+  const int m = (int)(idx - c * nk) - B;
+  const double* p = cells + 6 * c;
+  const double2 xa = axis_coeff(p[0], p[1], m, n, L);
+  const double2 ya = axis_coeff(p[2], p[3], m, n, L);
+  const double wr = p[4], wi = p[5];
+  X[idx] = make_double2(wr * xa.x - wi * xa.y, wr * xa.y + wi * xa.x);
+  Y[idx] = ya;
+}
+
+// One thread per box bin (column kx + BB, row ky + BB; rows fastest);
+// work-groups [clip * bpc, (clip + 1) * bpc) cover one clip's box.
+__global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X,
+                                                   const double2* __restrict__ Y,
+                                                   const int64_t* __restrict__ coff,
+                                                   const double* __restrict__ bg, int n, int B,
+                                                   int BB, int bpc, float2* __restrict__ Mq,
+                                                   float2* __restrict__ Mp) {
+  const int W = 2 * BB + 1, nk = 2 * B + 1;
+  const int clip = blockIdx.x / bpc;
+  const int b = (blockIdx.x - clip * bpc) * blockDim.x + threadIdx.x;
+  if (b >= W * W) return;
+  const int col = b / W, row = b - col * W;
+  const int kx = col - BB, ky = row - BB;
+  double re = 0.0, im = 0.0;
+  if (kx >= -B && kx <= B && ky >= -B && ky <= B) {
+    const int64_t c0 = coff[clip], c1 = coff[clip + 1];
+    for (int64_t c = c0; c < c1; ++c) {
+      const double2 x = X[c * nk + kx + B], y = Y[c * nk + ky + B];
+      re = fma(x.x, y.x, re);
+      re = fma(-x.y, y.y, re);
+      im = fma(x.x, y.y, im);
+      im = fma(x.y, y.x, im);
+    }
+    if (kx == 0 && ky == 0) {
+      const double nn = (double)n * (double)n;
+      re += bg[2 * clip] * nn;
+      im += bg[2 * clip + 1] * nn;
+    }
+  }
+  const float2 v = make_float2((float)re, (float)im);
+  Mq[(size_t)clip * W * W + (size_t)col * W + row] = v;
+  if (Mp) {
+    // column-paired copy: (col, row) at [(col + 1) / 2][row][(col + 1) % 2], column -1 zero
+    float2* mp = Mp + (size_t)clip * (W + 1) * W;
+    mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
+    if (col == 0) mp[(size_t)row * 2] = make_float2(0.f, 0.f);
+  }
+}
+
+hipError_t launch_rect_tables(const double* cells, int64_t ncells, int n, double L, int B,
+                              double2* X, double2* Y, hipStream_t s) {
+  if (ncells <= 0) return hipSuccess;
+  const int64_t total = ncells * (2 * B + 1);
+  const int64_t blocks = (total + 255) / 256;
+  if (blocks >= (1ll << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_rect_tables, dim3((unsigned)blocks), dim3(256), 0, s, cells, ncells, n, L,
+                     B, X, Y);
+  return hipGetLastError();
+}
+
+hipError_t launch_rect_band(const double2* X, const double2* Y, const int64_t* coff,
+                            const double* bg, int nclips, int n, int B, int BB, float2* Mq,
+                            float2* Mp, hipStream_t s) {
+  if (nclips <= 0) return hipSuccess;
+  if (B < 0 || BB < B || BB > 1024) return hipErrorInvalidValue;
+  const int W = 2 * BB + 1;
+  const int bpc = (W * W + 255) / 256;
+  if ((int64_t)bpc * nclips >= (1ll << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_rect_band, dim3((unsigned)(bpc * nclips)), dim3(256), 0, s, X, Y, coff,
+                     bg, n, B, BB, bpc, Mq, Mp);
+  return hipGetLastError();
+}
+
+}  // namespace huvk

@@ -163,4 +163,14 @@ struct RectFeature {
 void rasterize_rects(const std::vector<RectFeature>& rects, cd background, int n,
                      double pixel_nm, std::vector<cd>& out);
 
+// A disjoint rectangle [x0, x1) x [y0, y1) (nm) of weight w = value - background.
+struct RectCell {
+  double x0, x1, y0, y1;
+  cd w;
+};
+// Painter's-rule resolution of rectangles into disjoint cells (see the
+// definition); false when the edge grid would exceed max_cells cells.
+bool resolve_rect_cells(const std::vector<RectFeature>& rects, cd background, int n,
+                        double pixel_nm, int64_t max_cells, std::vector<RectCell>& out);
+
 }  // namespace huv

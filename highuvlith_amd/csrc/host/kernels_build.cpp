@@ -351,4 +351,60 @@ void rasterize_rects(const std::vector<RectFeature>& rects, cd bg, int n, double
   }
 }
 
+// Disjoint cells of painted rectangles (mask.rs:1365-1424, compress): the
+// rectangles are clipped to the field, the field and rectangle edges are
+// merged within 1e-12 L, each rectangle paints its cells in order (the later
+// one wins), runs of equal value along x become one cell and consecutive
+// rows with identical content one row.  Cells equal to the background are
+// dropped; each cell keeps value - background.
+bool resolve_rect_cells(const std::vector<RectFeature>& rects, cd bg, int n, double pixel,
+                        int64_t max_cells, std::vector<RectCell>& out) {
+  out.clear();
+  const double half = n * pixel / 2.0;
+  const double eps = 1e-12 * (2.0 * half);
+  struct R { double x0, x1, y0, y1; cd v; };
+  std::vector<R> rs;
+  for (const auto& f : rects) {
+    if (!(f.w > 0 && f.h > 0)) continue;
+    const double x0 = std::max(f.x - f.w / 2, -half), x1 = std::min(f.x + f.w / 2, half);
+    const double y0 = std::max(f.y - f.h / 2, -half), y1 = std::min(f.y + f.h / 2, half);
+    if (x1 - x0 <= 0 || y1 - y0 <= 0) continue;
+    rs.push_back({x0, x1, y0, y1, f.value});
+  }
+  if (rs.empty()) return true;
+  std::vector<double> xe{-half, half}, ye{-half, half};
+  for (auto& r : rs) {
+    xe.push_back(r.x0); xe.push_back(r.x1);
+    ye.push_back(r.y0); ye.push_back(r.y1);
+  }
+  dedup(xe, eps);
+  dedup(ye, eps);
+  const int64_t nx = (int64_t)xe.size() - 1, ny = (int64_t)ye.size() - 1;
+  if (nx < 1 || ny < 1 || nx > max_cells / ny) return false;
+  std::vector<cd> cells((size_t)(nx * ny), bg);
+  for (auto& r : rs) {
+    const int64_t i0 = edge_index(ye, r.y0, eps), i1 = edge_index(ye, r.y1, eps);
+    const int64_t j0 = edge_index(xe, r.x0, eps), j1 = edge_index(xe, r.x1, eps);
+    for (int64_t i = i0; i < std::min(i1, ny); ++i)
+      for (int64_t j = j0; j < std::min(j1, nx); ++j) cells[i * nx + j] = r.v;
+  }
+  int64_t i = 0;
+  while (i < ny) {
+    int64_t k = i + 1;
+    while (k < ny && std::equal(cells.begin() + k * nx, cells.begin() + (k + 1) * nx,
+                                cells.begin() + i * nx))
+      ++k;
+    int64_t j = 0;
+    while (j < nx) {
+      const cd v = cells[i * nx + j];
+      int64_t e = j + 1;
+      while (e < nx && cells[i * nx + e] == v) ++e;
+      if (v != bg) out.push_back({xe[j], xe[e], ye[i], ye[k], v - bg});
+      j = e;
+    }
+    i = k;
+  }
+  return true;
+}
+
 }  // namespace huv

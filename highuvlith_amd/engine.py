@@ -437,6 +437,44 @@ def band_spectrum_device(ctx, masks, band, pixel_transfer=False):
     return torch.view_as_complex(out)
 
 
+def _rect_batch(clips, backgrounds):
+    """Host arrays of a batch of rectangle masks: clips = [[(x, y, w, h, value), ...], ...]
+    (centre and size in nm, later rectangles win), backgrounds = one value or one per clip."""
+    nclips = len(clips)
+    if np.ndim(backgrounds) == 0:
+        backgrounds = [backgrounds] * nclips
+    if len(backgrounds) != nclips:
+        raise HuvError(2, "one background per clip")
+    offsets = np.zeros(nclips + 1, np.int64)
+    for i, c in enumerate(clips):
+        offsets[i + 1] = offsets[i] + len(c)
+    rects = np.zeros((max(int(offsets[-1]), 1), 6))
+    r = 0
+    for c in clips:
+        for (x, y, w, h, v) in c:
+            v = complex(v)
+            rects[r] = (x, y, w, h, v.real, v.imag)
+            r += 1
+    bg = np.array([[complex(b).real, complex(b).imag] for b in backgrounds],
+                  np.float64).reshape(nclips, 2)
+    return offsets, rects, np.ascontiguousarray(bg)
+
+
+def band_spectrum_rects_device(ctx, clips, grid, band, backgrounds=0.0):
+    """Exact band spectrum S[kx+B][ky+B] of rectangle masks (torch complex64 (N, W, W)),
+    in the layout and scale of band_spectrum_device (mask.rs:616-657 analytic path)."""
+    import torch
+    offsets, rects, bg = _rect_batch(clips, backgrounds)
+    nclips, W = len(clips), 2 * band + 1
+    out = torch.empty((nclips, W, W, 2), dtype=torch.float32, device="cuda")
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    check(ctx.lib.huv_band_spectrum_rects(
+        ctx.handle, nclips, grid.size, grid.pixel_nm, band,
+        i64ptr(offsets), dptr(rects), dptr(bg),
+        C.c_void_p(out.data_ptr())))
+    return torch.view_as_complex(out)
+
+
 # ---------------------------------------------------------------------------
 # Engine
 # ---------------------------------------------------------------------------
@@ -569,6 +607,30 @@ class AerialImageEngine:
             out = torch.empty((masks.shape[0], len(planes), n, n), dtype=torch.float32,
                               device=masks.device)
         return socs_images_device(self.context(), masks, planes, out, pixel_transfer, graph)
+
+    def compute_rects(self, clips, defocus_nm=(0.0,), backgrounds=0.0, out=None):
+        """Images of rectangle masks from their exact spectrum (compute(mask),
+        aerial.rs:1775-1777, for masks of rectangles): clips = [[(x, y, w, h, value),
+        ...], ...] in nm, later rectangles win; backgrounds = one amplitude or one
+        per clip (0 = dark field).  Returns a torch tensor (N, P, n, n)."""
+        import torch
+        n = self.grid.size
+        planes = [self.device_kernel_set(z) for z in defocus_nm]
+        ctx = self.context()
+        offsets, rects, bg = _rect_batch(clips, backgrounds)
+        if out is None:
+            out = torch.empty((len(clips), len(planes), n, n), dtype=torch.float32,
+                              device=f"cuda:{self.device}")
+        assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
+        if out.shape != (len(clips), len(planes), n, n):
+            raise HuvError(2, f"output shape {tuple(out.shape)} != {(len(clips), len(planes), n, n)}")
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
+        check(ctx.lib.huv_socs_images_rects(
+            ctx.handle, len(clips), n, self.grid.pixel_nm,
+            i64ptr(offsets), dptr(rects), dptr(bg), len(planes),
+            arr, C.c_void_p(out.data_ptr()), None))
+        return out
 
     def compute_from_transmittance(self, transmittance, defocus_nm=0.0):
         t = np.asarray(transmittance)

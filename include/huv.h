@@ -236,6 +236,33 @@ huv_status huv_socs_images_host(huv_ctx* ctx, int nclips, int n, const float* ma
 huv_status huv_band_spectrum(huv_ctx* ctx, int nclips, int n, int band, const float* masks,
                              float* out, int pixel_transfer);
 
+/* Rectangle masks (host arrays).  Clip i is the rectangles
+ * [offsets[i], offsets[i + 1]) of `rects`, each (x, y, w, h, re, im): centre
+ * and size in nm at wafer scale, complex amplitude inside; later rectangles
+ * win where they overlap (painter's rule); background[2 i], background[2 i + 1]
+ * is the clip's amplitude elsewhere.  Geometry outside the field
+ * [-n pixel_nm / 2, n pixel_nm / 2)^2 is clipped away.  A clip whose edge grid
+ * exceeds 2^20 cells is rejected (rasterise it instead). */
+
+/* Exact band spectrum of rectangle masks (replaces Mask::spectrum's analytic
+ * path for rectangles, mask.rs:616-657 with compress / axis_coefficients /
+ * add_term_spectrum at mask.rs:1365-1502): S[k] = n^2 c(m) e^{-i pi (mx+my)(n-1)/n}
+ * for |kx|, |ky| <= band, FP64 evaluation; out: DEVICE, nclips x (2B+1) x (2B+1)
+ * interleaved complex FP32 in huv_band_spectrum's layout [kx+B][ky+B].
+ * Synchronises the context stream before staging the geometry. */
+huv_status huv_band_spectrum_rects(huv_ctx* ctx, int nclips, int n, double pixel_nm, int band,
+                                   const int64_t* offsets, const double* rects,
+                                   const double* background, float* out);
+/* Aerial images of rectangle masks from their exact spectrum (replaces
+ * AerialImageEngine::compute(mask, defocus), aerial.rs:1775-1777, for
+ * rectangle masks: mask.spectrum then compute_from_spectrum).  planes and out
+ * as in huv_socs_images; opts is not used (no pixel transfer applies to an
+ * exact spectrum, and the per-call geometry upload is not captured in graphs). */
+huv_status huv_socs_images_rects(huv_ctx* ctx, int nclips, int n, double pixel_nm,
+                                 const int64_t* offsets, const double* rects,
+                                 const double* background, int nplanes, huv_kset* const* planes,
+                                 float* out, const huv_socs_opts* opts);
+
 /* ======================================================================== */
 /* ILT on the device (ilt.rs)                                                */
 /* ======================================================================== */
