@@ -8,8 +8,14 @@ sectors sigma 0.7-0.9 with a 35 deg opening, 5 focus planes
 (-60/-30/0/+30/+60 nm), K = 32 kernels per plane built on the host in FP64
 (2560 images per step).
 
-Also reported in the same JSON line (each skipped with --no-uneven,
---no-ilt, --no-warm):
+Also reported in the same JSON line (each skipped with --no-rects,
+--no-uneven, --no-ilt, --no-warm):
+  rects    the headline optics and focus planes on masks given as rectangle
+           lists (512 clips of 8 x 8 contacts with sub-pixel size, pitch and
+           offset, 64 disjoint cells each): exact band spectrum in FP64 on the
+           device, then the same field/coarse/image kernels; Mpix/s including
+           the host cell plan and upload, and the rect_band kernel against
+           the FP64 vector peak (8 flops per cell and band bin).
   uneven   1000 clips with edges {256, 512, 1024, 2048} px (p .4/.3/.2/.1),
            30-60 % clear, ArF 193i annular 0.7/0.9 at 2 nm, K = 32, one graph
            launch per batch, autotuned launch shapes; aggregate Mpix/s and
@@ -205,6 +211,61 @@ def bench_fixed(args, H, W, np, torch):
     return result
 
 
+def bench_rects(args, H, W, np, torch):
+    src, opt, grid, K = W.euv_dipole_1024_setup()
+    n = grid.size
+    eng = H.AerialImageEngine(src, opt, grid, max_kernels=K)
+    planes = [eng.device_kernel_set(z) for z in W.EUV_FOCUS_PLANES_NM]
+    B, P = planes[0].band, len(planes)
+    nclips = args.clips
+    rng = np.random.default_rng(77)
+    L = n * grid.pixel_nm
+    clips = []
+    for _ in range(nclips):
+        pitch = rng.uniform(0.08, 0.11) * L
+        size = rng.uniform(0.3, 0.6) * pitch
+        x0, y0 = rng.uniform(-0.5, 0.5, 2) * pitch
+        clips.append([(x0 + (i - 3.5) * pitch, y0 + (j - 3.5) * pitch, size, size, 1.0)
+                      for j in range(8) for i in range(8)])
+    cells = 64 * nclips
+    arrays = H.engine.rect_arrays(clips, 0.0)
+    ctx = eng.context()
+    out = torch.empty((nclips, P, n, n), dtype=torch.float32, device="cuda")
+
+    def step():
+        H.engine.socs_images_rects_device(ctx, grid, arrays, planes, out)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    ctx.set_profiling(True)
+    ctx.stage_times(reset=True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    stages = ctx.stage_times(reset=True)
+    ctx.set_profiling(False)
+    stage_ms = {k: v[0] / max(v[1], 1) for k, v in stages.items() if v[1] > 0}
+    tab_ms, band_ms = stage_ms["rows_band"], stage_ms["cols_band"]
+    flops = 8.0 * cells * (2 * B + 1) ** 2
+    tf = flops / (band_ms * 1e-3) / 1e12
+    return {"workload": f"{nclips} clips of 8x8 contacts (sub-pixel), 1024x1024 @ 1 nm, EUV "
+                        f"x-dipole, {P} focus planes, K={K}, exact rectangle spectrum",
+            "value": round(nclips * P * n * n / 1e6 / dt, 2), "unit": "Mpix/s",
+            "ms_per_batch": round(dt * 1e3, 4),
+            "rect_tables_ms": round(tab_ms, 4), "rect_band_ms": round(band_ms, 4),
+            "rect_band_cells": cells, "band": B,
+            "rect_band_roofline": {"bound": "fp64 vector", "achieved": round(tf, 3),
+                                   "peak": 78.6, "unit": "TFLOP/s",
+                                   "frac": round(tf / 78.6, 4)},
+            "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()}}
+
+
 def bench_uneven(args, H, W, np, torch):
     edges = W.uneven_edges(args.uneven_clips, seed=3)
     ctx = H.DeviceContext(0)
@@ -335,6 +396,7 @@ def main():
     ap.add_argument("--cpu-clips", type=int, default=2,
                     help="clips in the CPU-baseline sample (x 5 planes)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rects", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-ilt", action="store_true")
     ap.add_argument("--no-warm", action="store_true")
@@ -362,7 +424,8 @@ def main():
 
     result = bench_fixed(args, H, W, np, torch)
     print("fixed batch done", file=sys.stderr, flush=True)
-    for name, fn, skip in (("uneven", bench_uneven, args.no_uneven),
+    for name, fn, skip in (("rects", bench_rects, args.no_rects),
+                           ("uneven", bench_uneven, args.no_uneven),
                            ("ilt", bench_ilt, args.no_ilt),
                            ("warm", bench_warm, args.no_warm)):
         if skip:

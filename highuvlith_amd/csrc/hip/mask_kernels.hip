@@ -48,8 +48,11 @@ __global__ __launch_bounds__(256) void k_rect_tables(const double* __restrict__ 
   Y[idx] = ya;
 }
 
-// One thread per box bin (column kx + BB, row ky + BB; rows fastest);
-// work-groups [clip * bpc, (clip + 1) * bpc) cover one clip's box.
+// One thread per (row ky + BB, group of kRectCols consecutive columns kx + BB),
+// rows fastest, so each cell's row coefficient is loaded once for kRectCols
+// products; work-groups [clip * bpc, (clip + 1) * bpc) cover one clip's box.
+constexpr int kRectCols = 4;
+
 __global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X,
                                                    const double2* __restrict__ Y,
                                                    const int64_t* __restrict__ coff,
@@ -57,34 +60,59 @@ __global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X
                                                    int BB, int bpc, float2* __restrict__ Mq,
                                                    float2* __restrict__ Mp) {
   const int W = 2 * BB + 1, nk = 2 * B + 1;
+  const int G = (W + kRectCols - 1) / kRectCols;
   const int clip = blockIdx.x / bpc;
-  const int b = (blockIdx.x - clip * bpc) * blockDim.x + threadIdx.x;
-  if (b >= W * W) return;
-  const int col = b / W, row = b - col * W;
-  const int kx = col - BB, ky = row - BB;
-  double re = 0.0, im = 0.0;
-  if (kx >= -B && kx <= B && ky >= -B && ky <= B) {
+  const int t = (blockIdx.x - clip * bpc) * blockDim.x + threadIdx.x;
+  if (t >= W * G) return;
+  const int g = t / W, row = t - g * W;
+  const int ky = row - BB;
+  const bool row_live = ky >= -B && ky <= B;
+  int xi[kRectCols];
+  bool live[kRectCols];
+#pragma unroll
+  for (int j = 0; j < kRectCols; ++j) {
+    const int kx = kRectCols * g + j - BB;
+    live[j] = row_live && kx >= -B && kx <= B && kRectCols * g + j < W;
+    xi[j] = min(max(kx + B, 0), nk - 1);  // clamped: dead columns compute and are discarded
+  }
+  double re[kRectCols], im[kRectCols];
+#pragma unroll
+  for (int j = 0; j < kRectCols; ++j) re[j] = im[j] = 0.0;
+  if (row_live) {
     const int64_t c0 = coff[clip], c1 = coff[clip + 1];
+    const int yi = ky + B;
     for (int64_t c = c0; c < c1; ++c) {
-      const double2 x = X[c * nk + kx + B], y = Y[c * nk + ky + B];
-      re = fma(x.x, y.x, re);
-      re = fma(-x.y, y.y, re);
-      im = fma(x.x, y.y, im);
-      im = fma(x.y, y.x, im);
-    }
-    if (kx == 0 && ky == 0) {
-      const double nn = (double)n * (double)n;
-      re += bg[2 * clip] * nn;
-      im += bg[2 * clip + 1] * nn;
+      const double2 y = Y[c * nk + yi];
+      const double2* xr = X + c * nk;
+#pragma unroll
+      for (int j = 0; j < kRectCols; ++j) {
+        const double2 x = xr[xi[j]];
+        re[j] = fma(x.x, y.x, re[j]);
+        re[j] = fma(-x.y, y.y, re[j]);
+        im[j] = fma(x.x, y.y, im[j]);
+        im[j] = fma(x.y, y.x, im[j]);
+      }
     }
   }
-  const float2 v = make_float2((float)re, (float)im);
-  Mq[(size_t)clip * W * W + (size_t)col * W + row] = v;
-  if (Mp) {
-    // column-paired copy: (col, row) at [(col + 1) / 2][row][(col + 1) % 2], column -1 zero
-    float2* mp = Mp + (size_t)clip * (W + 1) * W;
-    mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
-    if (col == 0) mp[(size_t)row * 2] = make_float2(0.f, 0.f);
+  const double nn = (double)n * (double)n;
+  float2* mq = Mq + (size_t)clip * W * W;
+  float2* mp = Mp ? Mp + (size_t)clip * (W + 1) * W : nullptr;
+#pragma unroll
+  for (int j = 0; j < kRectCols; ++j) {
+    const int col = kRectCols * g + j;
+    if (col >= W) break;
+    double vr = live[j] ? re[j] : 0.0, vi = live[j] ? im[j] : 0.0;
+    if (col == BB && ky == 0) {  // kx = ky = 0: the background
+      vr += bg[2 * clip] * nn;
+      vi += bg[2 * clip + 1] * nn;
+    }
+    const float2 v = make_float2((float)vr, (float)vi);
+    mq[(size_t)col * W + row] = v;
+    if (mp) {
+      // column-paired copy: (col, row) at [(col + 1) / 2][row][(col + 1) % 2], column -1 zero
+      mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
+      if (col == 0) mp[(size_t)row * 2] = make_float2(0.f, 0.f);
+    }
   }
 }
 
@@ -105,7 +133,7 @@ hipError_t launch_rect_band(const double2* X, const double2* Y, const int64_t* c
   if (nclips <= 0) return hipSuccess;
   if (B < 0 || BB < B || BB > 1024) return hipErrorInvalidValue;
   const int W = 2 * BB + 1;
-  const int bpc = (W * W + 255) / 256;
+  const int bpc = (W * ((W + kRectCols - 1) / kRectCols) + 255) / 256;
   if ((int64_t)bpc * nclips >= (1ll << 31)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_rect_band, dim3((unsigned)(bpc * nclips)), dim3(256), 0, s, X, Y, coff,
                      bg, n, B, BB, bpc, Mq, Mp);

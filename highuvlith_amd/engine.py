@@ -437,9 +437,10 @@ def band_spectrum_device(ctx, masks, band, pixel_transfer=False):
     return torch.view_as_complex(out)
 
 
-def _rect_batch(clips, backgrounds):
-    """Host arrays of a batch of rectangle masks: clips = [[(x, y, w, h, value), ...], ...]
-    (centre and size in nm, later rectangles win), backgrounds = one value or one per clip."""
+def rect_arrays(clips, backgrounds=0.0):
+    """Host arrays (offsets, rects, background) of a batch of rectangle masks for the C ABI:
+    clips = [[(x, y, w, h, value), ...], ...] (centre and size in nm, later rectangles
+    win), backgrounds = one amplitude or one per clip."""
     nclips = len(clips)
     if np.ndim(backgrounds) == 0:
         backgrounds = [backgrounds] * nclips
@@ -464,7 +465,7 @@ def band_spectrum_rects_device(ctx, clips, grid, band, backgrounds=0.0):
     """Exact band spectrum S[kx+B][ky+B] of rectangle masks (torch complex64 (N, W, W)),
     in the layout and scale of band_spectrum_device (mask.rs:616-657 analytic path)."""
     import torch
-    offsets, rects, bg = _rect_batch(clips, backgrounds)
+    offsets, rects, bg = rect_arrays(clips, backgrounds)
     nclips, W = len(clips), 2 * band + 1
     out = torch.empty((nclips, W, W, 2), dtype=torch.float32, device="cuda")
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
@@ -473,6 +474,23 @@ def band_spectrum_rects_device(ctx, clips, grid, band, backgrounds=0.0):
         i64ptr(offsets), dptr(rects), dptr(bg),
         C.c_void_p(out.data_ptr())))
     return torch.view_as_complex(out)
+
+
+def socs_images_rects_device(ctx, grid, arrays, planes, out):
+    """Images of rectangle masks (arrays = rect_arrays(...)) into out (N, P, n, n) f32 CUDA,
+    on torch's current stream; the call synchronises that stream before staging."""
+    import torch
+    offsets, rects, bg = arrays
+    nclips, n = len(offsets) - 1, grid.size
+    assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
+    if out.shape != (nclips, len(planes), n, n):
+        raise HuvError(2, f"output shape {tuple(out.shape)} != {(nclips, len(planes), n, n)}")
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
+    check(ctx.lib.huv_socs_images_rects(
+        ctx.handle, nclips, n, grid.pixel_nm, i64ptr(offsets), dptr(rects), dptr(bg),
+        len(planes), arr, C.c_void_p(out.data_ptr()), None))
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -611,26 +629,18 @@ class AerialImageEngine:
     def compute_rects(self, clips, defocus_nm=(0.0,), backgrounds=0.0, out=None):
         """Images of rectangle masks from their exact spectrum (compute(mask),
         aerial.rs:1775-1777, for masks of rectangles): clips = [[(x, y, w, h, value),
-        ...], ...] in nm, later rectangles win; backgrounds = one amplitude or one
-        per clip (0 = dark field).  Returns a torch tensor (N, P, n, n)."""
+        ...], ...] in nm, later rectangles win, or the tuple of rect_arrays;
+        backgrounds = one amplitude or one per clip (0 = dark field).  Returns a
+        torch tensor (N, P, n, n)."""
         import torch
         n = self.grid.size
         planes = [self.device_kernel_set(z) for z in defocus_nm]
-        ctx = self.context()
-        offsets, rects, bg = _rect_batch(clips, backgrounds)
+        arrays = clips if isinstance(clips, tuple) else rect_arrays(clips, backgrounds)
+        nclips = len(arrays[0]) - 1
         if out is None:
-            out = torch.empty((len(clips), len(planes), n, n), dtype=torch.float32,
+            out = torch.empty((nclips, len(planes), n, n), dtype=torch.float32,
                               device=f"cuda:{self.device}")
-        assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
-        if out.shape != (len(clips), len(planes), n, n):
-            raise HuvError(2, f"output shape {tuple(out.shape)} != {(len(clips), len(planes), n, n)}")
-        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
-        arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
-        check(ctx.lib.huv_socs_images_rects(
-            ctx.handle, len(clips), n, self.grid.pixel_nm,
-            i64ptr(offsets), dptr(rects), dptr(bg), len(planes),
-            arr, C.c_void_p(out.data_ptr()), None))
-        return out
+        return socs_images_rects_device(self.context(), self.grid, arrays, planes, out)
 
     def compute_from_transmittance(self, transmittance, defocus_nm=0.0):
         t = np.asarray(transmittance)
