@@ -437,6 +437,79 @@ def band_spectrum_device(ctx, masks, band, pixel_transfer=False):
     return torch.view_as_complex(out)
 
 
+_MAX_GRATING_PERIODS = float(1 << 20)  # mask.rs:101
+
+
+def _grating_intervals(width, pitch, offset, L):
+    """Intervals of a grating (lines of `width` every `pitch`, one centred at
+    `offset`) inside the field [-L/2, L/2), and the amplitude fraction
+    (mask.rs grating axes; more than 2^20 periods become their mean)."""
+    half = L / 2.0
+    if not (math.isfinite(width) and math.isfinite(pitch) and math.isfinite(offset)):
+        return [], 1.0
+    if width <= 0.0 or pitch <= 0.0:
+        return [], 1.0
+    if width >= pitch:
+        return [(-half, half)], 1.0
+    if L / pitch > _MAX_GRATING_PERIODS:
+        return [(-half, half)], width / pitch
+    k_lo = math.floor((-half - offset - width / 2.0) / pitch)
+    k_hi = math.ceil((half - offset + width / 2.0) / pitch)
+    out = []
+    for k in range(k_lo, k_hi + 1):
+        c = offset + k * pitch
+        lo, hi = max(c - width / 2.0, -half), min(c + width / 2.0, half)
+        if hi > lo:
+            out.append((lo, hi))
+    return out, 1.0
+
+
+def mask_feature_rects(features, grid, dark_field=False, absorber=0j):
+    """The reference's rectangle-type mask features (mask.rs:120-180) as painted
+    rectangles for rect_arrays / compute_rects: returns (rects, background).
+
+    features: ("rect", x, y, w, h) | ("gray", x, y, w, h, T) |
+    ("lines", cd, pitch, orientation, offset) (orientation 0: lines along y,
+    repeating along x) | ("array", w, h, pitch_x, pitch_y, offset_x, offset_y).
+    A feature is clear (1) on a dark-field mask and the absorber amplitude on a
+    bright-field one, where the background is the other; a gray rectangle has
+    amplitude sqrt(T).  Gratings are expanded into their periods inside the
+    field (exact: the Fourier series of a field-periodic grating is the sum of
+    its clipped periods)."""
+    L = grid.size * grid.pixel_nm
+    bg = complex(absorber) if dark_field else 1.0 + 0j
+    fg = 1.0 + 0j if dark_field else complex(absorber)
+    rects = []
+    for f in features:
+        kind = f[0]
+        if kind == "rect":
+            x, y, w, h = map(float, f[1:5])
+            rects.append((x, y, w, h, fg))
+        elif kind == "gray":
+            x, y, w, h, t = map(float, f[1:6])
+            rects.append((x, y, w, h, math.sqrt(max(t, 0.0))))
+        elif kind == "lines":
+            cd, pitch, orient, offset = float(f[1]), float(f[2]), int(f[3]), float(f[4])
+            ivs, frac = _grating_intervals(cd, pitch, offset, L)
+            v = bg + (fg - bg) * frac
+            for lo, hi in ivs:
+                if orient == 0:
+                    rects.append(((lo + hi) / 2, 0.0, hi - lo, L, v))
+                else:
+                    rects.append((0.0, (lo + hi) / 2, L, hi - lo, v))
+        elif kind == "array":
+            w, h, px, py, ox, oy = map(float, f[1:7])
+            ix, fx = _grating_intervals(w, px, ox, L)
+            iy, fy = _grating_intervals(h, py, oy, L)
+            v = bg + (fg - bg) * (fx * fy)
+            for ylo, yhi in iy:
+                for xlo, xhi in ix:
+                    rects.append(((xlo + xhi) / 2, (ylo + yhi) / 2, xhi - xlo, yhi - ylo, v))
+        else:
+            raise HuvError(2, f"unsupported mask feature {kind!r} for the rectangle path")
+    return rects, bg
+
+
 def rect_arrays(clips, backgrounds=0.0):
     """Host arrays (offsets, rects, background) of a batch of rectangle masks for the C ABI:
     clips = [[(x, y, w, h, value), ...], ...] (centre and size in nm, later rectangles
@@ -641,6 +714,17 @@ class AerialImageEngine:
             out = torch.empty((nclips, len(planes), n, n), dtype=torch.float32,
                               device=f"cuda:{self.device}")
         return socs_images_rects_device(self.context(), self.grid, arrays, planes, out)
+
+    def compute_features(self, masks, defocus_nm=(0.0,), dark_field=False, absorber=0j):
+        """compute(mask) (aerial.rs:1775-1777) for a batch of masks given as lists of
+        rectangle-type features (see mask_feature_rects), from their exact spectrum.
+        Returns a torch tensor (N, P, n, n)."""
+        clips, bgs = [], []
+        for feats in masks:
+            r, b = mask_feature_rects(feats, self.grid, dark_field, absorber)
+            clips.append(r)
+            bgs.append(b)
+        return self.compute_rects(clips, defocus_nm, backgrounds=bgs)
 
     def compute_from_transmittance(self, transmittance, defocus_nm=0.0):
         t = np.asarray(transmittance)

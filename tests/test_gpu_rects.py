@@ -156,3 +156,35 @@ def test_invalid_rectangle_batches_are_rejected(gpu_ctx):
     many = [(float(i) * 0.2 - 100.0, float(i) * 0.2 - 100.0, 0.1, 0.1, 1.0) for i in range(1100)]
     with pytest.raises(H.HuvError):
         H.engine.band_spectrum_rects_device(gpu_ctx, [many], grid, 10)
+
+
+def test_gratings_and_arrays_match_oracle():
+    """Line/space gratings and rectangle arrays (mask.rs periodic primitives),
+    commensurate or not, mixed with rectangles: band and images from the
+    expanded rectangles against the oracle's exact spectrum of the features."""
+    n, px, band = 256, 1.25, 40
+    grid = H.GridConfig(n, px)
+    masks = [[("lines", 37.5, 80.0, 0, 3.3)],
+             [("lines", 22.0, 61.7, 1, -8.0), ("rect", 10.0, 20.0, 55.5, 33.3)],
+             [("array", 14.2, 18.9, 40.0, 64.0, 2.5, -6.0)]]
+    clips, bgs = [], []
+    for f in masks:
+        r, b = H.engine.mask_feature_rects(f, grid, dark_field=True, absorber=0.2j)
+        clips.append(r)
+        bgs.append(b)
+    ctx = H.DeviceContext(0)
+    got = H.engine.band_spectrum_rects_device(ctx, clips, grid, band, backgrounds=bgs)
+    got = got.cpu().numpy()
+    specs = [oracle.mask_spectrum(n, px, f, dark_field=True, absorber=0.2j) for f in masks]
+    for c in range(len(masks)):
+        err = np.abs(got[c] - _band(specs[c], band)).max() / n ** 2
+        print("grating band", c, err)
+        assert err <= SPEC_TOL, (c, err)
+    eng = H.AerialImageEngine(H.Source.arf(H.IlluminationShape.Annular(0.7, 0.9)),
+                              H.ProjectionOptics.immersion_193i(), grid, max_kernels=16)
+    img = eng.compute_features(masks, (0.0,), dark_field=True, absorber=0.2j).cpu().numpy()
+    for c in range(len(masks)):
+        ref = oracle.socs_image(eng.kernel_set(0.0), specs[c], eng.flare_fraction(), threads=8)
+        err = np.abs(img[c, 0] - ref).max()
+        print("grating image", c, err)
+        assert err <= IMG_TOL, (c, err)
