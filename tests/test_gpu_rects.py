@@ -188,3 +188,27 @@ def test_gratings_and_arrays_match_oracle():
         err = np.abs(img[c, 0] - ref).max()
         print("grating image", c, err)
         assert err <= IMG_TOL, (c, err)
+
+
+def test_tuned_launch_shapes_give_the_same_images():
+    """The rectangle path follows a launch shape set for its kernel sets:
+    a larger coarse grid (wider band box) and one or two x lanes agree with
+    the default shape."""
+    rng = np.random.default_rng(21)
+    n, px = 512, 2.0
+    grid = H.GridConfig(n, px)
+    eng = H.AerialImageEngine(H.Source.arf(H.IlluminationShape.Annular(0.7, 0.9)),
+                              H.ProjectionOptics.immersion_193i(), grid, max_kernels=16)
+    clips = [_random_rects(rng, n * px, k, 1.0) for k in (4, 12)]
+    base = eng.compute_rects(clips, (0.0, 30.0)).cpu().numpy()
+    ctx = eng.context()
+    planes = [eng.device_kernel_set(z) for z in (0.0, 30.0)]
+    B, K = planes[0].band, max(p.host.num_kernels for p in planes)
+    for m, xl in [(128, 1), (128, 2), (256, 2)]:
+        if m < 4 * B + 1:
+            continue
+        ctx.set_tuning(n, B, 2, K, m=m, xlanes=xl)
+        got = eng.compute_rects(clips, (0.0, 30.0)).cpu().numpy()
+        err = np.abs(got - base).max()
+        print("shape", m, xl, err)
+        assert err <= 2e-6, (m, xl, err)
