@@ -48,10 +48,11 @@ __global__ __launch_bounds__(256) void k_rect_tables(const double* __restrict__ 
   Y[idx] = ya;
 }
 
-// One thread per (row ky + BB, group of kRectCols consecutive columns kx + BB),
-// rows fastest, so each cell's row coefficient is loaded once for kRectCols
-// products; work-groups [clip * bpc, (clip + 1) * bpc) cover one clip's box.
-constexpr int kRectCols = 4;
+// One thread per tile of kRectT consecutive rows (ky + BB) x kRectT consecutive
+// columns (kx + BB), row tiles fastest: per cell it loads kRectT row and
+// kRectT column coefficients for kRectT^2 products.  Work-groups
+// [clip * bpc, (clip + 1) * bpc) cover one clip's box.
+constexpr int kRectT = 4;
 
 __global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X,
                                                    const double2* __restrict__ Y,
@@ -60,58 +61,71 @@ __global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X
                                                    int BB, int bpc, float2* __restrict__ Mq,
                                                    float2* __restrict__ Mp) {
   const int W = 2 * BB + 1, nk = 2 * B + 1;
-  const int G = (W + kRectCols - 1) / kRectCols;
+  const int G = (W + kRectT - 1) / kRectT;
   const int clip = blockIdx.x / bpc;
   const int t = (blockIdx.x - clip * bpc) * blockDim.x + threadIdx.x;
-  if (t >= W * G) return;
-  const int g = t / W, row = t - g * W;
-  const int ky = row - BB;
-  const bool row_live = ky >= -B && ky <= B;
-  int xi[kRectCols];
-  bool live[kRectCols];
+  if (t >= G * G) return;
+  const int gc = t / G, gr = t - gc * G;
+  int xi[kRectT], yi[kRectT];
+  bool xl[kRectT], yl[kRectT];
 #pragma unroll
-  for (int j = 0; j < kRectCols; ++j) {
-    const int kx = kRectCols * g + j - BB;
-    live[j] = row_live && kx >= -B && kx <= B && kRectCols * g + j < W;
-    xi[j] = min(max(kx + B, 0), nk - 1);  // clamped: dead columns compute and are discarded
+  for (int j = 0; j < kRectT; ++j) {
+    const int kx = kRectT * gc + j - BB, ky = kRectT * gr + j - BB;
+    xl[j] = kx >= -B && kx <= B && kRectT * gc + j < W;
+    yl[j] = ky >= -B && ky <= B && kRectT * gr + j < W;
+    // clamped: dead rows and columns compute and are discarded
+    xi[j] = min(max(kx + B, 0), nk - 1);
+    yi[j] = min(max(ky + B, 0), nk - 1);
   }
-  double re[kRectCols], im[kRectCols];
+  double re[kRectT][kRectT], im[kRectT][kRectT];
 #pragma unroll
-  for (int j = 0; j < kRectCols; ++j) re[j] = im[j] = 0.0;
-  if (row_live) {
-    const int64_t c0 = coff[clip], c1 = coff[clip + 1];
-    const int yi = ky + B;
-    for (int64_t c = c0; c < c1; ++c) {
-      const double2 y = Y[c * nk + yi];
-      const double2* xr = X + c * nk;
+  for (int i = 0; i < kRectT; ++i)
 #pragma unroll
-      for (int j = 0; j < kRectCols; ++j) {
-        const double2 x = xr[xi[j]];
-        re[j] = fma(x.x, y.x, re[j]);
-        re[j] = fma(-x.y, y.y, re[j]);
-        im[j] = fma(x.x, y.y, im[j]);
-        im[j] = fma(x.y, y.x, im[j]);
-      }
+    for (int j = 0; j < kRectT; ++j) re[i][j] = im[i][j] = 0.0;
+  const int64_t c0 = coff[clip], c1 = coff[clip + 1];
+  const double2* xr = X + c0 * nk;
+  const double2* yr = Y + c0 * nk;
+  for (int64_t c = c0; c < c1; ++c, xr += nk, yr += nk) {
+    double2 x[kRectT], y[kRectT];
+#pragma unroll
+    for (int j = 0; j < kRectT; ++j) {
+      x[j] = xr[xi[j]];
+      y[j] = yr[yi[j]];
     }
+#pragma unroll
+    for (int i = 0; i < kRectT; ++i)
+#pragma unroll
+      for (int j = 0; j < kRectT; ++j) {
+        re[i][j] = fma(x[j].x, y[i].x, re[i][j]);
+        re[i][j] = fma(-x[j].y, y[i].y, re[i][j]);
+        im[i][j] = fma(x[j].x, y[i].y, im[i][j]);
+        im[i][j] = fma(x[j].y, y[i].x, im[i][j]);
+      }
   }
   const double nn = (double)n * (double)n;
   float2* mq = Mq + (size_t)clip * W * W;
   float2* mp = Mp ? Mp + (size_t)clip * (W + 1) * W : nullptr;
 #pragma unroll
-  for (int j = 0; j < kRectCols; ++j) {
-    const int col = kRectCols * g + j;
+  for (int j = 0; j < kRectT; ++j) {
+    const int col = kRectT * gc + j;
     if (col >= W) break;
-    double vr = live[j] ? re[j] : 0.0, vi = live[j] ? im[j] : 0.0;
-    if (col == BB && ky == 0) {  // kx = ky = 0: the background
-      vr += bg[2 * clip] * nn;
-      vi += bg[2 * clip + 1] * nn;
-    }
-    const float2 v = make_float2((float)vr, (float)vi);
-    mq[(size_t)col * W + row] = v;
-    if (mp) {
-      // column-paired copy: (col, row) at [(col + 1) / 2][row][(col + 1) % 2], column -1 zero
-      mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
-      if (col == 0) mp[(size_t)row * 2] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < kRectT; ++i) {
+      const int row = kRectT * gr + i;
+      if (row >= W) break;
+      const bool live = xl[j] && yl[i];
+      double vr = live ? re[i][j] : 0.0, vi = live ? im[i][j] : 0.0;
+      if (col == BB && row == BB) {  // kx = ky = 0: the background
+        vr += bg[2 * clip] * nn;
+        vi += bg[2 * clip + 1] * nn;
+      }
+      const float2 v = make_float2((float)vr, (float)vi);
+      mq[(size_t)col * W + row] = v;
+      if (mp) {
+        // column-paired copy: (col, row) at [(col + 1) / 2][row][(col + 1) % 2], column -1 zero
+        mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
+        if (col == 0) mp[(size_t)row * 2] = make_float2(0.f, 0.f);
+      }
     }
   }
 }
@@ -133,7 +147,8 @@ hipError_t launch_rect_band(const double2* X, const double2* Y, const int64_t* c
   if (nclips <= 0) return hipSuccess;
   if (B < 0 || BB < B || BB > 1024) return hipErrorInvalidValue;
   const int W = 2 * BB + 1;
-  const int bpc = (W * ((W + kRectCols - 1) / kRectCols) + 255) / 256;
+  const int G = (W + kRectT - 1) / kRectT;
+  const int bpc = (G * G + 255) / 256;
   if ((int64_t)bpc * nclips >= (1ll << 31)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_rect_band, dim3((unsigned)(bpc * nclips)), dim3(256), 0, s, X, Y, coff,
                      bg, n, B, BB, bpc, Mq, Mp);
