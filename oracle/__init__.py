@@ -38,6 +38,9 @@ def load():
         "orc_socs_intensity": (C.c_int, [C.c_int, C.c_int, C.c_int64, PI64, PI64, C.c_int64, PD,
                                          PD, PD, PD, C.c_int]),
         "orc_normalize_flare": (None, [PD, C.c_int64, C.c_double, C.c_double]),
+        "orc_polygon_spectrum": (C.c_int, [C.c_int, C.c_double, C.c_int, PD, C.c_double,
+                                           C.c_double, PD]),
+        "orc_polygon_is_simple": (C.c_int, [C.c_int, PD]),
         "orc_mask_spectrum": (C.c_int, [C.c_int, C.c_double, C.c_int, PD, C.c_int, C.c_double,
                                         C.c_double, PD]),
         "orc_pixel_transfer": (None, [PD, C.c_int]),
@@ -142,6 +145,19 @@ def mask_spectrum(n, pixel_nm, features, dark_field=False, absorber=0j):
                                   complex(absorber).real, complex(absorber).imag, _d(out))
     if st != 0:
         raise RuntimeError("overlapping features beyond the exact cell budget")
+    return _from2(out)
+
+
+def polygon_spectrum(n, pixel_nm, vertices, value=1.0, spectrum=None):
+    """Add value x the exact spectrum of a simple polygon (clipped to the field) to
+    `spectrum` (n x n complex, FFT layout; zeros if None); raises for a
+    non-simple polygon."""
+    xy = np.ascontiguousarray(np.asarray(vertices, np.float64).reshape(-1, 2))
+    out = np.zeros((n, n, 2)) if spectrum is None else _c2(spectrum)
+    v = complex(value)
+    st = load().orc_polygon_spectrum(n, pixel_nm, len(xy), _d(xy), v.real, v.imag, _d(out))
+    if st != 0:
+        raise RuntimeError("self-intersecting polygon")
     return _from2(out)
 
 

@@ -16,6 +16,9 @@
 //                                Fourier-series spectrum of rectangles, gratings
 //                                and rectangle arrays; overlaps resolved into
 //                                disjoint cells)
+//   orc_polygon_spectrum         mask.rs:1222-1352 (simple-polygon test,
+//                                Sutherland-Hodgman clip to the field, exact
+//                                edge-sum Fourier integral)
 //   orc_abbe_image               tests/imaging_validation.rs:167-198 (direct Abbe
 //                                sum over source points, pupil at every grid
 //                                frequency, no truncation or eigen-solve)
@@ -513,6 +516,100 @@ int orc_mask_spectrum(int n, double pixel, int nfeat, const double* feats, int d
   std::fill(spec, spec + (size_t)n * n, cd(0, 0));
   spec[0] = bg * ((double)n * n);
   for (auto& t : terms) add_term_spectrum(spec, t, bg, f);
+  return 0;
+}
+
+// Simple polygon (mask.rs:1222-1243, polygon_is_simple): no two
+// non-adjacent edges cross properly.
+static double orient2(const double* a, const double* b, const double* c) {
+  return (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
+}
+int orc_polygon_is_simple(int nv, const double* xy) {
+  for (int a = 0; a < nv; ++a) {
+    const double *a0 = xy + 2 * a, *a1 = xy + 2 * ((a + 1) % nv);
+    for (int b = a + 2; b < nv; ++b) {
+      if (a == 0 && b == nv - 1) continue;
+      const double *b0 = xy + 2 * b, *b1 = xy + 2 * ((b + 1) % nv);
+      if (orient2(a0, a1, b0) * orient2(a0, a1, b1) < 0.0 &&
+          orient2(b0, b1, a0) * orient2(b0, b1, a1) < 0.0)
+        return 0;
+    }
+  }
+  return 1;
+}
+
+// Add w N^2 c(m) e^{-i pi (mx+my)(N-1)/N} of a simple polygon, clipped to the
+// field by Sutherland-Hodgman (mask.rs:1244-1275), with the divergence-theorem
+// edge sum (mask.rs:1286-1352):
+//   \iint_P e^{-ik.r} = (i s/|k|^2) sum_e (kx dy - ky dx) sinc(k.d/2) e^{-ik.r_mid},
+// s = the orientation sign, and the area at k = 0.  spec: n x n, FFT layout.
+// Returns 1 for a non-simple polygon (the reference rasterises it).
+int orc_polygon_spectrum(int n, double pixel, int nv, const double* xy, double wre, double wim,
+                         double* spec_out) {
+  if (!orc_polygon_is_simple(nv, xy)) return 1;
+  const double half = n * pixel / 2.0, L = n * pixel;
+  std::vector<std::pair<double, double>> poly;
+  for (int i = 0; i < nv; ++i) poly.push_back({xy[2 * i], xy[2 * i + 1]});
+  const int axes[4] = {0, 0, 1, 1};
+  const double signs[4] = {-1.0, 1.0, -1.0, 1.0};
+  for (int q = 0; q < 4 && poly.size() >= 3; ++q) {
+    const int ax = axes[q];
+    const double sg = signs[q];
+    auto co = [&](const std::pair<double, double>& p) { return ax == 0 ? p.first : p.second; };
+    auto in = [&](const std::pair<double, double>& p) { return sg * co(p) <= half; };
+    const double bound = sg * half;
+    auto cut = [&](const std::pair<double, double>& a, const std::pair<double, double>& b) {
+      const double t = (bound - co(a)) / (co(b) - co(a));
+      return std::make_pair(a.first + t * (b.first - a.first), a.second + t * (b.second - a.second));
+    };
+    std::vector<std::pair<double, double>> o;
+    const size_t len = poly.size();
+    for (size_t i = 0; i < len; ++i) {
+      const auto cur = poly[i], prev = poly[(i + len - 1) % len];
+      const bool pi_ = in(prev), ci = in(cur);
+      if (pi_ && ci) o.push_back(cur);
+      else if (pi_ && !ci) o.push_back(cut(prev, cur));
+      else if (!pi_ && ci) { o.push_back(cut(prev, cur)); o.push_back(cur); }
+    }
+    poly.swap(o);
+  }
+  if (poly.size() < 3) return 0;
+  const size_t m = poly.size();
+  double twice = 0.0;
+  for (size_t i = 0; i < m; ++i) {
+    const auto a = poly[i], b = poly[(i + 1) % m];
+    twice += a.first * b.second - b.first * a.second;
+  }
+  if (twice == 0.0) return 0;
+  const double sgn = twice > 0 ? 1.0 : -1.0;
+  const cd w(wre, wim);
+  const double nf = n;
+  const cd scale = w * (nf * nf / (L * L));
+  cd* spec = reinterpret_cast<cd*>(spec_out);
+  for (int iy = 0; iy < n; ++iy) {
+    const double my = (double)signed_order(iy, n), ky = 2.0 * PI * my / L;
+    for (int ix = 0; ix < n; ++ix) {
+      const double mx = (double)signed_order(ix, n), kx = 2.0 * PI * mx / L;
+      cd f;
+      if (ix == 0 && iy == 0) {
+        f = cd(0.5 * std::fabs(twice), 0.0);
+      } else {
+        cd acc(0, 0);
+        for (size_t i = 0; i < m; ++i) {
+          const auto a = poly[i], b = poly[(i + 1) % m];
+          const double dx = b.first - a.first, dy = b.second - a.second;
+          if (dx == 0.0 && dy == 0.0) continue;
+          const double kn = kx * dy - ky * dx;
+          if (kn == 0.0) continue;
+          const double cx = 0.5 * (a.first + b.first), cy = 0.5 * (a.second + b.second);
+          acc += std::polar(1.0, -(kx * cx + ky * cy)) * (kn * sinc(0.5 * (kx * dx + ky * dy)));
+        }
+        f = cd(0.0, sgn / (kx * kx + ky * ky)) * acc;
+      }
+      spec[(size_t)iy * n + ix] +=
+          scale * f * std::polar(1.0, -PI * (mx + my) * (nf - 1.0) / nf);
+    }
+  }
   return 0;
 }
 
