@@ -70,6 +70,11 @@ class huv_socs_opts(C.Structure):
     _fields_ = [("pixel_transfer", C.c_int), ("graph", C.c_int)]
 
 
+class huv_polygons(C.Structure):
+    _fields_ = [("poly_offsets", C.POINTER(C.c_int64)), ("vertex_offsets", C.POINTER(C.c_int64)),
+                ("vertices", C.POINTER(C.c_double)), ("values", C.POINTER(C.c_double))]
+
+
 class huv_socs_tuning(C.Structure):
     _fields_ = [("m", C.c_int), ("n1", C.c_int), ("xnb", C.c_int), ("xlanes", C.c_int)]
 
@@ -143,6 +148,11 @@ SIGNATURES = {
                                           PF]),
     "huv_socs_images_rects": (C.c_int, [P, C.c_int, C.c_int, C.c_double, PI64, PD, PD, C.c_int,
                                         C.POINTER(P), PF, C.POINTER(huv_socs_opts)]),
+    "huv_band_spectrum_features": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_int, PI64, PD,
+                                             C.POINTER(huv_polygons), PD, PF]),
+    "huv_socs_images_features": (C.c_int, [P, C.c_int, C.c_int, C.c_double, PI64, PD,
+                                           C.POINTER(huv_polygons), PD, C.c_int, C.POINTER(P), PF,
+                                           C.POINTER(huv_socs_opts)]),
     "huv_socs_images_batch": (C.c_int, [P, C.c_int, C.POINTER(huv_clip_group),
                                         C.POINTER(huv_socs_opts)]),
     "huv_ctx_set_autotune": (C.c_int, [P, C.c_int]),

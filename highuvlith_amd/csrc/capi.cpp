@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -268,17 +269,87 @@ huv_status validate_rects(int nclips, int n, const RectBatch& rb) {
   for (int i = 0; i < 2 * nclips; ++i)
     if (!std::isfinite(rb.background[i]))
       return fail(HUV_INVALID_PARAMETER, "background values must be finite");
+  if (rb.polys) {
+    const huv_polygons& P = *rb.polys;
+    if (!P.poly_offsets) return fail(HUV_INVALID_PARAMETER, "null polygon offsets");
+    if (P.poly_offsets[0] != 0) return fail(HUV_INVALID_PARAMETER, "polygon offsets must start at 0");
+    for (int i = 0; i < nclips; ++i)
+      if (P.poly_offsets[i + 1] < P.poly_offsets[i])
+        return fail(HUV_INVALID_PARAMETER, "polygon offsets must be non-decreasing");
+    const int64_t np = P.poly_offsets[nclips];
+    if (np > 0 && (!P.vertex_offsets || !P.values))
+      return fail(HUV_INVALID_PARAMETER, "null polygon arrays");
+    if (np > 0 && P.vertex_offsets[0] != 0)
+      return fail(HUV_INVALID_PARAMETER, "vertex offsets must start at 0");
+    for (int64_t j = 0; j < np; ++j)
+      if (P.vertex_offsets[j + 1] < P.vertex_offsets[j])
+        return fail(HUV_INVALID_PARAMETER, "vertex offsets must be non-decreasing");
+    const int64_t nv = np > 0 ? P.vertex_offsets[np] : 0;
+    if (nv > 0 && !P.vertices) return fail(HUV_INVALID_PARAMETER, "null polygon vertices");
+    for (int64_t k = 0; k < 2 * nv; ++k)
+      if (!std::isfinite(P.vertices[k]))
+        return fail(HUV_INVALID_PARAMETER, "polygon vertices must be finite");
+    for (int64_t k = 0; k < 2 * np; ++k)
+      if (!std::isfinite(P.values[k]))
+        return fail(HUV_INVALID_PARAMETER, "polygon values must be finite");
+  }
   return HUV_OK;
 }
+
+namespace {
+using Pt = std::pair<double, double>;
+double orient2(const Pt& a, const Pt& b, const Pt& c) {
+  return (b.first - a.first) * (c.second - a.second) - (b.second - a.second) * (c.first - a.first);
+}
+// No two non-adjacent edges cross properly (mask.rs:1222-1243).
+bool polygon_simple(const std::vector<Pt>& v) {
+  const size_t n = v.size();
+  for (size_t a = 0; a < n; ++a)
+    for (size_t b = a + 2; b < n; ++b) {
+      if (a == 0 && b == n - 1) continue;
+      const Pt &a0 = v[a], &a1 = v[(a + 1) % n], &b0 = v[b], &b1 = v[(b + 1) % n];
+      if (orient2(a0, a1, b0) * orient2(a0, a1, b1) < 0.0 &&
+          orient2(b0, b1, a0) * orient2(b0, b1, a1) < 0.0)
+        return false;
+    }
+  return true;
+}
+// Sutherland-Hodgman clip to [-half, half]^2 (mask.rs:1244-1275); slivers
+// along the border of a clipped concave polygon cancel in the edge sum.
+std::vector<Pt> clip_polygon(std::vector<Pt> poly, double half) {
+  for (int q = 0; q < 4 && poly.size() >= 3; ++q) {
+    const bool xa = q < 2;
+    const double sg = (q % 2 == 0) ? -1.0 : 1.0, bound = sg * half;
+    auto co = [&](const Pt& p) { return xa ? p.first : p.second; };
+    std::vector<Pt> o;
+    const size_t len = poly.size();
+    for (size_t i = 0; i < len; ++i) {
+      const Pt cur = poly[i], prev = poly[(i + len - 1) % len];
+      const bool pin = sg * co(prev) <= half, cin = sg * co(cur) <= half;
+      auto cut = [&]() {
+        const double t = (bound - co(prev)) / (co(cur) - co(prev));
+        return Pt(prev.first + t * (cur.first - prev.first), prev.second + t * (cur.second - prev.second));
+      };
+      if (pin && cin) o.push_back(cur);
+      else if (pin) o.push_back(cut());
+      else if (cin) { o.push_back(cut()); o.push_back(cur); }
+    }
+    poly.swap(o);
+  }
+  if (poly.size() < 3) poly.clear();
+  return poly;
+}
+}  // namespace
 
 huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectBatch& rb,
                            float2* Mq, float2* Mp) {
   // Host plan: disjoint cells per clip (painter's rule), staged with the cell
   // offsets and backgrounds in one pinned upload.
-  std::vector<double> cells;
-  std::vector<int64_t> coff((size_t)nclips + 1, 0);
+  std::vector<double> cells, edges, pdc((size_t)nclips * 2, 0.0);
+  std::vector<int64_t> coff((size_t)nclips + 1, 0), eoff((size_t)nclips + 1, 0);
   std::vector<huv::RectFeature> fs;
   std::vector<huv::RectCell> cl;
+  const double half = n * rb.pixel_nm / 2.0;
   for (int i = 0; i < nclips; ++i) {
     fs.clear();
     for (int64_t r = rb.offsets[i]; r < rb.offsets[i + 1]; ++r) {
@@ -292,12 +363,66 @@ huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectB
     for (const auto& e : cl) cells.insert(cells.end(), {e.x0, e.x1, e.y0, e.y1, e.w.real(), e.w.imag()});
     coff[i + 1] = (int64_t)(cells.size() / 6);
     if (coff[i + 1] > kRectMaxCells) return fail(HUV_INVALID_PARAMETER, "too many rectangle cells in one call");
+    if (rb.polys) {
+      // bounding boxes of the clip's features, for the overlap rule
+      std::vector<std::array<double, 4>> boxes;
+      for (const auto& f : fs) {
+        if (!(f.w > 0 && f.h > 0)) continue;
+        const double x0 = std::max(f.x - f.w / 2, -half), x1 = std::min(f.x + f.w / 2, half);
+        const double y0 = std::max(f.y - f.h / 2, -half), y1 = std::min(f.y + f.h / 2, half);
+        if (x1 > x0 && y1 > y0) boxes.push_back({x0, x1, y0, y1});
+      }
+      const huv_polygons& P = *rb.polys;
+      for (int64_t j = P.poly_offsets[i]; j < P.poly_offsets[i + 1]; ++j) {
+        std::vector<Pt> v;
+        for (int64_t k = P.vertex_offsets[j]; k < P.vertex_offsets[j + 1]; ++k)
+          v.push_back({P.vertices[2 * k], P.vertices[2 * k + 1]});
+        if (v.size() < 3) continue;
+        if (!polygon_simple(v))
+          return fail(HUV_INVALID_PARAMETER, "clip " + std::to_string(i) + ", polygon " +
+                                                 std::to_string(j - P.poly_offsets[i]) +
+                                                 ": edges cross; rasterise it instead");
+        v = clip_polygon(v, half);
+        if (v.empty()) continue;
+        double twice = 0.0;
+        std::array<double, 4> bb{v[0].first, v[0].first, v[0].second, v[0].second};
+        for (size_t k = 0; k < v.size(); ++k) {
+          const Pt &a = v[k], &b = v[(k + 1) % v.size()];
+          twice += a.first * b.second - b.first * a.second;
+          bb = {std::min(bb[0], a.first), std::max(bb[1], a.first), std::min(bb[2], a.second),
+                std::max(bb[3], a.second)};
+        }
+        if (twice == 0.0) continue;
+        for (const auto& o : boxes)
+          if (std::min(bb[1], o[1]) > std::max(bb[0], o[0]) && std::min(bb[3], o[3]) > std::max(bb[2], o[2]))
+            return fail(HUV_INVALID_PARAMETER, "clip " + std::to_string(i) + ", polygon " +
+                                                   std::to_string(j - P.poly_offsets[i]) +
+                                                   ": overlaps another feature; rasterise the clip instead");
+        boxes.push_back(bb);
+        const huv::cd w = huv::cd(P.values[2 * j], P.values[2 * j + 1]) - bg;
+        const double sg = twice > 0 ? 1.0 : -1.0;
+        pdc[2 * i] += w.real() * 0.5 * std::fabs(twice);
+        pdc[2 * i + 1] += w.imag() * 0.5 * std::fabs(twice);
+        for (size_t k = 0; k < v.size(); ++k) {
+          const Pt &a = v[k], &b = v[(k + 1) % v.size()];
+          const double dx = b.first - a.first, dy = b.second - a.second;
+          if (dx == 0.0 && dy == 0.0) continue;
+          edges.insert(edges.end(), {dx, dy, 0.5 * (a.first + b.first), 0.5 * (a.second + b.second),
+                                     sg * w.real(), sg * w.imag()});
+        }
+      }
+    }
+    eoff[i + 1] = (int64_t)(edges.size() / 6);
   }
   const int64_t ncells = coff[nclips];
   const size_t cell_b = cells.size() * sizeof(double);
   const size_t off_b = coff.size() * sizeof(int64_t);
   const size_t bg_b = (size_t)nclips * 2 * sizeof(double);
-  const size_t bytes = cell_b + off_b + bg_b;
+  const int64_t nedges = eoff[nclips];
+  const size_t edge_b = edges.size() * sizeof(double);
+  const size_t eoff_b = nedges > 0 ? eoff.size() * sizeof(int64_t) : 0;
+  const size_t pdc_b = nedges > 0 ? pdc.size() * sizeof(double) : 0;
+  const size_t bytes = cell_b + off_b + bg_b + edge_b + eoff_b + pdc_b;
   hipError_t e;
   // the staging area may still feed an earlier copy on the stream
   e = hipStreamSynchronize(c->stream());
@@ -306,6 +431,12 @@ huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectB
   if (cell_b) std::memcpy(c->pin_h, cells.data(), cell_b);
   std::memcpy(c->pin_h + cell_b, coff.data(), off_b);
   std::memcpy(c->pin_h + cell_b + off_b, rb.background, bg_b);
+  char* ph = c->pin_h + cell_b + off_b + bg_b;
+  if (nedges > 0) {
+    std::memcpy(ph, edges.data(), edge_b);
+    std::memcpy(ph + edge_b, eoff.data(), eoff_b);
+    std::memcpy(ph + edge_b + eoff_b, pdc.data(), pdc_b);
+  }
   e = hipMemcpyAsync(c->pin_d, c->pin_h, bytes, hipMemcpyHostToDevice, c->stream());
   if (e != hipSuccess) return hip_fail(e, "rectangle upload");
   const double* d_cells = (const double*)c->pin_d;
@@ -328,6 +459,13 @@ huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectB
   c->stage_begin(ST_COLS_BAND, &ev);
   e = huvk::launch_rect_band(X, Y, d_coff, d_bg, nclips, n, B, BB, Mq, Mp, c->stream());
   if (e != hipSuccess) return hip_fail(e, "rect_band launch");
+  if (nedges > 0) {
+    const char* pd = c->pin_d + cell_b + off_b + bg_b;
+    e = huvk::launch_poly_band((const double*)pd, (const int64_t*)(pd + edge_b),
+                               (const double*)(pd + edge_b + eoff_b), nclips, n, L, B, BB, Mq, Mp,
+                               c->stream());
+    if (e != hipSuccess) return hip_fail(e, "poly_band launch");
+  }
   c->stage_end(ST_COLS_BAND, ev);
   return HUV_OK;
 }
@@ -1065,17 +1203,19 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
   return socs_groups(c, 1, &g, opts, 1);
 }
 
-huv_status huv_socs_images_rects(huv_ctx* c, int nclips, int n, double pixel_nm,
-                                 const int64_t* offsets, const double* rects,
-                                 const double* background, int nplanes, huv_kset* const* planes,
-                                 float* out, const huv_socs_opts* opts) {
+huv_status huv_socs_images_features(huv_ctx* c, int nclips, int n, double pixel_nm,
+                                    const int64_t* offsets, const double* rects,
+                                    const huv_polygons* polys, const double* background,
+                                    int nplanes, huv_kset* const* planes, float* out,
+                                    const huv_socs_opts* opts) {
   (void)opts;  // the spectrum is exact (no pixel transfer); no graph capture
   if (!c || !planes || !out || !offsets || !background)
     return fail(HUV_INVALID_PARAMETER, "null argument");
   if (nclips < 0 || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0 and nplanes >= 1");
   huv_status st = validate_planes(c, n, nplanes, planes);
   if (st != HUV_OK) return st;
-  const RectBatch rb{pixel_nm, offsets, rects, background};
+  RectBatch rb{pixel_nm, offsets, rects, background};
+  rb.polys = polys;
   st = validate_rects(nclips, n, rb);
   if (st != HUV_OK) return st;
   if (nclips == 0) return HUV_OK;
@@ -1085,6 +1225,14 @@ huv_status huv_socs_images_rects(huv_ctx* c, int nclips, int n, double pixel_nm,
   auto it = c->tuned.find(tuning_key(n, planes, nplanes));
   return socs_forward(c, nclips, n, nullptr, nplanes, planes, out, false, nullptr,
                       it != c->tuned.end() ? &it->second : nullptr, &rb);
+}
+
+huv_status huv_socs_images_rects(huv_ctx* c, int nclips, int n, double pixel_nm,
+                                 const int64_t* offsets, const double* rects,
+                                 const double* background, int nplanes, huv_kset* const* planes,
+                                 float* out, const huv_socs_opts* opts) {
+  return huv_socs_images_features(c, nclips, n, pixel_nm, offsets, rects, nullptr, background,
+                                  nplanes, planes, out, opts);
 }
 
 huv_status huv_socs_images_batch(huv_ctx* c, int ngroups, const huv_clip_group* groups,
@@ -1185,19 +1333,28 @@ huv_status huv_band_spectrum(huv_ctx* c, int nclips, int n, int band, const floa
   return band_stage(c, nclips, n, band, band, masks, (float2*)out, pixel_transfer != 0);
 }
 
-huv_status huv_band_spectrum_rects(huv_ctx* c, int nclips, int n, double pixel_nm, int band,
-                                   const int64_t* offsets, const double* rects,
-                                   const double* background, float* out) {
+huv_status huv_band_spectrum_features(huv_ctx* c, int nclips, int n, double pixel_nm, int band,
+                                      const int64_t* offsets, const double* rects,
+                                      const huv_polygons* polys, const double* background,
+                                      float* out) {
   if (!c || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
   if (nclips < 0) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0");
   if (band < 0 || 2 * band + 1 > n) return fail(HUV_INVALID_PARAMETER, "band out of range");
-  const RectBatch rb{pixel_nm, offsets, rects, background};
+  RectBatch rb{pixel_nm, offsets, rects, background};
+  rb.polys = polys;
   huv_status st = validate_rects(nclips, n, rb);
   if (st != HUV_OK) return st;
   if (nclips == 0) return HUV_OK;
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
   return rect_band_stage(c, nclips, n, band, band, rb, (float2*)out, nullptr);
+}
+
+huv_status huv_band_spectrum_rects(huv_ctx* c, int nclips, int n, double pixel_nm, int band,
+                                   const int64_t* offsets, const double* rects,
+                                   const double* background, float* out) {
+  return huv_band_spectrum_features(c, nclips, n, pixel_nm, band, offsets, rects, nullptr,
+                                    background, out);
 }
 
 }  // extern "C"

@@ -97,6 +97,7 @@ struct RectBatch {
   const int64_t* offsets;
   const double* rects;
   const double* background;
+  const huv_polygons* polys = nullptr;  // optional simple polygons per clip
 };
 // Exact band spectrum of rectangle masks into Mq (and Mp), same layout as band_stage.
 huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectBatch& rb,

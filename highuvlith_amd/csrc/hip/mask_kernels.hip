@@ -130,6 +130,84 @@ __global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X
   }
 }
 
+// Simple polygons (mask.rs:1286-1352): for k = 2 pi m / L != 0
+//   \iint_P e^{-ik.r} = (i/|k|^2) sum_e s (kx dy - ky dx) sinc(k.d/2) e^{-ik.r_mid}
+// with s the polygon's orientation sign, and the area at k = 0.  edges:
+// (dx, dy, cx, cy, re, im) with (re, im) = s (value - background); pdc per
+// clip = sum of (value - background) area.  The box bins |kx|, |ky| <= B get
+//   (N^2/L^2) e^{-i pi (mx + my)(N - 1)/N} x integral
+// added to what the rectangle pass stored (plain and column-paired).
+__global__ __launch_bounds__(256) void k_poly_band(const double* __restrict__ edges,
+                                                   const int64_t* __restrict__ eoff,
+                                                   const double* __restrict__ pdc, int n,
+                                                   double L, int B, int BB, int bpc,
+                                                   float2* __restrict__ Mq,
+                                                   float2* __restrict__ Mp) {
+  const int W = 2 * BB + 1;
+  const int clip = blockIdx.x / bpc;
+  const int b = (blockIdx.x - clip * bpc) * blockDim.x + threadIdx.x;
+  if (b >= W * W) return;
+  const int col = b / W, row = b - col * W;
+  const int mx = col - BB, my = row - BB;
+  if (mx < -B || mx > B || my < -B || my > B) return;
+  const int64_t e0 = eoff[clip], e1 = eoff[clip + 1];
+  if (e0 == e1) return;
+  double fr, fi;
+  if (mx == 0 && my == 0) {
+    fr = pdc[2 * clip];
+    fi = pdc[2 * clip + 1];
+  } else {
+    const double kx = 2.0 * M_PI * mx / L, ky = 2.0 * M_PI * my / L;
+    double ar = 0.0, ai = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const double* p = edges + 6 * e;
+      const double dx = p[0], dy = p[1];
+      const double kn = kx * dy - ky * dx;
+      if (kn == 0.0) continue;
+      const double u = 0.5 * (kx * dx + ky * dy);
+      const double sc = u == 0.0 ? 1.0 : sin(u) / u;
+      double sv, cv;
+      sincos(-(kx * p[2] + ky * p[3]), &sv, &cv);
+      const double a = kn * sc;
+      // (re + i im) e^{i phase} a
+      ar += a * (p[4] * cv - p[5] * sv);
+      ai += a * (p[4] * sv + p[5] * cv);
+    }
+    // i / |k|^2
+    const double inv = 1.0 / (kx * kx + ky * ky);
+    fr = -ai * inv;
+    fi = ar * inv;
+  }
+  const double nf = (double)n;
+  const double scale = nf * nf / (L * L);
+  double t = (double)(mx + my) * (nf - 1.0) / nf;
+  t -= 2.0 * rint(0.5 * t);
+  double sv, cv;
+  sincospi(-t, &sv, &cv);
+  const double vr = scale * (fr * cv - fi * sv), vi = scale * (fr * sv + fi * cv);
+  float2* q = Mq + (size_t)clip * W * W + (size_t)col * W + row;
+  const float2 old = *q;
+  const float2 v = make_float2((float)((double)old.x + vr), (float)((double)old.y + vi));
+  *q = v;
+  if (Mp) {
+    float2* mp = Mp + (size_t)clip * (W + 1) * W;
+    mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
+  }
+}
+
+hipError_t launch_poly_band(const double* edges, const int64_t* eoff, const double* pdc,
+                            int nclips, int n, double L, int B, int BB, float2* Mq, float2* Mp,
+                            hipStream_t s) {
+  if (nclips <= 0) return hipSuccess;
+  if (B < 0 || BB < B || BB > 1024) return hipErrorInvalidValue;
+  const int W = 2 * BB + 1;
+  const int bpc = (W * W + 255) / 256;
+  if ((int64_t)bpc * nclips >= (1ll << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_poly_band, dim3((unsigned)(bpc * nclips)), dim3(256), 0, s, edges, eoff,
+                     pdc, n, L, B, BB, bpc, Mq, Mp);
+  return hipGetLastError();
+}
+
 hipError_t launch_rect_tables(const double* cells, int64_t ncells, int n, double L, int B,
                               double2* X, double2* Y, hipStream_t s) {
   if (ncells <= 0) return hipSuccess;

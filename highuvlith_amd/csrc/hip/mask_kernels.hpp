@@ -18,4 +18,12 @@ hipError_t launch_rect_band(const double2* X, const double2* Y, const int64_t* c
                             const double* bg, int nclips, int n, int B, int BB, float2* Mq,
                             float2* Mp, hipStream_t s);
 
+// Simple polygons added onto the box written by launch_rect_band.  edges:
+// (dx, dy, cx, cy, re, im) per edge, (re, im) = orientation sign x (value -
+// background); eoff: nclips + 1 edge offsets; pdc: per clip sum of
+// (value - background) x area (nm^2).
+hipError_t launch_poly_band(const double* edges, const int64_t* eoff, const double* pdc,
+                            int nclips, int n, double L, int B, int BB, float2* Mq, float2* Mp,
+                            hipStream_t s);
+
 }  // namespace huvk

@@ -510,6 +510,44 @@ def mask_feature_rects(features, grid, dark_field=False, absorber=0j):
     return rects, bg
 
 
+def mask_feature_lists(features, grid, dark_field=False, absorber=0j):
+    """Like mask_feature_rects, and ("polygon", [(x, y), ...]) features as
+    (vertices, amplitude) pairs: returns (rects, polygons, background)."""
+    rect_feats, polys = [], []
+    fg = 1.0 + 0j if dark_field else complex(absorber)
+    for f in features:
+        if f[0] == "polygon":
+            polys.append(([(float(x), float(y)) for (x, y) in f[1]], fg))
+        else:
+            rect_feats.append(f)
+    rects, bg = mask_feature_rects(rect_feats, grid, dark_field, absorber)
+    return rects, polys, bg
+
+
+class PolygonArrays:
+    """Host arrays of a batch's polygons for the C ABI (huv_polygons):
+    poly_clips = [[(vertices, value), ...], ...], one list per clip."""
+
+    def __init__(self, poly_clips):
+        nclips = len(poly_clips)
+        self.poly_offsets = np.zeros(nclips + 1, np.int64)
+        verts, vals, voff = [], [], [0]
+        for i, polys in enumerate(poly_clips):
+            self.poly_offsets[i + 1] = self.poly_offsets[i] + len(polys)
+            for vertices, value in polys:
+                verts.extend(vertices)
+                voff.append(len(verts))
+                v = complex(value)
+                vals.append((v.real, v.imag))
+        self.vertex_offsets = np.asarray(voff, np.int64)
+        self.vertices = np.ascontiguousarray(np.asarray(verts, np.float64).reshape(-1, 2)
+                                             if verts else np.zeros((1, 2)))
+        self.values = np.ascontiguousarray(np.asarray(vals, np.float64).reshape(-1, 2)
+                                           if vals else np.zeros((1, 2)))
+        self.struct = _lib.huv_polygons(i64ptr(self.poly_offsets), i64ptr(self.vertex_offsets),
+                                        dptr(self.vertices), dptr(self.values))
+
+
 def rect_arrays(clips, backgrounds=0.0):
     """Host arrays (offsets, rects, background) of a batch of rectangle masks for the C ABI:
     clips = [[(x, y, w, h, value), ...], ...] (centre and size in nm, later rectangles
@@ -532,6 +570,21 @@ def rect_arrays(clips, backgrounds=0.0):
     bg = np.array([[complex(b).real, complex(b).imag] for b in backgrounds],
                   np.float64).reshape(nclips, 2)
     return offsets, rects, np.ascontiguousarray(bg)
+
+
+def band_spectrum_features_device(ctx, clips, poly_clips, grid, band, backgrounds=0.0):
+    """Exact band spectrum of masks of rectangles (clips, as for rect_arrays) and simple
+    polygons (poly_clips, as for PolygonArrays); torch complex64 (N, W, W)."""
+    import torch
+    offsets, rects, bg = rect_arrays(clips, backgrounds)
+    pa = PolygonArrays(poly_clips)
+    nclips, W = len(clips), 2 * band + 1
+    out = torch.empty((nclips, W, W, 2), dtype=torch.float32, device="cuda")
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    check(ctx.lib.huv_band_spectrum_features(
+        ctx.handle, nclips, grid.size, grid.pixel_nm, band, i64ptr(offsets), dptr(rects),
+        C.byref(pa.struct), dptr(bg), C.c_void_p(out.data_ptr())))
+    return torch.view_as_complex(out)
 
 
 def band_spectrum_rects_device(ctx, clips, grid, band, backgrounds=0.0):
@@ -716,15 +769,30 @@ class AerialImageEngine:
         return socs_images_rects_device(self.context(), self.grid, arrays, planes, out)
 
     def compute_features(self, masks, defocus_nm=(0.0,), dark_field=False, absorber=0j):
-        """compute(mask) (aerial.rs:1775-1777) for a batch of masks given as lists of
-        rectangle-type features (see mask_feature_rects), from their exact spectrum.
-        Returns a torch tensor (N, P, n, n)."""
-        clips, bgs = [], []
+        """compute(mask) (aerial.rs:1775-1777) for a batch of masks given as feature
+        lists (rectangles, gray rectangles, gratings, arrays and simple polygons; see
+        mask_feature_lists), from their exact spectrum.  Returns a torch tensor
+        (N, P, n, n)."""
+        import torch
+        clips, polys, bgs = [], [], []
         for feats in masks:
-            r, b = mask_feature_rects(feats, self.grid, dark_field, absorber)
+            r, p, b = mask_feature_lists(feats, self.grid, dark_field, absorber)
             clips.append(r)
+            polys.append(p)
             bgs.append(b)
-        return self.compute_rects(clips, defocus_nm, backgrounds=bgs)
+        n = self.grid.size
+        planes = [self.device_kernel_set(z) for z in defocus_nm]
+        offsets, rects, bg = rect_arrays(clips, bgs)
+        pa = PolygonArrays(polys)
+        out = torch.empty((len(clips), len(planes), n, n), dtype=torch.float32,
+                          device=f"cuda:{self.device}")
+        ctx = self.context()
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
+        check(ctx.lib.huv_socs_images_features(
+            ctx.handle, len(clips), n, self.grid.pixel_nm, i64ptr(offsets), dptr(rects),
+            C.byref(pa.struct), dptr(bg), len(planes), arr, C.c_void_p(out.data_ptr()), None))
+        return out
 
     def compute_from_transmittance(self, transmittance, defocus_nm=0.0):
         t = np.asarray(transmittance)

@@ -253,6 +253,35 @@ huv_status huv_band_spectrum(huv_ctx* ctx, int nclips, int n, int band, const fl
 huv_status huv_band_spectrum_rects(huv_ctx* ctx, int nclips, int n, double pixel_nm, int band,
                                    const int64_t* offsets, const double* rects,
                                    const double* background, float* out);
+/* Simple polygons of a batch (host arrays): clip i holds polygons
+ * [poly_offsets[i], poly_offsets[i + 1]); polygon j has the vertices
+ * [vertex_offsets[j], vertex_offsets[j + 1]) of `vertices` ((x, y) in nm,
+ * either orientation) and complex amplitude values[2 j], values[2 j + 1].
+ * A polygon is clipped to the field; one whose edges cross, or whose bounding
+ * box meets another feature's of the same clip, is rejected (the reference
+ * rasterises those, mask.rs:1190-1243). */
+typedef struct {
+  const int64_t* poly_offsets;
+  const int64_t* vertex_offsets;
+  const double* vertices;
+  const double* values;
+} huv_polygons;
+
+/* Exact band spectrum of masks of rectangles and simple polygons (the
+ * analytic path of Mask::spectrum, mask.rs:616-657, polygons by the edge sum
+ * of mask.rs:1286-1352); arguments as in huv_band_spectrum_rects, polys may
+ * be NULL. */
+huv_status huv_band_spectrum_features(huv_ctx* ctx, int nclips, int n, double pixel_nm, int band,
+                                      const int64_t* offsets, const double* rects,
+                                      const huv_polygons* polys, const double* background,
+                                      float* out);
+/* Aerial images of masks of rectangles and simple polygons from their exact
+ * spectrum (AerialImageEngine::compute(mask, defocus), aerial.rs:1775-1777). */
+huv_status huv_socs_images_features(huv_ctx* ctx, int nclips, int n, double pixel_nm,
+                                    const int64_t* offsets, const double* rects,
+                                    const huv_polygons* polys, const double* background,
+                                    int nplanes, huv_kset* const* planes, float* out,
+                                    const huv_socs_opts* opts);
 /* Aerial images of rectangle masks from their exact spectrum (replaces
  * AerialImageEngine::compute(mask, defocus), aerial.rs:1775-1777, for
  * rectangle masks: mask.spectrum then compute_from_spectrum).  planes and out

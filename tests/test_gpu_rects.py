@@ -212,3 +212,56 @@ def test_tuned_launch_shapes_give_the_same_images():
         err = np.abs(got - base).max()
         print("shape", m, xl, err)
         assert err <= 2e-6, (m, xl, err)
+
+
+def test_polygons_match_oracle():
+    """Simple polygons (mask.rs:1286-1352: either orientation, concave,
+    clipped at the field edge, complex amplitude) next to rectangles: band and
+    images against the oracle's edge-sum spectrum (orc_polygon_spectrum)."""
+    n, px, band = 256, 1.5, 40
+    grid = H.GridConfig(n, px)
+    tri = [(-60.0, -50.0), (20.0, -45.0), (-30.0, 40.0)]
+    l_shape = [(40.0, -30.0), (230.0, -30.0), (230.0, 10.0), (70.0, 10.0), (70.0, 50.0),
+               (40.0, 50.0)]                                   # clipped at x = 192
+    cw = [(-150.0, 100.0), (-100.0, 100.0), (-100.0, 60.0), (-150.0, 60.0)][::-1]
+    v2 = 0.4 - 0.7j
+    rect_clips = [[(-120.0, -120.0, 30.0, 20.0, 1.0)], [], [(0.0, 150.0, 40.0, 20.0, 1.0)]]
+    poly_clips = [[(tri, 1.0)], [(l_shape, v2), (cw, 1.0)], [(tri, 1.0), (l_shape, 1.0)]]
+    ctx = H.DeviceContext(0)
+    got = H.engine.band_spectrum_features_device(ctx, rect_clips, poly_clips, grid, band)
+    got = got.cpu().numpy()
+    specs = []
+    for rects, polys in zip(rect_clips, poly_clips):
+        s = oracle.mask_spectrum(n, px, [("rect",) + tuple(r[:4]) for r in rects], dark_field=True)
+        for verts, v in polys:
+            s = oracle.polygon_spectrum(n, px, verts, v, spectrum=s)
+        specs.append(s)
+    for c in range(3):
+        err = np.abs(got[c] - _band(specs[c], band)).max() / n ** 2
+        print("polygon band", c, err)
+        assert err <= SPEC_TOL, (c, err)
+    eng = H.AerialImageEngine(H.Source.arf(H.IlluminationShape.Annular(0.7, 0.9)),
+                              H.ProjectionOptics.immersion_193i(), grid, max_kernels=16)
+    masks = [[("polygon", tri), ("rect", -120.0, -120.0, 30.0, 20.0)],
+             [("polygon", cw), ("polygon", l_shape)]]
+    img = eng.compute_features(masks, (0.0,), dark_field=True).cpu().numpy()
+    refs = [specs[0],
+            oracle.polygon_spectrum(n, px, l_shape, 1.0, spectrum=oracle.polygon_spectrum(n, px, cw))]
+    for c in range(2):
+        ref = oracle.socs_image(eng.kernel_set(0.0), refs[c], eng.flare_fraction(), threads=8)
+        err = np.abs(img[c, 0] - ref).max()
+        print("polygon image", c, err)
+        assert err <= IMG_TOL, (c, err)
+
+
+def test_crossing_or_overlapping_polygons_are_rejected():
+    n, px = 256, 1.0
+    grid = H.GridConfig(n, px)
+    ctx = H.DeviceContext(0)
+    bow = [(-40.0, -20.0), (40.0, 20.0), (40.0, -20.0), (-40.0, 20.0)]
+    with pytest.raises(H.HuvError):
+        H.engine.band_spectrum_features_device(ctx, [[]], [[(bow, 1.0)]], grid, 10)
+    tri = [(-30.0, -30.0), (30.0, -30.0), (-30.0, 30.0)]
+    with pytest.raises(H.HuvError):
+        H.engine.band_spectrum_features_device(ctx, [[(0.0, 0.0, 20.0, 20.0, 1.0)]],
+                                               [[(tri, 1.0)]], grid, 10)
