@@ -15,7 +15,9 @@ Also reported in the same JSON line (each skipped with --no-rects,
            offset, 64 disjoint cells each): exact band spectrum in FP64 on the
            device, then the same field/coarse/image kernels; Mpix/s including
            the host cell plan and upload, and the rect_band kernel against
-           the FP64 vector peak (8 flops per cell and band bin).
+           the FP64 vector peak (8 flops per cell and band bin); then the
+           same contacts as regular hexagons (simple polygons, edge-sum
+           kernel), reported as Mpix/s and edge-bins/s.
   uneven   1000 clips with edges {256, 512, 1024, 2048} px (p .4/.3/.2/.1),
            30-60 % clear, ArF 193i annular 0.7/0.9 at 2 nm, K = 32, one graph
            launch per batch, autotuned launch shapes; aggregate Mpix/s and
@@ -254,6 +256,35 @@ def bench_rects(args, H, W, np, torch):
     tab_ms, band_ms = stage_ms["rows_band"], stage_ms["cols_band"]
     flops = 8.0 * cells * (2 * B + 1) ** 2
     tf = flops / (band_ms * 1e-3) / 1e12
+
+    # The same clips with the contacts drawn as regular hexagons (simple
+    # polygons, 6 edges each): the spectrum band stage then also runs the
+    # polygon edge-sum kernel; timed the same way.
+    hexes = []
+    for rects in clips:
+        hexes.append([([(x + 0.5 * w * np.cos(a), y + 0.5 * w * np.sin(a))
+                        for a in np.arange(6) * (np.pi / 3)], 1.0) for (x, y, w, h, _) in rects])
+    edges = 6 * 64 * nclips
+    no_rects = H.engine.rect_arrays([[] for _ in clips], 0.0)
+    hex_arrays = H.engine.PolygonArrays(hexes)
+
+    def poly_step():
+        H.engine.socs_images_features_device(ctx, grid, no_rects, hex_arrays, planes, out)
+
+    poly_step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        poly_step()
+    torch.cuda.synchronize()
+    dt_poly = (time.perf_counter() - t) / args.steps
+    ctx.set_profiling(True)
+    ctx.stage_times(reset=True)
+    poly_step()
+    torch.cuda.synchronize()
+    pst = ctx.stage_times(reset=True)
+    ctx.set_profiling(False)
+    poly_band_ms = pst["cols_band"][0] / max(pst["cols_band"][1], 1)
     return {"workload": f"{nclips} clips of 8x8 contacts (sub-pixel), 1024x1024 @ 1 nm, EUV "
                         f"x-dipole, {P} focus planes, K={K}, exact rectangle spectrum",
             "value": round(nclips * P * n * n / 1e6 / dt, 2), "unit": "Mpix/s",
@@ -263,7 +294,14 @@ def bench_rects(args, H, W, np, torch):
             "rect_band_roofline": {"bound": "fp64 vector", "achieved": round(tf, 3),
                                    "peak": 78.6, "unit": "TFLOP/s",
                                    "frac": round(tf / 78.6, 4)},
-            "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()}}
+            "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+            "polygons": {"workload": f"same clips, contacts as regular hexagons ({edges} edges)",
+                         "value": round(nclips * P * n * n / 1e6 / dt_poly, 2),
+                         "unit": "Mpix/s (host polygon plan and upload included)",
+                         "ms_per_batch": round(dt_poly * 1e3, 4),
+                         "band_stage_ms": round(poly_band_ms, 4),
+                         "edge_bins_per_s": round(edges * (2 * B + 1) ** 2 / (poly_band_ms * 1e-3),
+                                                  1)}}
 
 
 def bench_uneven(args, H, W, np, torch):

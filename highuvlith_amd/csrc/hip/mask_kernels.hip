@@ -137,6 +137,19 @@ __global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X
 // clip = sum of (value - background) area.  The box bins |kx|, |ky| <= B get
 //   (N^2/L^2) e^{-i pi (mx + my)(N - 1)/N} x integral
 // added to what the rectangle pass stored (plain and column-paired).
+// One thread per tile of kRectT rows x kRectT columns: per edge it evaluates
+// e^{-i kx cx} and the angle kx dx / 2 at its first column and steps them
+// along the tile by angle addition (likewise along the rows), so each edge
+// costs four sine/cosine pairs for kRectT^2 bins; sin(k.d/2) is the sine of
+// the sum of the two half-angles, and sinc switches to its Taylor series
+// below |u| = 1e-4, where that sum would lose relative accuracy.
+__device__ __forceinline__ void cexp_neg(double t, double& c, double& s) {
+  double sv, cv;
+  sincos(t, &sv, &cv);
+  c = cv;
+  s = -sv;
+}
+
 __global__ __launch_bounds__(256) void k_poly_band(const double* __restrict__ edges,
                                                    const int64_t* __restrict__ eoff,
                                                    const double* __restrict__ pdc, int n,
@@ -144,54 +157,121 @@ __global__ __launch_bounds__(256) void k_poly_band(const double* __restrict__ ed
                                                    float2* __restrict__ Mq,
                                                    float2* __restrict__ Mp) {
   const int W = 2 * BB + 1;
+  const int G = (W + kRectT - 1) / kRectT;
   const int clip = blockIdx.x / bpc;
-  const int b = (blockIdx.x - clip * bpc) * blockDim.x + threadIdx.x;
-  if (b >= W * W) return;
-  const int col = b / W, row = b - col * W;
-  const int mx = col - BB, my = row - BB;
-  if (mx < -B || mx > B || my < -B || my > B) return;
+  const int t = (blockIdx.x - clip * bpc) * blockDim.x + threadIdx.x;
+  if (t >= G * G) return;
   const int64_t e0 = eoff[clip], e1 = eoff[clip + 1];
   if (e0 == e1) return;
-  double fr, fi;
-  if (mx == 0 && my == 0) {
-    fr = pdc[2 * clip];
-    fi = pdc[2 * clip + 1];
-  } else {
-    const double kx = 2.0 * M_PI * mx / L, ky = 2.0 * M_PI * my / L;
-    double ar = 0.0, ai = 0.0;
-    for (int64_t e = e0; e < e1; ++e) {
-      const double* p = edges + 6 * e;
-      const double dx = p[0], dy = p[1];
-      const double kn = kx * dy - ky * dx;
-      if (kn == 0.0) continue;
-      const double u = 0.5 * (kx * dx + ky * dy);
-      const double sc = u == 0.0 ? 1.0 : sin(u) / u;
-      double sv, cv;
-      sincos(-(kx * p[2] + ky * p[3]), &sv, &cv);
-      const double a = kn * sc;
-      // (re + i im) e^{i phase} a
-      ar += a * (p[4] * cv - p[5] * sv);
-      ai += a * (p[4] * sv + p[5] * cv);
+  const int gc = t / G, gr = t - gc * G;
+  const int mx0 = kRectT * gc - BB, my0 = kRectT * gr - BB;
+  const double dk = 2.0 * M_PI / L;
+  const double kx0 = dk * mx0, ky0 = dk * my0;
+  double ar[kRectT][kRectT], ai[kRectT][kRectT];
+#pragma unroll
+  for (int i = 0; i < kRectT; ++i)
+#pragma unroll
+    for (int j = 0; j < kRectT; ++j) ar[i][j] = ai[i][j] = 0.0;
+  for (int64_t e = e0; e < e1; ++e) {
+    const double* p = edges + 6 * e;
+    const double dx = p[0], dy = p[1], cx = p[2], cy = p[3], wr = p[4], wi = p[5];
+    // columns: w e^{-i kx cx} and (cos, sin)(kx dx / 2)
+    double pxr[kRectT], pxi[kRectT], cax[kRectT], sax[kRectT];
+    double pyr[kRectT], pyi[kRectT], cby[kRectT], sby[kRectT];
+    {
+      double c0, s0, cs, ss;
+      cexp_neg(kx0 * cx, c0, s0);
+      cexp_neg(dk * cx, cs, ss);
+      pxr[0] = wr * c0 - wi * s0;
+      pxi[0] = wr * s0 + wi * c0;
+      double sa, ca, sd, cd;
+      sincos(0.5 * kx0 * dx, &sa, &ca);
+      sincos(0.5 * dk * dx, &sd, &cd);
+      cax[0] = ca;
+      sax[0] = sa;
+#pragma unroll
+      for (int j = 1; j < kRectT; ++j) {
+        pxr[j] = pxr[j - 1] * cs - pxi[j - 1] * ss;
+        pxi[j] = pxr[j - 1] * ss + pxi[j - 1] * cs;
+        cax[j] = cax[j - 1] * cd - sax[j - 1] * sd;
+        sax[j] = sax[j - 1] * cd + cax[j - 1] * sd;
+      }
     }
-    // i / |k|^2
-    const double inv = 1.0 / (kx * kx + ky * ky);
-    fr = -ai * inv;
-    fi = ar * inv;
+    {
+      double c0, s0, cs, ss;
+      cexp_neg(ky0 * cy, c0, s0);
+      cexp_neg(dk * cy, cs, ss);
+      pyr[0] = c0;
+      pyi[0] = s0;
+      double sb, cb, sd, cd;
+      sincos(0.5 * ky0 * dy, &sb, &cb);
+      sincos(0.5 * dk * dy, &sd, &cd);
+      cby[0] = cb;
+      sby[0] = sb;
+#pragma unroll
+      for (int i = 1; i < kRectT; ++i) {
+        pyr[i] = pyr[i - 1] * cs - pyi[i - 1] * ss;
+        pyi[i] = pyr[i - 1] * ss + pyi[i - 1] * cs;
+        cby[i] = cby[i - 1] * cd - sby[i - 1] * sd;
+        sby[i] = sby[i - 1] * cd + cby[i - 1] * sd;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kRectT; ++i) {
+      const double ky = ky0 + dk * i;
+#pragma unroll
+      for (int j = 0; j < kRectT; ++j) {
+        const double kx = kx0 + dk * j;
+        const double kn = kx * dy - ky * dx;
+        const double u = 0.5 * (kx * dx + ky * dy);
+        const double su = sax[j] * cby[i] + cax[j] * sby[i];
+        // su / u by the hardware reciprocal refined with two Newton steps
+        double r = __builtin_amdgcn_rcp(u);
+        r = fma(r, fma(-u, r, 1.0), r);
+        r = fma(r, fma(-u, r, 1.0), r);
+        const double sc = fabs(u) < 1e-4 ? 1.0 - u * u * (1.0 / 6.0) : su * r;
+        const double a = kn * sc;
+        const double hr = pxr[j] * pyr[i] - pxi[j] * pyi[i];
+        const double hi = pxr[j] * pyi[i] + pxi[j] * pyr[i];
+        ar[i][j] = fma(a, hr, ar[i][j]);
+        ai[i][j] = fma(a, hi, ai[i][j]);
+      }
+    }
   }
   const double nf = (double)n;
   const double scale = nf * nf / (L * L);
-  double t = (double)(mx + my) * (nf - 1.0) / nf;
-  t -= 2.0 * rint(0.5 * t);
-  double sv, cv;
-  sincospi(-t, &sv, &cv);
-  const double vr = scale * (fr * cv - fi * sv), vi = scale * (fr * sv + fi * cv);
-  float2* q = Mq + (size_t)clip * W * W + (size_t)col * W + row;
-  const float2 old = *q;
-  const float2 v = make_float2((float)((double)old.x + vr), (float)((double)old.y + vi));
-  *q = v;
-  if (Mp) {
-    float2* mp = Mp + (size_t)clip * (W + 1) * W;
-    mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
+  float2* mq = Mq + (size_t)clip * W * W;
+  float2* mp = Mp ? Mp + (size_t)clip * (W + 1) * W : nullptr;
+#pragma unroll
+  for (int j = 0; j < kRectT; ++j) {
+    const int col = kRectT * gc + j, mx = mx0 + j;
+    if (col >= W) break;
+#pragma unroll
+    for (int i = 0; i < kRectT; ++i) {
+      const int row = kRectT * gr + i, my = my0 + i;
+      if (row >= W) break;
+      if (mx < -B || mx > B || my < -B || my > B) continue;
+      double fr, fi;
+      if (mx == 0 && my == 0) {
+        fr = pdc[2 * clip];
+        fi = pdc[2 * clip + 1];
+      } else {
+        const double kx = dk * mx, ky = dk * my;
+        const double inv = 1.0 / (kx * kx + ky * ky);  // i / |k|^2
+        fr = -ai[i][j] * inv;
+        fi = ar[i][j] * inv;
+      }
+      double tt = (double)(mx + my) * (nf - 1.0) / nf;
+      tt -= 2.0 * rint(0.5 * tt);
+      double sv, cv;
+      sincospi(-tt, &sv, &cv);
+      const double vr = scale * (fr * cv - fi * sv), vi = scale * (fr * sv + fi * cv);
+      float2* q = mq + (size_t)col * W + row;
+      const float2 old = *q;
+      const float2 v = make_float2((float)((double)old.x + vr), (float)((double)old.y + vi));
+      *q = v;
+      if (mp) mp[((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
+    }
   }
 }
 
@@ -201,7 +281,8 @@ hipError_t launch_poly_band(const double* edges, const int64_t* eoff, const doub
   if (nclips <= 0) return hipSuccess;
   if (B < 0 || BB < B || BB > 1024) return hipErrorInvalidValue;
   const int W = 2 * BB + 1;
-  const int bpc = (W * W + 255) / 256;
+  const int G = (W + kRectT - 1) / kRectT;
+  const int bpc = (G * G + 255) / 256;
   if ((int64_t)bpc * nclips >= (1ll << 31)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_poly_band, dim3((unsigned)(bpc * nclips)), dim3(256), 0, s, edges, eoff,
                      pdc, n, L, B, BB, bpc, Mq, Mp);

@@ -619,6 +619,26 @@ def socs_images_rects_device(ctx, grid, arrays, planes, out):
     return out
 
 
+def socs_images_features_device(ctx, grid, arrays, polygons, planes, out):
+    """Images of masks of rectangles (arrays = rect_arrays(...)) and simple polygons
+    (polygons = PolygonArrays(...)) into out (N, P, n, n) f32 CUDA, on torch's
+    current stream; the call synchronises that stream before staging."""
+    import torch
+    offsets, rects, bg = arrays
+    nclips, n = len(offsets) - 1, grid.size
+    assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
+    if out.shape != (nclips, len(planes), n, n):
+        raise HuvError(2, f"output shape {tuple(out.shape)} != {(nclips, len(planes), n, n)}")
+    if len(polygons.poly_offsets) != nclips + 1:
+        raise HuvError(2, "one polygon list per clip")
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
+    check(ctx.lib.huv_socs_images_features(
+        ctx.handle, nclips, n, grid.pixel_nm, i64ptr(offsets), dptr(rects),
+        C.byref(polygons.struct), dptr(bg), len(planes), arr, C.c_void_p(out.data_ptr()), None))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Engine
 # ---------------------------------------------------------------------------
@@ -782,17 +802,10 @@ class AerialImageEngine:
             bgs.append(b)
         n = self.grid.size
         planes = [self.device_kernel_set(z) for z in defocus_nm]
-        offsets, rects, bg = rect_arrays(clips, bgs)
-        pa = PolygonArrays(polys)
         out = torch.empty((len(clips), len(planes), n, n), dtype=torch.float32,
                           device=f"cuda:{self.device}")
-        ctx = self.context()
-        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
-        arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
-        check(ctx.lib.huv_socs_images_features(
-            ctx.handle, len(clips), n, self.grid.pixel_nm, i64ptr(offsets), dptr(rects),
-            C.byref(pa.struct), dptr(bg), len(planes), arr, C.c_void_p(out.data_ptr()), None))
-        return out
+        return socs_images_features_device(self.context(), self.grid, rect_arrays(clips, bgs),
+                                           PolygonArrays(polys), planes, out)
 
     def compute_from_transmittance(self, transmittance, defocus_nm=0.0):
         t = np.asarray(transmittance)
