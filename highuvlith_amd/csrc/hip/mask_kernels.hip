@@ -13,6 +13,8 @@
 // cell order (FP64 accumulation, one FP32 rounding at the store).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mask_kernels.hpp"
 
 namespace huvk {
@@ -137,107 +139,97 @@ __global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X
 // clip = sum of (value - background) area.  The box bins |kx|, |ky| <= B get
 //   (N^2/L^2) e^{-i pi (mx + my)(N - 1)/N} x integral
 // added to what the rectangle pass stored (plain and column-paired).
-// One thread per tile of kRectT rows x kRectT columns: per edge it evaluates
-// e^{-i kx cx} and the angle kx dx / 2 at its first column and steps them
-// along the tile by angle addition (likewise along the rows), so each edge
-// costs four sine/cosine pairs for kRectT^2 bins; sin(k.d/2) is the sine of
-// the sum of the two half-angles, and sinc switches to its Taylor series
-// below |u| = 1e-4, where that sum would lose relative accuracy.
-__device__ __forceinline__ void cexp_neg(double t, double& c, double& s) {
-  double sv, cv;
-  sincos(t, &sv, &cv);
-  c = cv;
-  s = -sv;
-}
-
-__global__ __launch_bounds__(256) void k_poly_band(const double* __restrict__ edges,
+// wpc work-groups per clip (of up to 512 threads), one thread per tile of
+// kRectT rows x kRectT columns (at most 1024 tiles: band boxes up to 127
+// wide).  The clip's edges
+// are taken in chunks; for each chunk the work-group first tabulates, per edge
+// and order m in [-B, B], w e^{-i kx cx} with (cos, sin)(kx dx / 2) and
+// e^{-i ky cy} with (cos, sin)(ky dy / 2) in LDS, then every tile reads its
+// kRectT columns and rows from there: sin(k.d/2) is the sine of the sum of
+// the two half-angles, sinc switches to its Taylor series below |u| = 1e-4
+// (where that sum loses relative accuracy), and 1/u is the hardware
+// reciprocal refined by two Newton steps.
+__global__ __launch_bounds__(512) void k_poly_band(const double* __restrict__ edges,
                                                    const int64_t* __restrict__ eoff,
                                                    const double* __restrict__ pdc, int n,
-                                                   double L, int B, int BB, int bpc,
+                                                   double L, int B, int BB, int chunk, int wpc,
                                                    float2* __restrict__ Mq,
                                                    float2* __restrict__ Mp) {
-  const int W = 2 * BB + 1;
+  extern __shared__ double4 tab[];  // [axis][edge in chunk][m + B]
+  const int W = 2 * BB + 1, nk = 2 * B + 1;
   const int G = (W + kRectT - 1) / kRectT;
-  const int clip = blockIdx.x / bpc;
-  const int t = (blockIdx.x - clip * bpc) * blockDim.x + threadIdx.x;
-  if (t >= G * G) return;
+  const int clip = blockIdx.x / wpc;
+  const int t = (blockIdx.x - clip * wpc) * blockDim.x + threadIdx.x;  // tile index
   const int64_t e0 = eoff[clip], e1 = eoff[clip + 1];
-  if (e0 == e1) return;
-  const int gc = t / G, gr = t - gc * G;
+  if (e0 == e1) return;  // uniform over the work-group
+  const bool active = t < G * G;
+  const int gc = active ? t / G : 0, gr = active ? t - (t / G) * G : 0;
   const int mx0 = kRectT * gc - BB, my0 = kRectT * gr - BB;
   const double dk = 2.0 * M_PI / L;
-  const double kx0 = dk * mx0, ky0 = dk * my0;
+  int xi[kRectT], yi[kRectT];
+#pragma unroll
+  for (int j = 0; j < kRectT; ++j) {
+    xi[j] = min(max(mx0 + j + B, 0), nk - 1);
+    yi[j] = min(max(my0 + j + B, 0), nk - 1);
+  }
   double ar[kRectT][kRectT], ai[kRectT][kRectT];
 #pragma unroll
   for (int i = 0; i < kRectT; ++i)
 #pragma unroll
     for (int j = 0; j < kRectT; ++j) ar[i][j] = ai[i][j] = 0.0;
-  for (int64_t e = e0; e < e1; ++e) {
-    const double* p = edges + 6 * e;
-    const double dx = p[0], dy = p[1], cx = p[2], cy = p[3], wr = p[4], wi = p[5];
-    // columns: w e^{-i kx cx} and (cos, sin)(kx dx / 2)
-    double pxr[kRectT], pxi[kRectT], cax[kRectT], sax[kRectT];
-    double pyr[kRectT], pyi[kRectT], cby[kRectT], sby[kRectT];
-    {
-      double c0, s0, cs, ss;
-      cexp_neg(kx0 * cx, c0, s0);
-      cexp_neg(dk * cx, cs, ss);
-      pxr[0] = wr * c0 - wi * s0;
-      pxi[0] = wr * s0 + wi * c0;
-      double sa, ca, sd, cd;
-      sincos(0.5 * kx0 * dx, &sa, &ca);
-      sincos(0.5 * dk * dx, &sd, &cd);
-      cax[0] = ca;
-      sax[0] = sa;
-#pragma unroll
-      for (int j = 1; j < kRectT; ++j) {
-        pxr[j] = pxr[j - 1] * cs - pxi[j - 1] * ss;
-        pxi[j] = pxr[j - 1] * ss + pxi[j - 1] * cs;
-        cax[j] = cax[j - 1] * cd - sax[j - 1] * sd;
-        sax[j] = sax[j - 1] * cd + cax[j - 1] * sd;
+  for (int64_t cb = e0; cb < e1; cb += chunk) {
+    const int ne = (int)min((int64_t)chunk, e1 - cb);
+    __syncthreads();  // the previous chunk's tables are consumed
+    for (int q = threadIdx.x; q < 2 * ne * nk; q += blockDim.x) {
+      const int axis = q / (ne * nk), r = q - axis * ne * nk;
+      const int el = r / nk, mi = r - el * nk;
+      const double* p = edges + 6 * (cb + el);
+      const double k = dk * (mi - B);
+      double sv, cv, sh, ch;
+      sincos(k * (axis == 0 ? p[2] : p[3]), &sv, &cv);
+      sincos(0.5 * k * (axis == 0 ? p[0] : p[1]), &sh, &ch);
+      // e^{-i k c}, times the edge weight on the x axis
+      double pr = cv, pi = -sv;
+      if (axis == 0) {
+        const double wr = p[4], wi = p[5];
+        const double tr = wr * pr - wi * pi;
+        pi = wr * pi + wi * pr;
+        pr = tr;
       }
+      tab[(axis * chunk + el) * nk + mi] = make_double4(pr, pi, ch, sh);
     }
-    {
-      double c0, s0, cs, ss;
-      cexp_neg(ky0 * cy, c0, s0);
-      cexp_neg(dk * cy, cs, ss);
-      pyr[0] = c0;
-      pyi[0] = s0;
-      double sb, cb, sd, cd;
-      sincos(0.5 * ky0 * dy, &sb, &cb);
-      sincos(0.5 * dk * dy, &sd, &cd);
-      cby[0] = cb;
-      sby[0] = sb;
+    __syncthreads();
+    if (active) {
+      for (int el = 0; el < ne; ++el) {
+        const double dx = edges[6 * (cb + el)], dy = edges[6 * (cb + el) + 1];
+        double4 yv[kRectT];
 #pragma unroll
-      for (int i = 1; i < kRectT; ++i) {
-        pyr[i] = pyr[i - 1] * cs - pyi[i - 1] * ss;
-        pyi[i] = pyr[i - 1] * ss + pyi[i - 1] * cs;
-        cby[i] = cby[i - 1] * cd - sby[i - 1] * sd;
-        sby[i] = sby[i - 1] * cd + cby[i - 1] * sd;
-      }
-    }
+        for (int i = 0; i < kRectT; ++i) yv[i] = tab[(chunk + el) * nk + yi[i]];
 #pragma unroll
-    for (int i = 0; i < kRectT; ++i) {
-      const double ky = ky0 + dk * i;
+        for (int j = 0; j < kRectT; ++j) {
+          const double4 xv = tab[el * nk + xi[j]];  // one column at a time (register budget)
+          const double kx = dk * (mx0 + j);
 #pragma unroll
-      for (int j = 0; j < kRectT; ++j) {
-        const double kx = kx0 + dk * j;
-        const double kn = kx * dy - ky * dx;
-        const double u = 0.5 * (kx * dx + ky * dy);
-        const double su = sax[j] * cby[i] + cax[j] * sby[i];
-        // su / u by the hardware reciprocal refined with two Newton steps
-        double r = __builtin_amdgcn_rcp(u);
-        r = fma(r, fma(-u, r, 1.0), r);
-        r = fma(r, fma(-u, r, 1.0), r);
-        const double sc = fabs(u) < 1e-4 ? 1.0 - u * u * (1.0 / 6.0) : su * r;
-        const double a = kn * sc;
-        const double hr = pxr[j] * pyr[i] - pxi[j] * pyi[i];
-        const double hi = pxr[j] * pyi[i] + pxi[j] * pyr[i];
-        ar[i][j] = fma(a, hr, ar[i][j]);
-        ai[i][j] = fma(a, hi, ai[i][j]);
+          for (int i = 0; i < kRectT; ++i) {
+            const double ky = dk * (my0 + i);
+            const double kn = kx * dy - ky * dx;
+            const double u = 0.5 * (kx * dx + ky * dy);
+            const double su = xv.w * yv[i].z + xv.z * yv[i].w;
+            double r = __builtin_amdgcn_rcp(u);
+            r = fma(r, fma(-u, r, 1.0), r);
+            r = fma(r, fma(-u, r, 1.0), r);
+            const double sc = fabs(u) < 1e-4 ? 1.0 - u * u * (1.0 / 6.0) : su * r;
+            const double a = kn * sc;
+            const double hr = xv.x * yv[i].x - xv.y * yv[i].y;
+            const double hi = xv.x * yv[i].y + xv.y * yv[i].x;
+            ar[i][j] = fma(a, hr, ar[i][j]);
+            ai[i][j] = fma(a, hi, ai[i][j]);
+          }
+        }
       }
     }
   }
+  if (!active) return;
   const double nf = (double)n;
   const double scale = nf * nf / (L * L);
   float2* mq = Mq + (size_t)clip * W * W;
@@ -279,13 +271,18 @@ hipError_t launch_poly_band(const double* edges, const int64_t* eoff, const doub
                             int nclips, int n, double L, int B, int BB, float2* Mq, float2* Mp,
                             hipStream_t s) {
   if (nclips <= 0) return hipSuccess;
-  if (B < 0 || BB < B || BB > 1024) return hipErrorInvalidValue;
-  const int W = 2 * BB + 1;
+  if (B < 0 || BB < B) return hipErrorInvalidValue;
+  const int W = 2 * BB + 1, nk = 2 * B + 1;
   const int G = (W + kRectT - 1) / kRectT;
-  const int bpc = (G * G + 255) / 256;
-  if ((int64_t)bpc * nclips >= (1ll << 31)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_poly_band, dim3((unsigned)(bpc * nclips)), dim3(256), 0, s, edges, eoff,
-                     pdc, n, L, B, BB, bpc, Mq, Mp);
+  if (G * G > 1024) return hipErrorInvalidValue;  // band boxes up to 127 wide
+  const int wpc = (G * G + 511) / 512;
+  const int threads = ((((G * G + wpc - 1) / wpc) + 63) / 64) * 64;
+  if ((int64_t)wpc * nclips >= (1ll << 31)) return hipErrorInvalidValue;
+  // edges per chunk: two tables of nk 32-byte entries per edge within 48 KiB of LDS
+  const int chunk = std::max(1, (48 * 1024) / (2 * nk * (int)sizeof(double4)));
+  const size_t lds = (size_t)2 * chunk * nk * sizeof(double4);
+  hipLaunchKernelGGL(k_poly_band, dim3((unsigned)(wpc * nclips)), dim3(threads), lds, s, edges,
+                     eoff, pdc, n, L, B, BB, chunk, wpc, Mq, Mp);
   return hipGetLastError();
 }
 
