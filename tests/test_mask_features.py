@@ -44,3 +44,22 @@ def test_many_periods_become_the_mean_and_unknown_kinds_are_errors():
     assert len(rects) == 1 and abs(rects[0][4] - 0.5) < 1e-12 and rects[0][2] == 256.0
     with pytest.raises(H.HuvError):
         E.mask_feature_rects([("polygon", [(0, 0), (1, 0), (0, 1)])], g)
+
+
+def test_feature_lists_split_polygons_and_pack_offsets():
+    """Host packing for the C ABI: polygons are split from the rectangle-type
+    features in order, and the polygon / vertex offsets index the flat arrays."""
+    g = H.GridConfig(64, 4.0)
+    feats = [("rect", 0.0, 0.0, 10.0, 10.0), ("polygon", [(0, 0), (5, 0), (0, 5)]),
+             ("lines", 30.0, 64.0, 0, 5.0), ("polygon", [(1, 1), (9, 1), (9, 9), (1, 9)])]
+    rects, polys, bg = E.mask_feature_lists(feats, g, dark_field=True, absorber=0.1j)
+    assert bg == 0.1j and len(polys) == 2
+    assert [len(v) for v, _ in polys] == [3, 4] and all(val == 1.0 for _, val in polys)
+    assert len(rects) == 1 + 5  # the rectangle and the five clipped line pieces
+    pa = E.PolygonArrays([polys, [], [polys[1]]])
+    assert list(pa.poly_offsets) == [0, 2, 2, 3]
+    assert list(pa.vertex_offsets) == [0, 3, 7, 11]
+    assert pa.vertices.shape == (11, 2) and pa.values.shape == (3, 2)
+    assert np.array_equal(pa.vertices[7:11], np.asarray(polys[1][0], float))
+    empty = E.PolygonArrays([[], []])
+    assert list(empty.poly_offsets) == [0, 0, 0] and list(empty.vertex_offsets) == [0]
