@@ -252,6 +252,11 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* mas
 // mask.rs:95, MAX_CELLS) and the host staging of all clips.
 constexpr int64_t kRectMaxGrid = 1 << 20;
 constexpr int64_t kRectMaxCells = 1ll << 26;
+// Polygons: the crossing test is quadratic in the vertex count, so one polygon
+// holds at most kPolyMaxVertices vertices; one call stages at most
+// kPolyMaxEdges clipped edges.
+constexpr int64_t kPolyMaxVertices = 8192;
+constexpr int64_t kPolyMaxEdges = 1ll << 26;
 
 huv_status validate_rects(int nclips, int n, const RectBatch& rb) {
   if (!rb.offsets || !rb.background) return fail(HUV_INVALID_PARAMETER, "null rectangle argument");
@@ -281,9 +286,12 @@ huv_status validate_rects(int nclips, int n, const RectBatch& rb) {
       return fail(HUV_INVALID_PARAMETER, "null polygon arrays");
     if (np > 0 && P.vertex_offsets[0] != 0)
       return fail(HUV_INVALID_PARAMETER, "vertex offsets must start at 0");
-    for (int64_t j = 0; j < np; ++j)
+    for (int64_t j = 0; j < np; ++j) {
       if (P.vertex_offsets[j + 1] < P.vertex_offsets[j])
         return fail(HUV_INVALID_PARAMETER, "vertex offsets must be non-decreasing");
+      if (P.vertex_offsets[j + 1] - P.vertex_offsets[j] > kPolyMaxVertices)
+        return fail(HUV_INVALID_PARAMETER, "a polygon has more than 8192 vertices");
+    }
     const int64_t nv = np > 0 ? P.vertex_offsets[np] : 0;
     if (nv > 0 && !P.vertices) return fail(HUV_INVALID_PARAMETER, "null polygon vertices");
     for (int64_t k = 0; k < 2 * nv; ++k)
@@ -413,6 +421,7 @@ huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectB
       }
     }
     eoff[i + 1] = (int64_t)(edges.size() / 6);
+    if (eoff[i + 1] > kPolyMaxEdges) return fail(HUV_INVALID_PARAMETER, "too many polygon edges in one call");
   }
   const int64_t ncells = coff[nclips];
   const size_t cell_b = cells.size() * sizeof(double);

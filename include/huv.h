@@ -259,7 +259,8 @@ huv_status huv_band_spectrum_rects(huv_ctx* ctx, int nclips, int n, double pixel
  * either orientation) and complex amplitude values[2 j], values[2 j + 1].
  * A polygon is clipped to the field; one whose edges cross, or whose bounding
  * box meets another feature's of the same clip, is rejected (the reference
- * rasterises those, mask.rs:1190-1243). */
+ * rasterises those, mask.rs:1190-1243), as is a polygon of more than 8192
+ * vertices or a call of more than 2^26 clipped edges. */
 typedef struct {
   const int64_t* poly_offsets;
   const int64_t* vertex_offsets;

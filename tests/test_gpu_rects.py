@@ -265,3 +265,7 @@ def test_crossing_or_overlapping_polygons_are_rejected():
     with pytest.raises(H.HuvError):
         H.engine.band_spectrum_features_device(ctx, [[(0.0, 0.0, 20.0, 20.0, 1.0)]],
                                                [[(tri, 1.0)]], grid, 10)
+    circle = [(60.0 * np.cos(a), 60.0 * np.sin(a)) for a in np.linspace(0, 2 * np.pi, 9000,
+                                                                           endpoint=False)]
+    with pytest.raises(H.HuvError):  # more than 8192 vertices
+        H.engine.band_spectrum_features_device(ctx, [[]], [[(circle, 1.0)]], grid, 10)
