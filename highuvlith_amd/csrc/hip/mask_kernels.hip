@@ -1,4 +1,5 @@
-// Exact mask spectrum of rectangle masks, evaluated on the device.
+// Exact mask spectrum of masks given as rectangles and simple polygons,
+// evaluated on the device (polygons: see k_poly_band below).
 //
 // The spectrum convention is the one of mask.rs:31-45 (crates/highuvlith-core):
 // for the field [-L/2, L/2)^2, L = N p, and the signed order m,

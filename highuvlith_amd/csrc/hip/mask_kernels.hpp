@@ -1,5 +1,5 @@
-// Exact band spectrum of rectangle masks on the device (FP64 evaluation,
-// FP32 band boxes in the layout the field kernels read).
+// Exact band spectrum of masks of rectangles and simple polygons on the device
+// (FP64 evaluation, FP32 band boxes in the layout the field kernels read).
 #pragma once
 
 #include <hip/hip_runtime.h>
