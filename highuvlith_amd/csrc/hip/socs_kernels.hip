@@ -658,14 +658,16 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     for (int h = 0; h < F; ++h) d[h] = cmul(d[h], twh[h * C + c]);
     fft_reg<+1, F>(d);
     // Rows y = P (2 pr) + p and y + P, columns x = c + C i: vector buffer
-    // stores with the column offset C i in an SGPR.
+    // stores with the column offset C i in an SGPR, marked non-temporal (cache
+    // policy bit 1): the images stream out without evicting the next step's
+    // inputs from L2 and the MALL.
     const int va = ((P * (2 * pr) + p) * n + c) * (int)sizeof(float);
     const int vb = va + P * n * (int)sizeof(float);
 #pragma unroll
     for (int i = 0; i < F; ++i) {
       const int so = C * i * (int)sizeof(float);
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].re), orc, va, so, 0);
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].im), orc, vb, so, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].re), orc, va, so, 2);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].im), orc, vb, so, 2);
     }
   }
 }
