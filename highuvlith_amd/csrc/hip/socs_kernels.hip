@@ -28,7 +28,9 @@ namespace huvk {
 // ---------------------------------------------------------------------------
 // rows_band: two real rows packed as z = a + i b, FFT_n forward, keep bins
 // k in [0, B] and [n - B, n).  tmp layout: [clip][kk][pair], kk in [0, 2B],
-// kk <= B -> k = kk, kk > B -> k = n - (kk - B).
+// kk <= B -> k = kk, kk > B -> k = n - (kk - B).  The masks are read once, as
+// non-temporal loads, so they do not evict the row spectra that cols_band
+// reads next.
 // ---------------------------------------------------------------------------
 template <int N1, int N2, int TEAMS>
 __global__ __launch_bounds__(TEAMS * N2) void k_rows_band(const float* __restrict__ masks, int n,
@@ -47,7 +49,10 @@ __global__ __launch_bounds__(TEAMS * N2) void k_rows_band(const float* __restric
   float2* out = tmp + (size_t)clip * nk * npairs;
   fft_four_step<-1, N1, N2>(
       t, scratch[team], tw_n,
-      [&](int i) { return active ? c32{ra[i], rb[i]} : c32{0.f, 0.f}; },
+      [&](int i) {
+        return active ? c32{__builtin_nontemporal_load(ra + i), __builtin_nontemporal_load(rb + i)}
+                      : c32{0.f, 0.f};
+      },
       [&](int k, c32 v) {
         if (!active) return;
         int kk = -1;
