@@ -10,6 +10,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhuv.so")
 
 HUV_OK = 0
+HUV_INVALID_PARAMETER = 1
+HUV_UNSUPPORTED = 8
 STATUS_NAMES = {
     1: "InvalidParameter",
     2: "DimensionMismatch",

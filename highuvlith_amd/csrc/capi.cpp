@@ -65,10 +65,43 @@ void Workspace::release() {
   }
 }
 
+void* Ctx::buf(int slot, size_t bytes, hipError_t& err) {
+  if (!fixed_mode) return ws.get(slot, bytes, err);
+  err = hipSuccess;
+  const size_t need = (bytes + 255) / 256 * 256;
+  if (fixed_used + need > fixed_cap) {
+    err = hipErrorOutOfMemory;
+    return nullptr;
+  }
+  void* p = fixed + fixed_used;
+  fixed_used += need;
+  return p;
+}
+
+SocsSizes socs_sizes(int nclips, int n, int B, int m, bool paired, int nplanes, int S,
+                     bool raster) {
+  auto r = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t W = 2 * (size_t)huvk::coarse_band(m) + 1;
+  const size_t nimg = (size_t)nclips * nplanes;
+  const size_t G = 2 * (size_t)B;
+  SocsSizes z;
+  if (raster) z.tmp_rows = r((size_t)nclips * (2 * B + 1) * (n / 2) * sizeof(float2));
+  z.band = r((size_t)nclips * W * W * sizeof(float2));
+  if (paired) z.band_paired = r((size_t)nclips * (W + 1) * W * sizeof(float2));
+  z.coarse = r((size_t)S * nimg * m * m * sizeof(float));
+  z.tmp_coarse = r(nimg * (2 * G + 1) * (m / 2) * sizeof(float2));
+  z.iband = r(nimg * (G + 1) * (2 * G + 1) * sizeof(float2));
+  return z;
+}
+
 const float2* Ctx::twiddles(int N, hipError_t& err) {
   err = hipSuccess;
   auto it = tw.find(N);
   if (it != tw.end()) return it->second;
+  if (fixed_mode) {  // no-allocation calls only read tables built earlier
+    err = hipErrorNotReady;
+    return nullptr;
+  }
   std::vector<float2> h((size_t)N);
   for (int k = 0; k < N; ++k) {
     const double a = -2.0 * M_PI * (double)k / (double)N;
@@ -86,11 +119,19 @@ const float2* Ctx::twiddles(int N, hipError_t& err) {
   return d;
 }
 
+static long sinc_key(int n, int B, bool pixel_transfer) {
+  return ((long)n << 20) | ((long)B << 1) | (pixel_transfer ? 1 : 0);
+}
+
 const float* Ctx::sinc_table(int n, int B, bool pixel_transfer, hipError_t& err) {
   err = hipSuccess;
-  const long key = ((long)n << 20) | ((long)B << 1) | (pixel_transfer ? 1 : 0);
+  const long key = sinc_key(n, B, pixel_transfer);
   auto it = sinc.find(key);
   if (it != sinc.end()) return it->second;
+  if (fixed_mode) {
+    err = hipErrorNotReady;
+    return nullptr;
+  }
   const int W = 2 * B + 1;
   std::vector<float> h((size_t)W);
   for (int k = -B; k <= B; ++k) {
@@ -233,7 +274,7 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* mas
   const float* st = c->sinc_table(n, B, pixel_transfer, e);
   if (!st) return hip_fail(e, "sinc table");
   const size_t tmp_bytes = (size_t)nclips * (2 * B + 1) * (n / 2) * sizeof(float2);
-  float2* tmp = (float2*)c->ws.get(Workspace::TMP_ROWS, tmp_bytes, e);
+  float2* tmp = (float2*)c->buf(Workspace::TMP_ROWS, tmp_bytes, e);
   if (!tmp) return hip_fail(e, "workspace");
   hipEvent_t ev;
   c->stage_begin(ST_ROWS_BAND, &ev);
@@ -358,6 +399,11 @@ huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectB
   std::vector<huv::RectFeature> fs;
   std::vector<huv::RectCell> cl;
   const double half = n * rb.pixel_nm / 2.0;
+  // overlaps of at most 1e-9 of the field length count as none (mask.rs:838-841,1099-1108)
+  const double eps = 1e-9 * (2.0 * half);
+  if (rb.polys && B > huvk::kPolyMaxBand)
+    return fail(HUV_UNSUPPORTED, "polygon masks support band half-widths up to " +
+                                     std::to_string(huvk::kPolyMaxBand));
   for (int i = 0; i < nclips; ++i) {
     fs.clear();
     for (int64_t r = rb.offsets[i]; r < rb.offsets[i + 1]; ++r) {
@@ -402,7 +448,8 @@ huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectB
         }
         if (twice == 0.0) continue;
         for (const auto& o : boxes)
-          if (std::min(bb[1], o[1]) > std::max(bb[0], o[0]) && std::min(bb[3], o[3]) > std::max(bb[2], o[2]))
+          if (std::min(bb[1], o[1]) - std::max(bb[0], o[0]) > eps &&
+              std::min(bb[3], o[3]) - std::max(bb[2], o[2]) > eps)
             return fail(HUV_INVALID_PARAMETER, "clip " + std::to_string(i) + ", polygon " +
                                                    std::to_string(j - P.poly_offsets[i]) +
                                                    ": overlaps another feature; rasterise the clip instead");
@@ -461,21 +508,23 @@ huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectB
   }
   const double L = (double)n * rb.pixel_nm;
   hipEvent_t ev;
-  c->stage_begin(ST_ROWS_BAND, &ev);
+  c->stage_begin(ST_RECT_TABLES, &ev);
   e = huvk::launch_rect_tables(d_cells, ncells, n, L, B, X, Y, c->stream());
   if (e != hipSuccess) return hip_fail(e, "rect_tables launch");
-  c->stage_end(ST_ROWS_BAND, ev);
-  c->stage_begin(ST_COLS_BAND, &ev);
+  c->stage_end(ST_RECT_TABLES, ev);
+  c->stage_begin(ST_RECT_BAND, &ev);
   e = huvk::launch_rect_band(X, Y, d_coff, d_bg, nclips, n, B, BB, Mq, Mp, c->stream());
   if (e != hipSuccess) return hip_fail(e, "rect_band launch");
+  c->stage_end(ST_RECT_BAND, ev);
   if (nedges > 0) {
     const char* pd = c->pin_d + cell_b + off_b + bg_b;
+    c->stage_begin(ST_POLY_BAND, &ev);
     e = huvk::launch_poly_band((const double*)pd, (const int64_t*)(pd + edge_b),
                                (const double*)(pd + edge_b + eoff_b), nclips, n, L, B, BB, Mq, Mp,
                                c->stream());
     if (e != hipSuccess) return hip_fail(e, "poly_band launch");
+    c->stage_end(ST_POLY_BAND, ev);
   }
-  c->stage_end(ST_COLS_BAND, ev);
   return HUV_OK;
 }
 }  // namespace huvi
@@ -864,6 +913,10 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
   key.push_back((uint64_t)m);
   key.push_back((uint64_t)paired);
   auto it = c->plane_packs.find(key);
+  if (it == c->plane_packs.end() && c->fixed_mode)
+    return fail(HUV_INVALID_PARAMETER,
+                "kernel stacks of this plane list are not built yet: call huv_socs_images once "
+                "with the same kernel sets before the no-allocation entry point");
   if (it == c->plane_packs.end()) {
     Ctx::PlanePack pk;
     for (int p = 0; p < nplanes; ++p) {
@@ -921,7 +974,7 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
 // Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
                         huv_kset* const* planes, float* out, bool pt, float2** Mq_out,
-                        const huv_socs_tuning* tune, const RectBatch* rects) {
+                        const huv_socs_tuning* tune, const RectBatch* rects, int max_splits) {
   const int B = planes[0]->B;
   const int m = tune && tune->m ? tune->m : planes[0]->m;
   const int BB = huvk::coarse_band(m);
@@ -934,13 +987,13 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   hipError_t e;
   const int W = 2 * BB + 1;
   const int nimg = nclips * nplanes;
-  float2* Mq = (float2*)c->ws.get(Workspace::BAND, (size_t)nclips * W * W * sizeof(float2), e);
+  float2* Mq = (float2*)c->buf(Workspace::BAND, (size_t)nclips * W * W * sizeof(float2), e);
   if (!Mq) return hip_fail(e, "workspace");
   if (Mq_out) *Mq_out = Mq;
   // the two-lane field kernel reads the column-paired copy of the band
   float2* Mp = nullptr;
   if (xlanes == 2) {
-    Mp = (float2*)c->ws.get(Workspace::BAND_PAIRED, (size_t)nclips * (W + 1) * W * sizeof(float2), e);
+    Mp = (float2*)c->buf(Workspace::BAND_PAIRED, (size_t)nclips * (W + 1) * W * sizeof(float2), e);
     if (!Mp) return hip_fail(e, "workspace");
   }
   huv_status st = rects ? rect_band_stage(c, nclips, n, B, BB, *rects, Mq, Mp)
@@ -951,8 +1004,13 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   int Kmax;
   st = plane_pack(c, nplanes, planes, m, xlanes == 2, &kern, &lam, &Kmax);
   if (st != HUV_OK) return st;
-  const int S = huvk::fields_splits(m, nclips, Kmax);
-  float* coarse = (float*)c->ws.get(Workspace::COARSE, (size_t)S * nimg * m * m * sizeof(float), e);
+  int S = huvk::fields_splits(m, nclips, Kmax);
+  if (max_splits > 0 && S > max_splits) {
+    // every split must own at least one kernel (its partials are summed)
+    const int ks = (Kmax + max_splits - 1) / max_splits;
+    S = (Kmax + ks - 1) / ks;
+  }
+  float* coarse = (float*)c->buf(Workspace::COARSE, (size_t)S * nimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");
   const float2* tw_m = c->twiddles(m, e);
   if (!tw_m) return hip_fail(e, "twiddle table");
@@ -965,9 +1023,9 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
   const float2* tw_n = c->twiddles(n, e);
   if (!tw_n) return hip_fail(e, "twiddle table");
   const int G = 2 * B;
-  float2* tmp1 = (float2*)c->ws.get(Workspace::TMP_COARSE, (size_t)nimg * (2 * G + 1) * (m / 2) * sizeof(float2), e);
+  float2* tmp1 = (float2*)c->buf(Workspace::TMP_COARSE, (size_t)nimg * (2 * G + 1) * (m / 2) * sizeof(float2), e);
   if (!tmp1) return hip_fail(e, "workspace");
-  float2* Ib = (float2*)c->ws.get(Workspace::IBAND, (size_t)nimg * (G + 1) * (2 * G + 1) * sizeof(float2), e);
+  float2* Ib = (float2*)c->buf(Workspace::IBAND, (size_t)nimg * (G + 1) * (2 * G + 1) * sizeof(float2), e);
   if (!Ib) return hip_fail(e, "workspace");
   c->stage_begin(ST_COARSE_ROWS, &ev);
   e = huvk::launch_coarse_rows(m, B, nimg, S, coarse, tw_m, tmp1, c->stream());
@@ -1349,6 +1407,9 @@ huv_status huv_band_spectrum_features(huv_ctx* c, int nclips, int n, double pixe
   if (!c || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
   if (nclips < 0) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0");
   if (band < 0 || 2 * band + 1 > n) return fail(HUV_INVALID_PARAMETER, "band out of range");
+  if (polys && band > huvk::kPolyMaxBand)
+    return fail(HUV_UNSUPPORTED, "polygon masks support band half-widths up to " +
+                                     std::to_string(huvk::kPolyMaxBand));
   RectBatch rb{pixel_nm, offsets, rects, background};
   rb.polys = polys;
   huv_status st = validate_rects(nclips, n, rb);

@@ -77,11 +77,31 @@ struct Ctx {
   hipStream_t stream() const { return active; }
   const float2* twiddles(int N, hipError_t& err);
   const float* sinc_table(int n, int B, bool pixel_transfer, hipError_t& err);
+  // This device's fixed scratch array (resolved once per device at context
+  // creation) and, while a no-allocation call runs, the part of it handed out
+  // so far: workspace requests are then carved from it in order instead of
+  // growing the workspace.
+  char* fixed = nullptr;
+  size_t fixed_cap = 0;
+  bool fixed_mode = false;
+  size_t fixed_used = 0;
+  void* buf(int slot, size_t bytes, hipError_t& err);
   ~Ctx();
 };
 
+// Device bytes of one forward call's workspace slots (socs_forward), each
+// rounded up to 256 bytes: band-FFT row spectra, band box, its column-paired
+// copy (two x lanes), S coarse-image partials, coarse row band, intensity band.
+struct SocsSizes {
+  size_t tmp_rows = 0, band = 0, band_paired = 0, coarse = 0, tmp_coarse = 0, iband = 0;
+  size_t total() const { return tmp_rows + band + band_paired + coarse + tmp_coarse + iband; }
+};
+SocsSizes socs_sizes(int nclips, int n, int B, int m, bool paired, int nplanes, int S,
+                     bool raster);
+
 enum Stage { ST_ROWS_BAND = 0, ST_COLS_BAND, ST_FIELDS, ST_COARSE_ROWS, ST_COARSE_COLS,
-             ST_BAND_IMAGE, ST_ILT_POINTWISE, ST_ILT_ADJOINT, ST_ILT_OTHER, ST_COUNT };
+             ST_BAND_IMAGE, ST_ILT_POINTWISE, ST_ILT_ADJOINT, ST_ILT_OTHER, ST_RECT_TABLES,
+             ST_RECT_BAND, ST_POLY_BAND, ST_COUNT };
 
 // Band spectrum of nclips real n x n masks: bins |k| <= B computed, stored
 // in boxes of half-width BB >= B (zero beyond B).  Layout Mq[clip][kx + BB][ky + BB];
@@ -117,7 +137,8 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
 // `masks` is not read.
 huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
                         huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out,
-                        const huv_socs_tuning* tune = nullptr, const RectBatch* rects = nullptr);
+                        const huv_socs_tuning* tune = nullptr, const RectBatch* rects = nullptr,
+                        int max_splits = 0);
 
 }  // namespace huvi
 

@@ -141,8 +141,7 @@ __global__ __launch_bounds__(256) void k_rect_band(const double2* __restrict__ X
 //   (N^2/L^2) e^{-i pi (mx + my)(N - 1)/N} x integral
 // added to what the rectangle pass stored (plain and column-paired).
 // wpc work-groups per clip (of up to 512 threads), one thread per tile of
-// kRectT rows x kRectT columns (at most 1024 tiles: band boxes up to 127
-// wide).  The clip's edges
+// kRectT rows x kRectT columns.  The clip's edges
 // are taken in chunks; for each chunk the work-group first tabulates, per edge
 // and order m in [-B, B], w e^{-i kx cx} with (cos, sin)(kx dx / 2) and
 // e^{-i ky cy} with (cos, sin)(ky dy / 2) in LDS, then every tile reads its
@@ -272,10 +271,9 @@ hipError_t launch_poly_band(const double* edges, const int64_t* eoff, const doub
                             int nclips, int n, double L, int B, int BB, float2* Mq, float2* Mp,
                             hipStream_t s) {
   if (nclips <= 0) return hipSuccess;
-  if (B < 0 || BB < B) return hipErrorInvalidValue;
+  if (B < 0 || BB < B || B > kPolyMaxBand || BB > 1024) return hipErrorInvalidValue;
   const int W = 2 * BB + 1, nk = 2 * B + 1;
   const int G = (W + kRectT - 1) / kRectT;
-  if (G * G > 1024) return hipErrorInvalidValue;  // band boxes up to 127 wide
   const int wpc = (G * G + 511) / 512;
   const int threads = ((((G * G + wpc - 1) / wpc) + 63) / 64) * 64;
   if ((int64_t)wpc * nclips >= (1ll << 31)) return hipErrorInvalidValue;

@@ -18,6 +18,10 @@ hipError_t launch_rect_band(const double2* X, const double2* Y, const int64_t* c
                             const double* bg, int nclips, int n, int B, int BB, float2* Mq,
                             float2* Mp, hipStream_t s);
 
+// Largest band half-width of the polygon pass: one edge's two order tables
+// (2 x (2B + 1) entries of 32 bytes) must fit 64 KiB of LDS.
+constexpr int kPolyMaxBand = 511;
+
 // Simple polygons added onto the box written by launch_rect_band.  edges:
 // (dx, dy, cx, cy, re, im) per edge, (re, im) = orientation sign x (value -
 // background); eoff: nclips + 1 edge offsets; pdc: per clip sum of

@@ -314,7 +314,8 @@ class DeviceContext:
         check(self.lib.huv_ctx_synchronize(self.handle))
 
     STAGES = ("rows_band", "cols_band", "socs_fields", "coarse_rows", "coarse_cols",
-              "band_image", "ilt_pointwise", "ilt_adjoint", "ilt_other")
+              "band_image", "ilt_pointwise", "ilt_adjoint", "ilt_other", "rect_tables",
+              "rect_band", "poly_band")
 
     def set_profiling(self, on=True):
         check(self.lib.huv_ctx_set_profiling(self.handle, int(bool(on))))
@@ -487,7 +488,13 @@ def mask_feature_rects(features, grid, dark_field=False, absorber=0j):
             rects.append((x, y, w, h, fg))
         elif kind == "gray":
             x, y, w, h, t = map(float, f[1:6])
-            rects.append((x, y, w, h, math.sqrt(max(t, 0.0))))
+            # mask.rs:360 rejects a transmittance outside [0, 1] when the mask is
+            # built; the spectrum drops a non-finite one and clamps (mask.rs:1009-1012)
+            if not math.isfinite(t):
+                continue
+            if not 0.0 <= t <= 1.0:
+                raise HuvError(1, f"gray transmittance {t} must lie in [0, 1]")
+            rects.append((x, y, w, h, math.sqrt(t)))
         elif kind == "lines":
             cd, pitch, orient, offset = float(f[1]), float(f[2]), int(f[3]), float(f[4])
             ivs, frac = _grating_intervals(cd, pitch, offset, L)
@@ -576,6 +583,8 @@ def band_spectrum_features_device(ctx, clips, poly_clips, grid, band, background
     """Exact band spectrum of masks of rectangles (clips, as for rect_arrays) and simple
     polygons (poly_clips, as for PolygonArrays); torch complex64 (N, W, W)."""
     import torch
+    if len(poly_clips) != len(clips):
+        raise HuvError(2, "one polygon list per clip")
     offsets, rects, bg = rect_arrays(clips, backgrounds)
     pa = PolygonArrays(poly_clips)
     nclips, W = len(clips), 2 * band + 1

@@ -150,7 +150,8 @@ huv_status huv_ctx_synchronize(huv_ctx* ctx);
 /* Per-stage timing with HIP events recorded on the active stream around each
  * kernel launch (off by default).  Stages: 0 rows_band, 1 cols_band,
  * 2 socs_fields, 3 coarse_rows, 4 coarse_cols, 5 band_image, 6 ilt_pointwise,
- * 7 ilt_adjoint, 8 ilt_other.  huv_ctx_stage_times synchronises, returns the
+ * 7 ilt_adjoint, 8 ilt_other, 9 rect_tables, 10 rect_band, 11 poly_band (the
+ * last three: feature-list masks).  huv_ctx_stage_times synchronises, returns the
  * accumulated milliseconds and launch counts, and optionally resets them. */
 huv_status huv_ctx_set_profiling(huv_ctx* ctx, int on);
 huv_status huv_ctx_stage_times(huv_ctx* ctx, double* ms, int64_t* calls, int nstages, int reset);
@@ -259,8 +260,11 @@ huv_status huv_band_spectrum_rects(huv_ctx* ctx, int nclips, int n, double pixel
  * either orientation) and complex amplitude values[2 j], values[2 j + 1].
  * A polygon is clipped to the field; one whose edges cross, or whose bounding
  * box meets another feature's of the same clip, is rejected (the reference
- * rasterises those, mask.rs:1190-1243), as is a polygon of more than 8192
- * vertices or a call of more than 2^26 clipped edges. */
+ * rasterises those, mask.rs:1190-1243; boxes that overlap by at most 1e-9 of
+ * the field length count as disjoint, mask.rs:838-841,1099-1108), as is a
+ * polygon of more than 8192 vertices or a call of more than 2^26 clipped
+ * edges.  With polygons the band half-width is at most 511 (HUV_UNSUPPORTED
+ * beyond). */
 typedef struct {
   const int64_t* poly_offsets;
   const int64_t* vertex_offsets;

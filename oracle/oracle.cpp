@@ -49,10 +49,13 @@ static const double PI = 3.14159265358979323846;
 // ---------------------------------------------------------------------------
 namespace {
 
+// Radix-4 Stockham (autosort) transform with one radix-2 stage when log2 n
+// is odd; twiddles tabulated per stage in the plan.
 struct Plan1D {
-  int n = 0, logn = 0;
-  std::vector<int> rev;
-  std::vector<cd> tw;  // exp(-2 pi i k / n), k < n/2
+  int n = 0;
+  // stage s works on sub-length L_s = n / 4^s (radix 4) or 2 (the last
+  // radix-2 stage); tw[s] holds (w^p, w^2p, w^3p), p < L_s / 4, w = e^{-2 pi i / L_s}
+  std::vector<std::vector<cd>> tw;
 };
 
 std::mutex g_plan_mu;
@@ -64,36 +67,63 @@ std::shared_ptr<Plan1D> plan_for(int n) {
   if (it != g_plans.end()) return it->second;
   auto p = std::make_shared<Plan1D>();
   p->n = n;
-  while ((1 << p->logn) < n) ++p->logn;
-  p->rev.resize(n);
-  for (int i = 0; i < n; ++i) {
-    int r = 0;
-    for (int b = 0; b < p->logn; ++b)
-      if (i & (1 << b)) r |= 1 << (p->logn - 1 - b);
-    p->rev[i] = r;
+  for (int L = n; L >= 4; L /= 4) {
+    std::vector<cd> t((size_t)3 * (L / 4));
+    for (int q = 0; q < L / 4; ++q) {
+      t[3 * q] = std::polar(1.0, -2.0 * PI * q / L);
+      t[3 * q + 1] = std::polar(1.0, -2.0 * PI * 2.0 * q / L);
+      t[3 * q + 2] = std::polar(1.0, -2.0 * PI * 3.0 * q / L);
+    }
+    p->tw.push_back(std::move(t));
   }
-  p->tw.resize(std::max(1, n / 2));
-  for (int k = 0; k < n / 2; ++k) p->tw[k] = std::polar(1.0, -2.0 * PI * k / n);
   g_plans[n] = p;
   return p;
 }
 
-// In-place unnormalised 1-D DFT; inverse uses conjugate twiddles (no scale).
-void fft1d(cd* x, const Plan1D& p, bool inverse) {
+// In-place unnormalised 1-D DFT (n a power of two); inverse uses conjugate
+// twiddles (no scale).  work: n scratch values.
+void fft1d(cd* x, const Plan1D& p, bool inverse, cd* work) {
   const int n = p.n;
-  for (int i = 0; i < n; ++i)
-    if (i < p.rev[i]) std::swap(x[i], x[p.rev[i]]);
-  for (int len = 2; len <= n; len <<= 1) {
-    const int half = len >> 1, stride = n / len;
-    for (int i = 0; i < n; i += len)
-      for (int j = 0; j < half; ++j) {
-        cd w = p.tw[j * stride];
-        if (inverse) w = std::conj(w);
-        const cd u = x[i + j], v = x[i + j + half] * w;
-        x[i + j] = u + v;
-        x[i + j + half] = u - v;
+  if (n <= 1) return;
+  cd* src = x;
+  cd* dst = work;
+  int L = n, s = 1, stage = 0;
+  while (L >= 4) {
+    const int q4 = L / 4;
+    const cd* tw = p.tw[stage].data();
+    for (int q = 0; q < q4; ++q) {
+      cd w1 = tw[3 * q], w2 = tw[3 * q + 1], w3 = tw[3 * q + 2];
+      if (inverse) {
+        w1 = std::conj(w1);
+        w2 = std::conj(w2);
+        w3 = std::conj(w3);
       }
+      for (int r = 0; r < s; ++r) {
+        const cd a = src[r + s * q], b = src[r + s * (q + q4)];
+        const cd c = src[r + s * (q + 2 * q4)], d = src[r + s * (q + 3 * q4)];
+        const cd apc = a + c, amc = a - c, bpd = b + d, bmd = b - d;
+        // multiply (b - d) by -i (forward) or +i (inverse)
+        const cd jbmd = inverse ? cd(-bmd.imag(), bmd.real()) : cd(bmd.imag(), -bmd.real());
+        dst[r + s * (4 * q)] = apc + bpd;
+        dst[r + s * (4 * q + 1)] = w1 * (amc + jbmd);
+        dst[r + s * (4 * q + 2)] = w2 * (apc - bpd);
+        dst[r + s * (4 * q + 3)] = w3 * (amc - jbmd);
+      }
+    }
+    std::swap(src, dst);
+    L /= 4;
+    s *= 4;
+    ++stage;
   }
+  if (L == 2) {
+    for (int r = 0; r < s; ++r) {
+      const cd a = src[r], b = src[r + s];
+      dst[r] = a + b;
+      dst[r + s] = a - b;
+    }
+    std::swap(src, dst);
+  }
+  if (src != x) std::copy(src, src + n, x);
 }
 
 void transpose_into(const cd* src, cd* dst, int rows, int cols) {
@@ -109,14 +139,15 @@ void transpose_into(const cd* src, cd* dst, int rows, int cols) {
 void transform2d(cd* data, int ny, int nx, bool inverse, const int64_t* rows, int64_t nrows,
                  std::vector<cd>& tmp) {
   auto px = plan_for(nx), py = plan_for(ny);
+  std::vector<cd> work((size_t)std::max(nx, ny));
   if (!rows) {
-    for (int r = 0; r < ny; ++r) fft1d(data + (size_t)r * nx, *px, inverse);
+    for (int r = 0; r < ny; ++r) fft1d(data + (size_t)r * nx, *px, inverse, work.data());
   } else {
-    for (int64_t k = 0; k < nrows; ++k) fft1d(data + (size_t)rows[k] * nx, *px, inverse);
+    for (int64_t k = 0; k < nrows; ++k) fft1d(data + (size_t)rows[k] * nx, *px, inverse, work.data());
   }
   tmp.resize((size_t)nx * ny);
   transpose_into(data, tmp.data(), ny, nx);
-  for (int c = 0; c < nx; ++c) fft1d(tmp.data() + (size_t)c * ny, *py, inverse);
+  for (int c = 0; c < nx; ++c) fft1d(tmp.data() + (size_t)c * ny, *py, inverse, work.data());
   transpose_into(tmp.data(), data, nx, ny);
 }
 
@@ -717,9 +748,11 @@ struct IltProblem {
     return t;
   }
   std::vector<std::vector<cd>> fields(const IltCond& c, const std::vector<cd>& spec) const {
-    const size_t K = c.lam.size();
+    const int K = (int)c.lam.size();
     std::vector<std::vector<cd>> f(K);
-    for (size_t k = 0; k < K; ++k) {
+    // kernels in parallel; each field is computed alone (same values for any thread count)
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < K; ++k) {
       std::vector<cd> p(npix()), tmp;
       const cd* ker = c.kernels.data() + k * npix();
       for (size_t i = 0; i < npix(); ++i) p[i] = ker[i] * spec[i];
@@ -732,8 +765,10 @@ struct IltProblem {
   }
   std::vector<double> intensity(const IltCond& c, const std::vector<std::vector<cd>>& f) const {
     std::vector<double> img(npix(), 0.0);
-    for (size_t k = 0; k < f.size(); ++k)
-      for (size_t i = 0; i < npix(); ++i) img[i] += c.lam[k] * std::norm(f[k][i]);
+    const int64_t np = (int64_t)npix();
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < np; ++i)
+      for (size_t k = 0; k < f.size(); ++k) img[i] += c.lam[k] * std::norm(f[k][i]);
     if (flare > 0.0) {
       double mean = 0;
       for (double v : img) mean += v;
@@ -826,8 +861,12 @@ struct IltProblem {
         for (size_t i = 0; i < npix(); ++i) dcdm[i] += c.weight * g0[i];
         continue;
       }
-      std::vector<cd> v(npix()), tmp;
-      for (size_t k = 0; k < f.size(); ++k) {
+      // per-kernel terms in parallel, then summed in kernel order
+      const int K = (int)f.size();
+      std::vector<std::vector<double>> part(K);
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < K; ++k) {
+        std::vector<cd> v(npix()), tmp;
         for (size_t i = 0; i < npix(); ++i) v[i] = f[k][i] * g0[i];
         transform2d(v.data(), n, n, false, nullptr, 0, tmp);
         const cd* ker = c.kernels.data() + k * npix();
@@ -835,8 +874,13 @@ struct IltProblem {
         transform2d(v.data(), n, n, true, nullptr, 0, tmp);
         const double sc = c.lam[k] / npx;
         const double w2 = 2.0 * c.weight;
-        for (size_t i = 0; i < npix(); ++i) dcdm[i] += w2 * (conj_c * (v[i] * sc)).real();
+        part[k].resize(npix());
+        for (size_t i = 0; i < npix(); ++i) part[k][i] = w2 * (conj_c * (v[i] * sc)).real();
       }
+      const int64_t np = (int64_t)npix();
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < np; ++i)
+        for (int k = 0; k < K; ++k) dcdm[i] += part[k][i];
     }
     std::vector<double> rg;
     total += regularization(m, rg);

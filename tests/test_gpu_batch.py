@@ -170,3 +170,42 @@ def test_graph_is_dropped_with_its_kernel_sets():
         print("K", K, "err", err)
         assert err <= TOL
         del planes, eng
+
+
+def test_uneven_batch_at_k32_with_a_2048_class():
+    """The uneven-batch workload's shape classes at its own kernel count: edges
+    256/512/1024/2048 px at 2 nm, ArF 193i annular 0.7/0.9, K = 32 per size, one
+    graph launch, 30-60 % dense Manhattan content.  The batch equals the
+    per-group calls bit for bit (graph replays included) and the first clip of
+    every class matches the FP64 oracle within the gate."""
+    import torch
+    from highuvlith_amd import workloads as W
+    ctx = H.DeviceContext(0)
+    rng = np.random.default_rng(17)
+    groups, engs = [], []
+    for n, count in [(256, 3), (512, 2), (1024, 2), (2048, 1)]:
+        src, opt, grid, K = W.arf_annular_2nm_setup(n)
+        eng = H.AerialImageEngine(src, opt, grid, max_kernels=K, context=ctx)
+        assert eng.num_kernels() == 32
+        planes = [eng.device_kernel_set(0.0)]
+        cf = torch.from_numpy(rng.uniform(0.3, 0.6, count))
+        masks = W.manhattan_clips(count, n, seed=200 + n, clear_fraction=cf,
+                                  max_side=max(16, n // 8))
+        out = torch.empty((count, 1, n, n), device="cuda")
+        groups.append((masks, planes, out))
+        engs.append(eng)
+    for rep in range(3):
+        for _, _, out in groups:
+            out.fill_(float("nan"))
+        H.engine.socs_images_batch_device(ctx, groups, graph=True)
+        torch.cuda.synchronize()
+        for masks, planes, out in groups:
+            single = torch.empty_like(out)
+            H.engine.socs_images_device(ctx, masks, planes, single)
+            torch.cuda.synchronize()
+            assert (single - out).abs().max().item() == 0.0, (rep, masks.shape)
+    for eng, (masks, planes, out) in zip(engs, groups):
+        n = masks.shape[-1]
+        err = np.abs(out[0, 0].cpu().numpy() - _oracle(eng, masks[0].cpu().numpy())).max()
+        print("uneven K=32", n, "band", planes[0].band, "coarse", planes[0].coarse, "err", err)
+        assert err <= TOL

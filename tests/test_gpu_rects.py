@@ -269,3 +269,52 @@ def test_crossing_or_overlapping_polygons_are_rejected():
                                                                            endpoint=False)]
     with pytest.raises(H.HuvError):  # more than 8192 vertices
         H.engine.band_spectrum_features_device(ctx, [[]], [[(circle, 1.0)]], grid, 10)
+
+
+def test_polygons_sharing_an_edge_up_to_rounding_are_accepted():
+    """Two polygons whose shared edge differs by a few ulps overlap by far less
+    than 1e-9 of the field length, which the reference counts as no overlap
+    (mask.rs:838-841,1099-1108): accepted, and the band equals the oracle's."""
+    n, px, band = 256, 1.0, 30
+    grid = H.GridConfig(n, px)
+    xe = 10.0
+    left = [(-40.0, -20.0), (xe, -20.0), (xe, 20.0), (-40.0, 25.0)]
+    xr = float(np.nextafter(np.nextafter(xe, -1.0), -1.0))   # 2 ulps to the left
+    right = [(xr, -20.0), (50.0, -25.0), (50.0, 20.0), (xr, 20.0)]
+    ctx = H.DeviceContext(0)
+    got = H.engine.band_spectrum_features_device(ctx, [[]], [[(left, 1.0), (right, 0.5)]],
+                                                 grid, band).cpu().numpy()
+    ref = oracle.polygon_spectrum(n, px, right, 0.5, spectrum=oracle.polygon_spectrum(n, px, left))
+    err = np.abs(got[0] - _band(ref, band)).max() / n ** 2
+    print("shared edge", err)
+    assert err <= SPEC_TOL, err
+    # a real overlap (1 nm) is still refused
+    right2 = [(xe - 1.0, -20.0), (50.0, -25.0), (50.0, 20.0), (xe - 1.0, 20.0)]
+    with pytest.raises(H.HuvError):
+        H.engine.band_spectrum_features_device(ctx, [[]], [[(left, 1.0), (right2, 0.5)]],
+                                               grid, band)
+
+
+@pytest.mark.parametrize("band", [64, 100])
+def test_polygon_band_wider_than_127_columns(band):
+    """Band boxes of 129 and 201 columns (more than 1024 tiles of 4 x 4): the
+    polygon pass deals a clip over more work-groups; the band must equal the
+    oracle's edge-sum spectrum."""
+    n, px = 512, 2.0
+    grid = H.GridConfig(n, px)
+    tri = [(-60.0, -50.0), (20.0, -45.0), (-30.0, 40.0)]
+    hexa = [(150.0 + 40.0 * np.cos(a), -100.0 + 40.0 * np.sin(a)) for a in np.arange(6) * np.pi / 3]
+    ctx = H.DeviceContext(0)
+    got = H.engine.band_spectrum_features_device(
+        ctx, [[(-200.0, 200.0, 50.0, 30.0, 1.0)], []], [[(tri, 1.0)], [(hexa, 0.3 + 0.2j)]],
+        grid, band).cpu().numpy()
+    s0 = oracle.polygon_spectrum(n, px, tri, 1.0, spectrum=oracle.mask_spectrum(
+        n, px, [("rect", -200.0, 200.0, 50.0, 30.0)], dark_field=True))
+    s1 = oracle.polygon_spectrum(n, px, hexa, 0.3 + 0.2j)
+    for c, s in enumerate((s0, s1)):
+        err = np.abs(got[c] - _band(s, band)).max() / n ** 2
+        print("wide band", band, c, err)
+        assert err <= SPEC_TOL, (band, c, err)
+    with pytest.raises(H.HuvError):   # beyond the polygon pass's LDS tables
+        H.engine.band_spectrum_features_device(ctx, [[]], [[(tri, 1.0)]], H.GridConfig(2048, 1.0),
+                                               600)

@@ -63,3 +63,24 @@ def test_feature_lists_split_polygons_and_pack_offsets():
     assert np.array_equal(pa.vertices[7:11], np.asarray(polys[1][0], float))
     empty = E.PolygonArrays([[], []])
     assert list(empty.poly_offsets) == [0, 0, 0] and list(empty.vertex_offsets) == [0]
+
+
+def test_gray_transmittance_range():
+    """Gray rectangles: transmittance T in [0, 1] gives amplitude sqrt(T); a
+    value outside [0, 1] is an invalid parameter (mask.rs:360) and a non-finite
+    one is dropped as the reference's spectrum drops it (mask.rs:1009-1011)."""
+    g = H.GridConfig(64, 4.0)
+    rects, _ = E.mask_feature_rects([("gray", 0.0, 0.0, 20.0, 20.0, 0.36)], g, dark_field=True)
+    assert len(rects) == 1 and abs(rects[0][4] - 0.6) < 1e-15
+    rects, _ = E.mask_feature_rects([("gray", 0.0, 0.0, 20.0, 20.0, float("nan")),
+                                     ("rect", 5.0, 5.0, 4.0, 4.0)], g, dark_field=True)
+    assert len(rects) == 1 and rects[0][4] == 1.0
+    for bad in (-0.1, 1.5):
+        with pytest.raises(H.HuvError):
+            E.mask_feature_rects([("gray", 0.0, 0.0, 20.0, 20.0, bad)], g)
+
+
+def test_polygon_lists_must_match_the_clips():
+    g = H.GridConfig(64, 4.0)
+    with pytest.raises(H.HuvError):
+        E.band_spectrum_features_device(None, [[], []], [[]], g, 8)
