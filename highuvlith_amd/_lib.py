@@ -103,6 +103,9 @@ class huv_ilt_config(C.Structure):
         ("bin_weight", C.c_double),
         ("absorber_re", C.c_double),
         ("absorber_im", C.c_double),
+        ("min_feature_nm", C.c_double),
+        ("pixel_nm", C.c_double),
+        ("gradient", C.c_int),
     ]
 
 
@@ -145,6 +148,10 @@ SIGNATURES = {
                                   C.POINTER(huv_socs_opts)]),
     "huv_socs_images_host": (C.c_int, [P, C.c_int, C.c_int, PF, C.c_int, C.POINTER(P), PF,
                                        C.POINTER(huv_socs_opts)]),
+    "huv_socs_images_complex": (C.c_int, [P, C.c_int, C.c_int, PF, C.c_int, C.POINTER(P), PF,
+                                          C.POINTER(huv_socs_opts)]),
+    "huv_socs_images_spectrum": (C.c_int, [P, C.c_int, C.c_int, PD, C.c_int, C.POINTER(P),
+                                           PF]),
     "huv_band_spectrum": (C.c_int, [P, C.c_int, C.c_int, C.c_int, PF, PF, C.c_int]),
     "huv_band_spectrum_rects": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_int, PI64, PD, PD,
                                           PF]),
@@ -178,6 +185,11 @@ SIGNATURES = {
                                    PD, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "huv_ilt_optimize_to": (C.c_int, [P, C.c_int, PF, PF, C.c_int, C.c_double, C.c_double,
                                       C.c_int, PD, PD, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "huv_ilt_run_stage": (C.c_int, [P, C.c_int, PF, PF, C.c_int, C.c_double, C.c_double, C.c_int,
+                                    PD, PD, PD, C.c_int, C.POINTER(C.c_int), C.c_int, PF,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "huv_ilt_set_binarization": (C.c_int, [P, C.c_double]),
+    "huv_ilt_mask": (C.c_int, [P, C.c_int, PF, PF]),
 }
 
 _lib = None

@@ -288,6 +288,62 @@ huv_status band_stage(Ctx* c, int nclips, int n, int B, int BB, const float* mas
   return HUV_OK;
 }
 
+huv_status band_stage_complex(Ctx* c, int nclips, int n, int B, int BB, const float2* masks,
+                              float2* Mq, bool pixel_transfer, int n1, float2* Mp) {
+  hipError_t e;
+  const float2* tw_n = c->twiddles(n, e);
+  if (!tw_n) return hip_fail(e, "twiddle table");
+  const float* st = c->sinc_table(n, B, pixel_transfer, e);
+  if (!st) return hip_fail(e, "sinc table");
+  const size_t tmp_bytes = (size_t)nclips * (2 * B + 1) * n * sizeof(float2);
+  float2* tmp = (float2*)c->buf(Workspace::TMP_ROWS, tmp_bytes, e);
+  if (!tmp) return hip_fail(e, "workspace");
+  hipEvent_t ev;
+  c->stage_begin(ST_ROWS_BAND, &ev);
+  e = huvk::launch_rows_band_c(n, n1, B, nclips, masks, tw_n, tmp, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "rows_band_c launch");
+  c->stage_end(ST_ROWS_BAND, ev);
+  c->stage_begin(ST_COLS_BAND, &ev);
+  e = huvk::launch_cols_band_c(n, n1, B, BB, nclips, tmp, tw_n, st, Mq, Mp, c->stream());
+  if (e != hipSuccess) return hip_fail(e, "cols_band_c launch");
+  c->stage_end(ST_COLS_BAND, ev);
+  return HUV_OK;
+}
+
+huv_status spectrum_band_stage(Ctx* c, int nclips, int n, int B, int BB, const double* spectra,
+                               float2* Mq, float2* Mp) {
+  const int W = 2 * BB + 1;
+  const size_t box = (size_t)W * W, pbox = (size_t)(W + 1) * W;
+  const size_t bytes = (size_t)nclips * (box + (Mp ? pbox : 0)) * sizeof(float2);
+  hipError_t e = hipStreamSynchronize(c->stream());  // the staging area may feed an earlier copy
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  if (!c->staging(bytes, e)) return hip_fail(e, "staging buffer");
+  float2* hq = (float2*)c->pin_h;
+  float2* hp = hq + (size_t)nclips * box;
+  std::memset(c->pin_h, 0, bytes);
+  for (int i = 0; i < nclips; ++i) {
+    const double* sp = spectra + (size_t)i * n * n * 2;
+    for (int kx = -B; kx <= B; ++kx)
+      for (int ky = -B; ky <= B; ++ky) {
+        const size_t idx = (size_t)((ky + n) % n) * n + (size_t)((kx + n) % n);
+        const float2 v = make_float2((float)sp[2 * idx], (float)sp[2 * idx + 1]);
+        const int col = kx + BB, row = ky + BB;
+        hq[(size_t)i * box + (size_t)col * W + row] = v;
+        if (Mp) hp[(size_t)i * pbox + ((size_t)((col + 1) >> 1) * W + row) * 2 + ((col + 1) & 1)] = v;
+      }
+  }
+  hipEvent_t ev;
+  c->stage_begin(ST_COLS_BAND, &ev);
+  e = hipMemcpyAsync(Mq, hq, (size_t)nclips * box * sizeof(float2), hipMemcpyHostToDevice,
+                     c->stream());
+  if (e == hipSuccess && Mp)
+    e = hipMemcpyAsync(Mp, hp, (size_t)nclips * pbox * sizeof(float2), hipMemcpyHostToDevice,
+                       c->stream());
+  if (e != hipSuccess) return hip_fail(e, "spectrum upload");
+  c->stage_end(ST_COLS_BAND, ev);
+  return HUV_OK;
+}
+
 
 // Bounds on one call's rectangle geometry: the edge grid of one clip (as in
 // mask.rs:95, MAX_CELLS) and the host staging of all clips.
@@ -560,7 +616,7 @@ static huv::Illumination to_ill(const huv_illumination* i) {
 extern "C" {
 
 const char* huv_last_error(void) { return g_err.c_str(); }
-int huv_abi_version(void) { return 100; }
+int huv_abi_version(void) { return 101; }
 
 double huv_illumination_at(const huv_illumination* ill, double sx, double sy) {
   if (!ill) return 0.0;
@@ -972,9 +1028,9 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
 
 // Forward images of real masks: band spectrum -> fused fields -> epilogue.
 // Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
-huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
+huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                         huv_kset* const* planes, float* out, bool pt, float2** Mq_out,
-                        const huv_socs_tuning* tune, const RectBatch* rects, int max_splits) {
+                        const huv_socs_tuning* tune, int max_splits) {
   const int B = planes[0]->B;
   const int m = tune && tune->m ? tune->m : planes[0]->m;
   const int BB = huvk::coarse_band(m);
@@ -996,8 +1052,21 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplan
     Mp = (float2*)c->buf(Workspace::BAND_PAIRED, (size_t)nclips * (W + 1) * W * sizeof(float2), e);
     if (!Mp) return hip_fail(e, "workspace");
   }
-  huv_status st = rects ? rect_band_stage(c, nclips, n, B, BB, *rects, Mq, Mp)
-                        : band_stage(c, nclips, n, B, BB, masks, Mq, pt, n1, Mp);
+  huv_status st = HUV_OK;
+  switch (src.kind) {
+    case MaskSource::REAL:
+      st = band_stage(c, nclips, n, B, BB, src.raster, Mq, pt, n1, Mp);
+      break;
+    case MaskSource::COMPLEX:
+      st = band_stage_complex(c, nclips, n, B, BB, (const float2*)src.raster, Mq, pt, n1, Mp);
+      break;
+    case MaskSource::FEATURES:
+      st = rect_band_stage(c, nclips, n, B, BB, *src.rects, Mq, Mp);
+      break;
+    case MaskSource::SPECTRUM:
+      st = spectrum_band_stage(c, nclips, n, B, BB, src.spectra, Mq, Mp);
+      break;
+  }
   if (st != HUV_OK) return st;
   const float2* kern;
   const float* lam;
@@ -1059,7 +1128,7 @@ std::vector<int> tuning_key(int n, const huv_kset* const* planes, int nplanes) {
 // Times every launch shape on this call's inputs (coordinate descent: coarse
 // grid, then band-FFT factor, then x-pass block) and records the fastest.
 // The last run leaves a valid result in `out`.
-huv_status autotune_socs(Ctx* c, int nclips, int n, const float* masks, int nplanes,
+huv_status autotune_socs(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                          huv_kset* const* planes, float* out, bool pt, huv_socs_tuning* best) {
   hipEvent_t a = nullptr, b = nullptr;
   hipError_t e = hipEventCreate(&a);
@@ -1072,7 +1141,7 @@ huv_status autotune_socs(Ctx* c, int nclips, int n, const float* masks, int npla
     double bestms = 1e30;
     for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and builds tables
       hipEventRecord(a, c->stream());
-      huv_status s2 = socs_forward(c, nclips, n, masks, nplanes, planes, out, pt, nullptr, &t);
+      huv_status s2 = socs_forward(c, nclips, n, src, nplanes, planes, out, pt, nullptr, &t);
       if (s2 != HUV_OK) return s2;
       hipEventRecord(b, c->stream());
       hipError_t e2 = hipEventSynchronize(b);
@@ -1118,7 +1187,7 @@ huv_status autotune_socs(Ctx* c, int nclips, int n, const float* masks, int npla
     }
   }
   // leave the chosen shape's result in `out`
-  if (st == HUV_OK) st = socs_forward(c, nclips, n, masks, nplanes, planes, out, pt, nullptr, &cur);
+  if (st == HUV_OK) st = socs_forward(c, nclips, n, src, nplanes, planes, out, pt, nullptr, &cur);
   c->profiling = prof;
   hipEventDestroy(a);
   hipEventDestroy(b);
@@ -1127,7 +1196,7 @@ huv_status autotune_socs(Ctx* c, int nclips, int n, const float* masks, int npla
 }
 
 // Launch shape for a call: the table entry, a fresh autotune, or defaults.
-huv_status socs_shape(Ctx* c, int nclips, int n, const float* masks, int nplanes,
+huv_status socs_shape(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                       huv_kset* const* planes, float* out, bool pt, huv_socs_tuning* t,
                       bool* computed) {
   *computed = false;
@@ -1139,7 +1208,7 @@ huv_status socs_shape(Ctx* c, int nclips, int n, const float* masks, int nplanes
   }
   *t = huv_socs_tuning{0, 0, 0, 0};
   if (!c->autotune) return HUV_OK;
-  huv_status st = autotune_socs(c, nclips, n, masks, nplanes, planes, out, pt, t);
+  huv_status st = autotune_socs(c, nclips, n, src, nplanes, planes, out, pt, t);
   if (st != HUV_OK) return st;
   c->tuned[key] = *t;
   *computed = true;
@@ -1219,7 +1288,12 @@ struct GroupPlan {
 };
 
 huv_status socs_groups(Ctx* c, int ngroups, const huv_clip_group* groups, const huv_socs_opts* opts,
-                       int tag) {
+                       int tag, bool complex_masks) {
+  auto source = [&](const huv_clip_group& g) {
+    MaskSource m = MaskSource::real(g.masks);
+    if (complex_masks) m.kind = MaskSource::COMPLEX;
+    return m;
+  };
   const bool pt = opts && opts->pixel_transfer;
   std::vector<GroupPlan> plan;
   std::vector<uint64_t> key{(uint64_t)tag, (uint64_t)pt, (uint64_t)ngroups};
@@ -1229,8 +1303,8 @@ huv_status socs_groups(Ctx* c, int ngroups, const huv_clip_group* groups, const 
     if (g.nclips == 0) continue;
     GroupPlan gp{g, {0, 0, 0, 0}};
     bool computed = false;
-    huv_status st = socs_shape(c, g.nclips, g.n, g.masks, g.nplanes, g.planes, g.out, pt, &gp.t,
-                               &computed);
+    huv_status st = socs_shape(c, g.nclips, g.n, source(g), g.nplanes, g.planes, g.out, pt,
+                               &gp.t, &computed);
     if (st != HUV_OK) return st;
     plan.push_back(gp);
     key.insert(key.end(), {(uint64_t)g.n, (uint64_t)g.nclips, (uint64_t)(uintptr_t)g.masks,
@@ -1243,8 +1317,8 @@ huv_status socs_groups(Ctx* c, int ngroups, const huv_clip_group* groups, const 
   }
   auto body = [&]() -> huv_status {
     for (const auto& gp : plan) {
-      huv_status st = socs_forward(c, gp.g.nclips, gp.g.n, gp.g.masks, gp.g.nplanes, gp.g.planes,
-                                   gp.g.out, pt, nullptr, &gp.t);
+      huv_status st = socs_forward(c, gp.g.nclips, gp.g.n, source(gp.g), gp.g.nplanes,
+                                   gp.g.planes, gp.g.out, pt, nullptr, &gp.t);
       if (st != HUV_OK) return st;
     }
     return HUV_OK;
@@ -1267,7 +1341,38 @@ huv_status huv_socs_images(huv_ctx* c, int nclips, int n, const float* masks, in
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
   huv_clip_group g{n, nclips, masks, nplanes, planes, out};
-  return socs_groups(c, 1, &g, opts, 1);
+  return socs_groups(c, 1, &g, opts, 1, false);
+}
+
+huv_status huv_socs_images_complex(huv_ctx* c, int nclips, int n, const float* masks,
+                                   int nplanes, huv_kset* const* planes, float* out,
+                                   const huv_socs_opts* opts) {
+  if (!c || !masks || !planes || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (nclips < 0 || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0 and nplanes >= 1");
+  huv_status st = validate_planes(c, n, nplanes, planes);
+  if (st != HUV_OK) return st;
+  if (nclips == 0) return HUV_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  huv_clip_group g{n, nclips, masks, nplanes, planes, out};
+  return socs_groups(c, 1, &g, opts, 3, true);
+}
+
+huv_status huv_socs_images_spectrum(huv_ctx* c, int nclips, int n, const double* spectra,
+                                    int nplanes, huv_kset* const* planes, float* out) {
+  if (!c || !spectra || !planes || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (nclips < 0 || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0 and nplanes >= 1");
+  huv_status st = validate_planes(c, n, nplanes, planes);
+  if (st != HUV_OK) return st;
+  if (nclips == 0) return HUV_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  auto it = c->tuned.find(tuning_key(n, planes, nplanes));
+  MaskSource src;
+  src.kind = MaskSource::SPECTRUM;
+  src.spectra = spectra;
+  return socs_forward(c, nclips, n, src, nplanes, planes, out, false, nullptr,
+                      it != c->tuned.end() ? &it->second : nullptr);
 }
 
 huv_status huv_socs_images_features(huv_ctx* c, int nclips, int n, double pixel_nm,
@@ -1290,8 +1395,11 @@ huv_status huv_socs_images_features(huv_ctx* c, int nclips, int n, double pixel_
   hipSetDevice(c->device);
   // a launch shape tuned for these planes applies (its band-FFT factor is unused)
   auto it = c->tuned.find(tuning_key(n, planes, nplanes));
-  return socs_forward(c, nclips, n, nullptr, nplanes, planes, out, false, nullptr,
-                      it != c->tuned.end() ? &it->second : nullptr, &rb);
+  MaskSource src;
+  src.kind = MaskSource::FEATURES;
+  src.rects = &rb;
+  return socs_forward(c, nclips, n, src, nplanes, planes, out, false, nullptr,
+                      it != c->tuned.end() ? &it->second : nullptr);
 }
 
 huv_status huv_socs_images_rects(huv_ctx* c, int nclips, int n, double pixel_nm,
@@ -1316,7 +1424,7 @@ huv_status huv_socs_images_batch(huv_ctx* c, int ngroups, const huv_clip_group* 
   }
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  return socs_groups(c, ngroups, groups, opts, 2);
+  return socs_groups(c, ngroups, groups, opts, 2, false);
 }
 
 huv_status huv_ctx_set_autotune(huv_ctx* c, int on) {

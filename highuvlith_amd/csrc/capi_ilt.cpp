@@ -24,6 +24,10 @@ struct huv_ilt {
   std::vector<huv_kset*> conds;
   huv_ilt_config cfg{};
   double flare = 0.0;
+  bool complex_abs = false;  // absorber with an imaginary part: complex transmittance
+  bool proxy = false;        // local-proxy gradient
+  int Bf = 0;                // density-filter band half-width (0: no filter)
+  float filter_c = 0.0f;     // filter exponent per squared bin index, 2 pi^2 s^2 df^2
   float* dose_d = nullptr;
   float* weight_d = nullptr;
   ~huv_ilt() {
@@ -39,6 +43,11 @@ constexpr int kBpi = 64;  // reduction blocks per image
 struct IltBuffers {
   float *m, *t, *image, *G, *dcdm, *reg_grad, *G0g;
   float2 *Gb, *Rout, *H;
+  float2* tc = nullptr;                       // complex transmittance (complex absorber)
+  float *thf = nullptr, *gm = nullptr;        // filtered theta, chain-rule temporary (filter)
+  float2 *Xf = nullptr, *If = nullptr;        // filter band box and half band
+  float2 *g0tmp = nullptr, *abuf = nullptr;   // adjoint on coarse grids above 64
+  double* proxy_part = nullptr;               // per-image sums of G (local proxy)
   double *fid_part, *reg_part, *cost_d;
   int KS = 1;  // adjoint work-groups per image (kernels dealt over them)
 };
@@ -55,10 +64,13 @@ huv_status ilt_buffers(Ctx* c, const huv_ilt* p, int T, IltBuffers& b) {
   const size_t N = (size_t)p->n * p->n;
   const int W = 2 * p->B + 1, W2 = 4 * p->B + 1;
   const size_t TC = (size_t)T * p->ncond;
-  char* a = (char*)c->ws.get(Workspace::ILT0, 2 * (size_t)T * N * sizeof(float), e);
+  // m, then t (real) or tc (interleaved complex)
+  const size_t tw = p->complex_abs ? 2 : 1;
+  char* a = (char*)c->ws.get(Workspace::ILT0, (1 + tw) * (size_t)T * N * sizeof(float), e);
   if (!a) return hip_fail(e, "ILT workspace");
   b.m = (float*)a;
   b.t = b.m + (size_t)T * N;
+  b.tc = p->complex_abs ? (float2*)b.t : nullptr;
   b.image = (float*)c->ws.get(Workspace::ILT1, TC * N * sizeof(float), e);
   if (!b.image) return hip_fail(e, "ILT workspace");
   b.G = (float*)c->ws.get(Workspace::ILT2, TC * N * sizeof(float), e);
@@ -81,7 +93,68 @@ huv_status ilt_buffers(Ctx* c, const huv_ilt* p, int T, IltBuffers& b) {
   b.fid_part = (double*)a;
   b.reg_part = b.fid_part + TC * kBpi;
   b.cost_d = b.reg_part + (size_t)T * kBpi;
+  if (p->m > 64) {
+    const size_t ng0 = TC * W2 * p->m, nab = TC * b.KS * W * p->m;
+    a = (char*)c->ws.get(Workspace::ILT9, (ng0 + nab) * sizeof(float2), e);
+    if (!a) return hip_fail(e, "ILT workspace");
+    b.g0tmp = (float2*)a;
+    b.abuf = b.g0tmp + ng0;
+  }
+  if (p->Bf > 0) {
+    const int Wf = 2 * p->Bf + 1;
+    const size_t nx = (size_t)T * Wf * Wf, ni = (size_t)T * (p->Bf + 1) * Wf;
+    a = (char*)c->ws.get(Workspace::ILT10,
+                         2 * (size_t)T * N * sizeof(float) + (nx + ni) * sizeof(float2), e);
+    if (!a) return hip_fail(e, "ILT workspace");
+    b.thf = (float*)a;
+    b.gm = b.thf + (size_t)T * N;
+    b.Xf = (float2*)(b.gm + (size_t)T * N);
+    b.If = b.Xf + nx;
+  }
+  if (p->proxy) {
+    b.proxy_part = (double*)c->ws.get(Workspace::ILT11, TC * kBpi * sizeof(double), e);
+    if (!b.proxy_part) return hip_fail(e, "ILT workspace");
+  }
   return HUV_OK;
+}
+
+// out = G_s * in for count real n x n fields (the density filter, applied on
+// the |k| <= Bf band where its spectrum exceeds 1e-12; ilt.rs:524-535).
+huv_status ilt_filter(huv_ilt* p, int count, const float* in, float* out, IltBuffers& b) {
+  Ctx* c = p->ctx;
+  const int n = p->n;
+  huv_status st = band_stage(c, count, n, p->Bf, p->Bf, in, b.Xf, false);
+  if (st != HUV_OK) return st;
+  hipError_t e = huvk::launch_ilt_filter_band(b.Xf, p->Bf, p->filter_c,
+                                              (float)(1.0 / ((double)n * n)), count, b.If,
+                                              c->stream());
+  if (e != hipSuccess) return hip_fail(e, "ilt filter band launch");
+  const float2* tw_n = c->twiddles(n, e);
+  if (!tw_n) return hip_fail(e, "twiddle table");
+  e = huvk::launch_band_image(n, p->Bf, count, b.If, tw_n, out, c->stream());
+  return e == hipSuccess ? HUV_OK : hip_fail(e, "ilt filter image launch");
+}
+
+// m (and the transmittance t) of count targets from theta: the density
+// filter first when configured.
+huv_status ilt_mask_t(huv_ilt* p, int count, const float* theta, float* m, void* t,
+                      IltBuffers& b) {
+  Ctx* c = p->ctx;
+  const int64_t total = (int64_t)count * p->n * p->n;
+  const float* th = theta;
+  if (p->Bf > 0) {
+    huv_status st = ilt_filter(p, count, theta, b.thf, b);
+    if (st != HUV_OK) return st;
+    th = b.thf;
+  }
+  hipError_t e;
+  if (p->complex_abs || !t)
+    e = huvk::launch_ilt_mask_c(th, m, (float2*)t, total, (float)p->cfg.beta,
+                                (float)p->cfg.absorber_re, (float)p->cfg.absorber_im, c->stream());
+  else
+    e = huvk::launch_ilt_mask(th, m, (float*)t, total, (float)p->cfg.beta,
+                              (float)p->cfg.absorber_re, c->stream());
+  return e == hipSuccess ? HUV_OK : hip_fail(e, "ilt mask launch");
 }
 
 // Cost (into b.cost_d, device) and optionally the theta-gradient.
@@ -92,14 +165,15 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const float* theta,
   hipError_t e;
   const int n = p->n, C = p->ncond, B = p->B, m = p->m;
   const int64_t N = (int64_t)n * n;
-  const float a = (float)p->cfg.absorber_re;
   hipEvent_t ev;
   c->stage_begin(ST_ILT_POINTWISE, &ev);
-  e = huvk::launch_ilt_mask(theta, b.m, b.t, (int64_t)T * N, (float)p->cfg.beta, a, s);
-  if (e != hipSuccess) return hip_fail(e, "ilt mask launch");
+  huv_status st = ilt_mask_t(p, T, theta, b.m, p->complex_abs ? (void*)b.tc : (void*)b.t, b);
+  if (st != HUV_OK) return st;
   c->stage_end(ST_ILT_POINTWISE, ev);
   float2* Mq = nullptr;
-  huv_status st = socs_forward(c, T, n, b.t, C, p->conds.data(), b.image, false, &Mq);
+  MaskSource src = MaskSource::real(b.t);
+  if (p->complex_abs) src.kind = MaskSource::COMPLEX;
+  st = socs_forward(c, T, n, src, C, p->conds.data(), b.image, false, &Mq);
   if (st != HUV_OK) return st;
   c->stage_begin(ST_ILT_POINTWISE, &ev);
   e = huvk::launch_ilt_pointwise(b.image, targets, T * C, C, N, kBpi, p->cfg.cost_kind,
@@ -117,33 +191,61 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const float* theta,
   if (e != hipSuccess) return hip_fail(e, "ilt cost reduce launch");
   c->stage_end(ST_ILT_POINTWISE, ev);
   if (!grad) return HUV_OK;
-  // |g| <= 2B band of F(G) for every (target, condition) image.
-  st = band_stage(c, T * C, n, 2 * B, 2 * B, b.G, b.Gb, false);
-  if (st != HUV_OK) return st;
-  const int* dK;
-  const float2* const* dkern;
-  const float* const* dlam;
-  st = plane_table(c, C, p->conds.data(), &dK, &dkern, &dlam);
-  if (st != HUV_OK) return st;
-  c->stage_begin(ST_ILT_ADJOINT, &ev);
-  const double n2 = (double)n * n;
-  const float g0_scale = 1.0f / ((float)m * (float)m);
-  const float lam_scale = (float)n2;  // device lambda carries 1/n^4
-  e = huvk::launch_ilt_adjoint(m, T * C, b.KS, Mq, b.Gb, B, C, dK, dkern, dlam, lam_scale,
-                               g0_scale, (float)p->flare, b.G0g, b.Rout, s);
-  if (e != hipSuccess) return hip_fail(e, "ilt adjoint launch");
-  c->stage_end(ST_ILT_ADJOINT, ev);
+  if (p->proxy) {
+    // local proxy: dC/dm = sum_c w_c G0_c (ilt.rs:719-721)
+    c->stage_begin(ST_ILT_OTHER, &ev);
+    e = huvk::launch_ilt_proxy(b.G, T, C, N, kBpi, p->weight_d, (float)p->flare, b.proxy_part,
+                               b.dcdm, s);
+    if (e != hipSuccess) return hip_fail(e, "ilt proxy launch");
+    c->stage_end(ST_ILT_OTHER, ev);
+  } else {
+    // |g| <= 2B band of F(G) for every (target, condition) image.
+    st = band_stage(c, T * C, n, 2 * B, 2 * B, b.G, b.Gb, false);
+    if (st != HUV_OK) return st;
+    const int* dK;
+    const float2* const* dkern;
+    const float* const* dlam;
+    st = plane_table(c, C, p->conds.data(), &dK, &dkern, &dlam);
+    if (st != HUV_OK) return st;
+    c->stage_begin(ST_ILT_ADJOINT, &ev);
+    const double n2 = (double)n * n;
+    const float g0_scale = 1.0f / ((float)m * (float)m);
+    const float lam_scale = (float)n2;  // device lambda carries 1/n^4
+    if (m <= 64) {
+      e = huvk::launch_ilt_adjoint(m, T * C, b.KS, Mq, b.Gb, B, C, dK, dkern, dlam, lam_scale,
+                                   g0_scale, (float)p->flare, b.G0g, b.Rout, s);
+    } else {
+      if ((int64_t)T * C * b.KS > 65535)
+        return fail(HUV_INVALID_PARAMETER, "targets x conditions x kernels exceeds 65535 per "
+                                           "evaluation on coarse grids above 64: pass fewer targets");
+      const float2* tw_m = c->twiddles(m, e);
+      if (!tw_m) return hip_fail(e, "twiddle table");
+      e = huvk::launch_ilt_adjoint_big(m, T * C, b.KS, Mq, b.Gb, B, C, dK, dkern, dlam,
+                                       lam_scale, g0_scale, (float)p->flare, tw_m, b.g0tmp,
+                                       b.G0g, b.abuf, b.Rout, s);
+    }
+    if (e != hipSuccess) return hip_fail(e, "ilt adjoint launch");
+    c->stage_end(ST_ILT_ADJOINT, ev);
+    c->stage_begin(ST_ILT_OTHER, &ev);
+    e = huvk::launch_ilt_hband(b.Rout, B, T, C, b.KS, p->weight_d,
+                               (float)(1.0 - p->cfg.absorber_re), (float)(-p->cfg.absorber_im),
+                               (float)(1.0 / n2), b.H, s);
+    if (e != hipSuccess) return hip_fail(e, "ilt hband launch");
+    const float2* tw_n = c->twiddles(n, e);
+    if (!tw_n) return hip_fail(e, "twiddle table");
+    e = huvk::launch_band_image(n, B, T, b.H, tw_n, b.dcdm, s);
+    if (e != hipSuccess) return hip_fail(e, "ilt band image launch");
+    c->stage_end(ST_ILT_OTHER, ev);
+  }
+  // chain through the sigmoid, then the (self-adjoint) density filter
   c->stage_begin(ST_ILT_OTHER, &ev);
-  e = huvk::launch_ilt_hband(b.Rout, B, T, C, b.KS, p->weight_d,
-                             (float)((1.0 - p->cfg.absorber_re) / n2), b.H, s);
-  if (e != hipSuccess) return hip_fail(e, "ilt hband launch");
-  const float2* tw_n = c->twiddles(n, e);
-  if (!tw_n) return hip_fail(e, "twiddle table");
-  e = huvk::launch_band_image(n, B, T, b.H, tw_n, b.dcdm, s);
-  if (e != hipSuccess) return hip_fail(e, "ilt band image launch");
   e = huvk::launch_ilt_chain(b.dcdm, reg ? b.reg_grad : nullptr, b.m, (float)p->cfg.beta,
-                             (int64_t)T * N, grad, s);
+                             (int64_t)T * N, p->Bf > 0 ? b.gm : grad, s);
   if (e != hipSuccess) return hip_fail(e, "ilt chain launch");
+  if (p->Bf > 0) {
+    st = ilt_filter(p, T, b.gm, grad, b);
+    if (st != HUV_OK) return st;
+  }
   c->stage_end(ST_ILT_OTHER, ev);
   return HUV_OK;
 }
@@ -171,12 +273,33 @@ huv_status huv_ilt_create(huv_ctx* c, int n, int ncond, huv_kset* const* conds, 
   if (!(cfg->tv_weight >= 0.0 && cfg->bin_weight >= 0.0)) return fail(HUV_INVALID_PARAMETER, "regularization weights must be non-negative");
   if (cfg->cost_kind == 1 && !(cfg->threshold > 0.0 && cfg->steepness > 0.0))
     return fail(HUV_INVALID_PARAMETER, "threshold / steepness: must be positive and finite");
-  if (cfg->absorber_im != 0.0)
-    return fail(HUV_UNSUPPORTED, "complex absorber amplitudes are not supported on the GPU path (real only)");
+  if (!(std::isfinite(cfg->absorber_re) && std::isfinite(cfg->absorber_im)))
+    return fail(HUV_INVALID_PARAMETER, "absorber: must be finite");
+  if (!(cfg->min_feature_nm >= 0.0 && std::isfinite(cfg->min_feature_nm)))
+    return fail(HUV_INVALID_PARAMETER, "min_feature_nm: must be non-negative and finite");
+  if (cfg->gradient != 0 && cfg->gradient != 1)
+    return fail(HUV_INVALID_PARAMETER, "gradient: 0 (adjoint) or 1 (local proxy)");
+  if (n < 256) return fail(HUV_UNSUPPORTED, "GPU ILT needs grids of 256 or more");
   // The adjoint works on the stored kernel boxes (half-width coarse_band(m)).
   const int m = conds[0]->m, B = conds[0]->BB;
-  if (m > 64)
-    return fail(HUV_UNSUPPORTED, "GPU ILT adjoint supports coarse grids up to 64 (band <= 15)");
+  if (m > 256 || 4 * B + 1 > n)
+    return fail(HUV_UNSUPPORTED, "GPU ILT adjoint supports coarse grids up to 256 (band <= 63)");
+  // Density filter (ilt.rs:443-454): exp(-2 pi^2 s^2 f^2), s = min_feature / 1.349,
+  // kept on the bins where it exceeds 1e-12.
+  int Bf = 0;
+  float filter_c = 0.0f;
+  if (cfg->min_feature_nm > 0.0) {
+    if (!(cfg->pixel_nm > 0.0 && std::isfinite(cfg->pixel_nm)))
+      return fail(HUV_INVALID_PARAMETER, "pixel_nm: must be positive with a density filter");
+    const double s = cfg->min_feature_nm / 1.349, df = 1.0 / (n * cfg->pixel_nm);
+    const double cc = 2.0 * M_PI * M_PI * s * s * df * df;
+    const double kmax = std::sqrt(std::log(1e12) / cc);
+    if (kmax > 127.0 || 2.0 * std::floor(kmax) + 1.0 > n)
+      return fail(HUV_UNSUPPORTED, "density filter wider than the 127-bin band (min_feature_nm "
+                                   "too small for this grid)");
+    Bf = std::max(1, (int)std::floor(kmax));
+    filter_c = (float)cc;
+  }
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
   auto p = std::make_unique<huv_ilt>();
@@ -188,6 +311,10 @@ huv_status huv_ilt_create(huv_ctx* c, int n, int ncond, huv_kset* const* conds, 
   p->conds.assign(conds, conds + ncond);
   p->cfg = *cfg;
   p->flare = conds[0]->flare;
+  p->complex_abs = cfg->absorber_im != 0.0;
+  p->proxy = cfg->gradient == 1;
+  p->Bf = Bf;
+  p->filter_c = filter_c;
   std::vector<float> d(dose, dose + ncond), w(weight, weight + ncond);
   hipError_t e = hipMalloc(&p->dose_d, ncond * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&p->weight_d, ncond * sizeof(float));
@@ -221,15 +348,27 @@ huv_status huv_ilt_cost_grad(huv_ilt* p, int T, const float* targets, const floa
 
 namespace {
 
+// Optional trace of a stage: gradient norms and mask snapshots.
+struct StageTrace {
+  double* gnorm_hist = nullptr;
+  int every = 0;
+  const int* offset = nullptr;
+  int nslots = 0;
+  float* snapshots = nullptr;
+};
+
 // run_stage for every target; stop_cost (nullable) ends a target as soon as
 // its cost is at or below stop_cost[t] (termination 3).
 huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, int max_iterations,
                         double learning_rate, double tol, int use_cg, const double* stop_cost,
-                        double* cost_hist, int* iterations, int* termination) {
+                        double* cost_hist, int* iterations, int* termination,
+                        const StageTrace& tr = StageTrace()) {
   if (!p || !targets || !theta || !cost_hist || !iterations || !termination)
     return fail(HUV_INVALID_PARAMETER, "null argument");
   if (T < 1 || max_iterations < 0) return fail(HUV_INVALID_PARAMETER, "bad target count or budget");
   if (!(learning_rate > 0.0)) return fail(HUV_INVALID_PARAMETER, "learning_rate: must be positive and finite");
+  if (tr.every < 0 || (tr.every > 0 && (tr.nslots < 1 || !tr.snapshots)))
+    return fail(HUV_INVALID_PARAMETER, "snapshots: need a buffer of at least one slot");
   Ctx* c = p->ctx;
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
@@ -378,21 +517,40 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   std::vector<int> active(T, 1), small(T, 0), it(T, 0), term(T, 1), attempt(T, 0), searching(T, 0),
       accepted(T, 0);
   for (int t = 0; t < T; ++t)
-    for (int k = 0; k < H; ++k) cost_hist[(size_t)t * H + k] = NAN;
+    for (int k = 0; k < H; ++k) {
+      cost_hist[(size_t)t * H + k] = NAN;
+      if (tr.gnorm_hist) tr.gnorm_hist[(size_t)t * H + k] = NAN;
+    }
   st = eval(theta, grad, cost);
   if (st != HUV_OK) return st;
+  // per-target budget: the iterations left after earlier stages
+  std::vector<int> limit(T, max_iterations);
   for (int t = 0; t < T; ++t) {
+    if (tr.offset) limit[t] = max_iterations - tr.offset[t];
     cost_hist[(size_t)t * H] = cost[t];
+    if (limit[t] <= 0) active[t] = 0;  // budget spent: MaxIterations
     if (stop_cost && cost[t] <= stop_cost[t]) {
       active[t] = 0;
       term[t] = 3;
     }
   }
   if (!reduce({{grad, grad, 0}}, all_t)) return fail_e("dot");
-  for (int t = 0; t < T; ++t) gnorm[t] = std::sqrt(h_red[t]);
+  for (int t = 0; t < T; ++t) {
+    gnorm[t] = std::sqrt(h_red[t]);
+    if (tr.gnorm_hist) tr.gnorm_hist[(size_t)t * H] = gnorm[t];
+  }
   for (int t = 0; t < T; ++t) { h_a[t] = -1.0; h_b[t] = 0.0; }
   if (!dir_rows(grad, all_t)) return fail_e("dir");
-  const bool cg = use_cg != 0;
+  // the local proxy runs steepest descent and accepts on plain decrease (ilt.rs:767-801)
+  const bool proxy = p->proxy;
+  const bool cg = use_cg != 0 && !proxy;
+  // mask snapshots: global iteration of target t, and the rows due after this iteration
+  auto slot_of = [&](int t) {
+    const int g = (tr.offset ? tr.offset[t] : 0) + it[t];
+    return (tr.every > 0 && g > 0 && g % tr.every == 0 && g / tr.every - 1 < tr.nslots)
+               ? g / tr.every - 1 : -1;
+  };
+  std::vector<int> snap_src, snap_dst;
   std::vector<int> lst, rst;
 
   for (int iter = 0; iter < max_iterations; ++iter) {
@@ -411,7 +569,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         dmax[t] = h_red[t];
         if (dmax[t] == 0.0) { searching[t] = 0; continue; }
         slope[t] = h_red[(size_t)T + t] / dmax[t];
-        if (slope[t] >= 0.0) {
+        if (slope[t] >= 0.0 && !proxy) {
           // not a descent direction: restart along -grad (consumes this attempt)
           h_a[t] = -1.0;
           h_b[t] = 0.0;
@@ -468,7 +626,9 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         for (int t : which) {
           bool ok = false;
           for (int j = pfirst[t]; j < pfirst[t] + pcount[t]; ++j) {
-            if (cost_c[j] <= cost[t] + 1e-4 * pstep[j] * slope[t] && std::isfinite(cost_c[j])) {
+            const bool ok_j = proxy ? cost_c[j] < cost[t]
+                                    : cost_c[j] <= cost[t] + 1e-4 * pstep[j] * slope[t];
+            if (ok_j && std::isfinite(cost_c[j])) {
               searching[t] = 0;
               accepted[t] = 1;
               s_used[t] = pstep[j];
@@ -533,6 +693,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       it[t] += 1;
       cost_hist[(size_t)t * H + it[t]] = cost[t];
       gnorm[t] = std::sqrt(gngn[t]);
+      if (tr.gnorm_hist) tr.gnorm_hist[(size_t)t * H + it[t]] = gnorm[t];
       const double beta = cg ? (gg[t] > 0.0 ? std::max(0.0, (gngn[t] - gng[t]) / gg[t]) : 0.0) : 0.0;
       h_a[t] = -1.0;
       h_b[t] = beta;
@@ -547,9 +708,32 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         active[t] = 0;
         term[t] = 3;
       }
+      if (active[t] && it[t] >= limit[t]) active[t] = 0;  // MaxIterations
     }
     if (!dir_rows(gnew, which)) return fail_e("dir");
     if (!copy_rows(gnew, grad, which)) return fail_e("gradient copy");
+    // snapshots of the masks of targets whose global iteration is due
+    snap_src.clear();
+    snap_dst.clear();
+    for (int t : which) {
+      const int slot = slot_of(t);
+      if (slot >= 0) {
+        snap_src.push_back(t);
+        snap_dst.push_back(t * tr.nslots + slot);
+      }
+    }
+    if (!snap_src.empty()) {
+      const int cnt = (int)snap_src.size();
+      const int* sr = (const int*)put(snap_src.data(), cnt * sizeof(int));
+      const int* dr = (const int*)put(snap_dst.data(), cnt * sizeof(int));
+      if (!sr || !dr || !flush()) return fail_e("snapshot staging");
+      e = huvk::launch_rows_gather(theta, sr, N, cnt, th_c, s);
+      if (e != hipSuccess) return fail_e("snapshot gather");
+      st = ilt_mask_t(p, cnt, th_c, g_c, nullptr, b);
+      if (st != HUV_OK) return st;
+      e = huvk::launch_rows_scatter(g_c, dr, N, cnt, tr.snapshots, s);
+      if (e != hipSuccess) return fail_e("snapshot scatter");
+    }
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return fail_e("synchronize");
@@ -578,6 +762,42 @@ huv_status huv_ilt_optimize_to(huv_ilt* p, int T, const float* targets, float* t
   if (!stop_cost) return fail(HUV_INVALID_PARAMETER, "null argument");
   return ilt_optimize(p, T, targets, theta, max_iterations, learning_rate, tol, use_cg, stop_cost,
                       cost_hist, iterations, termination);
+}
+
+huv_status huv_ilt_run_stage(huv_ilt* p, int T, const float* targets, float* theta,
+                             int max_iterations, double learning_rate, double tol, int use_cg,
+                             const double* stop_cost, double* cost_hist, double* gnorm_hist,
+                             int snapshot_every, const int* iteration_offset, int nslots,
+                             float* snapshots, int* iterations, int* termination) {
+  StageTrace tr;
+  tr.gnorm_hist = gnorm_hist;
+  tr.every = snapshot_every;
+  tr.offset = iteration_offset;
+  tr.nslots = nslots;
+  tr.snapshots = snapshots;
+  return ilt_optimize(p, T, targets, theta, max_iterations, learning_rate, tol, use_cg, stop_cost,
+                      cost_hist, iterations, termination, tr);
+}
+
+huv_status huv_ilt_set_binarization(huv_ilt* p, double bin_weight) {
+  if (!p) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (!(bin_weight >= 0.0 && std::isfinite(bin_weight)))
+    return fail(HUV_INVALID_PARAMETER, "binarization_weight: must be non-negative and finite");
+  std::lock_guard<std::mutex> lk(p->ctx->mu);
+  p->cfg.bin_weight = bin_weight;
+  return HUV_OK;
+}
+
+huv_status huv_ilt_mask(huv_ilt* p, int T, const float* theta, float* m) {
+  if (!p || !theta || !m) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (T < 1) return fail(HUV_INVALID_PARAMETER, "need at least one target");
+  Ctx* c = p->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  IltBuffers b;
+  huv_status st = ilt_buffers(c, p, T, b);
+  if (st != HUV_OK) return st;
+  return ilt_mask_t(p, T, theta, m, nullptr, b);
 }
 
 }  // extern "C"

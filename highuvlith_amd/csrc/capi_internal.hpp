@@ -19,7 +19,7 @@ huv_status hip_fail(hipError_t e, const char* what);
 // Device scratch owned by a context, grown on demand and reused across calls.
 struct Workspace {
   enum Slot { TMP_ROWS = 0, BAND, BAND_PAIRED, COARSE, TMP_COARSE, IBAND, ILT0, ILT1, ILT2, ILT3,
-              ILT4, ILT5, ILT6, ILT7, ILT8, RECT_TAB, kSlots };
+              ILT4, ILT5, ILT6, ILT7, ILT8, ILT9, ILT10, ILT11, RECT_TAB, kSlots };
   void* ptr[kSlots] = {};
   size_t size[kSlots] = {};
   uint64_t generation = 0;  // bumped whenever a slot is reallocated
@@ -124,6 +124,27 @@ huv_status rect_band_stage(Ctx* c, int nclips, int n, int B, int BB, const RectB
                            float2* Mq, float2* Mp);
 huv_status validate_rects(int nclips, int n, const RectBatch& rb);
 
+// Band spectrum of nclips complex n x n rasters (device, interleaved FP32):
+// same output as band_stage, every row and both signs of kx transformed.
+huv_status band_stage_complex(Ctx* c, int nclips, int n, int B, int BB, const float2* masks,
+                              float2* Mq, bool pixel_transfer, int n1 = 0, float2* Mp = nullptr);
+// Band of nclips host spectra (n x n interleaved complex FP64, the FFT layout
+// of Fft2D::forward) into Mq (and Mp): the |k| <= B bins, rounded to FP32 and
+// uploaded through the pinned staging area (synchronises the stream first).
+huv_status spectrum_band_stage(Ctx* c, int nclips, int n, int B, int BB, const double* spectra,
+                               float2* Mq, float2* Mp);
+
+// What a forward call images: real or complex device rasters, feature lists
+// (exact spectrum), or host spectra.
+struct MaskSource {
+  enum Kind { REAL = 0, COMPLEX, FEATURES, SPECTRUM };
+  Kind kind = REAL;
+  const float* raster = nullptr;     // REAL: [nclips][n][n]; COMPLEX: interleaved pairs
+  const RectBatch* rects = nullptr;  // FEATURES
+  const double* spectra = nullptr;   // SPECTRUM (host)
+  static MaskSource real(const float* p) { MaskSource s; s.raster = p; return s; }
+};
+
 huv_status validate_planes(Ctx* c, int n, int nplanes, huv_kset* const* planes);
 huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int** dK,
                        const float2* const** dkern, const float* const** dlam);
@@ -132,13 +153,11 @@ huv_status plane_table(Ctx* c, int nplanes, huv_kset* const* planes, const int**
 huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool paired,
                       const float2** kern, const float** lam, int* Kmax);
 // Forward images; tune = NULL uses the kernel sets' own coarse grid and the
-// default launch shapes (the ILT adjoint relies on that band layout).  With
-// `rects` the band spectrum is the exact one of those rectangle masks and
-// `masks` is not read.
-huv_status socs_forward(Ctx* c, int nclips, int n, const float* masks, int nplanes,
+// default launch shapes (the ILT adjoint relies on that band layout).
+// pixel_transfer applies to rasters only.
+huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                         huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out,
-                        const huv_socs_tuning* tune = nullptr, const RectBatch* rects = nullptr,
-                        int max_splits = 0);
+                        const huv_socs_tuning* tune = nullptr, int max_splits = 0);
 
 }  // namespace huvi
 

@@ -32,6 +32,37 @@ __global__ void k_ilt_mask(const float* __restrict__ theta, float* __restrict__ 
   }
 }
 
+// Complex absorber a (attenuated PSM): m = sigma(beta theta), t = a + (1 - a) m
+// as interleaved complex values (ilt.rs:544-552).
+__global__ void k_ilt_mask_c(const float* __restrict__ theta, float* __restrict__ m,
+                             float2* __restrict__ t, int64_t total, float beta, float ar,
+                             float ai) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float mm = 1.0f / (1.0f + expf(-beta * theta[i]));
+    if (m) m[i] = mm;
+    if (t) t[i] = make_float2(ar + (1.0f - ar) * mm, ai - ai * mm);
+  }
+}
+
+// Gaussian density filter on the band (ilt.rs:443-454): the Hermitian half
+// band Ib[t][gy + Bf][gx] (gx in [0, Bf]) = X[gx][gy] exp(-c (gx^2 + gy^2)) / n^2
+// of the band box X[t][gx + Bf][gy + Bf] of a real field (band_image input).
+__global__ void k_ilt_filter_band(const float2* __restrict__ X, int Bf, float c, float inv_n2,
+                                  int count, float2* __restrict__ Ib) {
+  const int W = 2 * Bf + 1, half = (Bf + 1) * W;
+  const int64_t total = (int64_t)count * half;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = i / half;
+    const int r = (int)(i - t * half);
+    const int gy = r / (Bf + 1) - Bf, gx = r % (Bf + 1);
+    const float h = expf(-c * (float)(gx * gx + gy * gy)) * inv_n2;
+    const float2 v = X[(size_t)t * W * W + (size_t)(gx + Bf) * W + (gy + Bf)];
+    Ib[i] = make_float2(v.x * h, v.y * h);
+  }
+}
+
 // Block-wise partial sums in FP64, fixed order within the block.
 template <int THREADS>
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -308,11 +339,195 @@ __global__ __launch_bounds__(M) void k_ilt_adjoint(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint on coarse grids M = 128, 192, 256 (band half-width B up to 63): the
+// same algebra as k_ilt_g0 / k_ilt_adjoint, with the length-M transforms done
+// by teams of threads (four-step, LDS) and the per-(image, kernel) field held
+// in HBM between passes.  A job is (image, kernel) with image = target x
+// ncond + condition and kernel k < KS (KS = the largest kernel count; jobs
+// with k >= K_c of their condition contribute zero).
+//   g0_rows / g0_cols: G0_b on the coarse grid (as k_ilt_g0), via tmp
+//                      [img][4B+1][M].
+//   adj_rows:  band row ky: T[job][ky][x] = IFFT_M(K_k . M)(x).
+//   adj_cols:  column x: E = IFFT_M over ky, times G0_b[y][x], FFT_M back,
+//              rows |fy| <= B stored in place of T.
+//   adj_out:   row fy: FFT_M over x, |fx| <= B kept, times lambda_k conj(K_k)
+//              into Rpart[img][k][fx + B][fy + B] (summed by k_ilt_hband).
+// ---------------------------------------------------------------------------
+template <int M>
+struct AdjFft {
+  static constexpr int N1 = M == 128 ? 8 : M == 192 ? 12 : 16;
+  static constexpr int N2 = M / N1;
+  static constexpr int TEAMS = 256 / N2;
+};
+constexpr int kAdjColTeams = 8;
+
+// signed frequency of FFT bin h if |f| <= band, else a value outside the band
+__device__ __forceinline__ int band_freq(int h, int M, int band) {
+  if (h <= band) return h;
+  if (h - M >= -band) return h - M;
+  return 1 << 20;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_ilt_g0_rows(const float2* __restrict__ Gb, int B,
+                                                     float flare, const float2* __restrict__ tw_m,
+                                                     float2* __restrict__ tmp) {
+  using F = AdjFft<M>;
+  __shared__ c32 scratch[F::TEAMS][(F::N1 + 1) * F::N2];
+  const int team = threadIdx.x / F::N2, t = threadIdx.x % F::N2;
+  const int G = 2 * B, W2 = 4 * B + 1;
+  const int img = blockIdx.y;
+  const int iy = blockIdx.x * F::TEAMS + team;
+  const bool active = iy < W2;
+  const float2* gb = Gb + (size_t)img * W2 * W2;
+  const int gy = iy - G;
+  fft_four_step<+1, F::N1, F::N2>(
+      t, scratch[team], tw_m,
+      [&](int h) {
+        const int gx = band_freq(h, M, G);
+        if (!active || gx > G) return c32{0.f, 0.f};
+        const float2 a = gb[(size_t)(gx + G) * W2 + iy];
+        // flare adjoint: non-DC terms scale by (1 - phi)
+        const float f = (gx == 0 && gy == 0) ? 1.0f : 1.0f - flare;
+        return c32{a.x * f, a.y * f};
+      },
+      [&](int x, c32 v) {
+        if (active) tmp[((size_t)img * W2 + iy) * M + x] = make_float2(v.re, v.im);
+      });
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_ilt_g0_cols(const float2* __restrict__ tmp, int B,
+                                                     float g0_scale, const float2* __restrict__ tw_m,
+                                                     float* __restrict__ G0g) {
+  using F = AdjFft<M>;
+  __shared__ c32 scratch[F::TEAMS][(F::N1 + 1) * F::N2];
+  const int team = threadIdx.x / F::N2, t = threadIdx.x % F::N2;
+  const int G = 2 * B, W2 = 4 * B + 1;
+  const int img = blockIdx.y;
+  const int x = blockIdx.x * F::TEAMS + team;
+  const bool active = x < M;
+  fft_four_step<+1, F::N1, F::N2>(
+      t, scratch[team], tw_m,
+      [&](int h) {
+        const int gy = band_freq(h, M, G);
+        if (!active || gy > G) return c32{0.f, 0.f};
+        const float2 a = tmp[((size_t)img * W2 + gy + G) * M + x];
+        return c32{a.x, a.y};
+      },
+      [&](int y, c32 v) {
+        if (active) G0g[((size_t)img * M + y) * M + x] = v.re * g0_scale;
+      });
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_ilt_adj_rows(
+    const float2* __restrict__ Mq, int B, int ncond, int KS, const int* __restrict__ cond_K,
+    const float2* const* __restrict__ cond_kern, const float2* __restrict__ tw_m,
+    float2* __restrict__ buf) {
+  using F = AdjFft<M>;
+  __shared__ c32 scratch[F::TEAMS][(F::N1 + 1) * F::N2];
+  const int team = threadIdx.x / F::N2, t = threadIdx.x % F::N2;
+  const int W = 2 * B + 1;
+  const int job = blockIdx.y, img = job / KS, k = job % KS;
+  const int tgt = img / ncond, c = img % ncond;
+  const int iy = blockIdx.x * F::TEAMS + team;
+  const bool active = iy < W;
+  const bool live = active && k < cond_K[c];
+  const float2* kk = cond_kern[c] + (size_t)(live ? k : 0) * W * W;
+  const float2* mq = Mq + (size_t)tgt * W * W;
+  fft_four_step<+1, F::N1, F::N2>(
+      t, scratch[team], tw_m,
+      [&](int h) {
+        const int kx = band_freq(h, M, B);
+        if (!live || kx > B) return c32{0.f, 0.f};
+        const float2 a = kk[(size_t)(kx + B) * W + iy], b = mq[(size_t)(kx + B) * W + iy];
+        return cmul(c32{a.x, a.y}, c32{b.x, b.y});
+      },
+      [&](int x, c32 v) {
+        if (active) buf[((size_t)job * W + iy) * M + x] = make_float2(v.re, v.im);
+      });
+}
+
+template <int M>
+__global__ __launch_bounds__(kAdjColTeams * AdjFft<M>::N2) void k_ilt_adj_cols(
+    const float* __restrict__ G0g, int B, int KS, const float2* __restrict__ tw_m,
+    float2* __restrict__ buf) {
+  using F = AdjFft<M>;
+  // fewer teams than the other passes: each also holds its column (M values)
+  __shared__ c32 scratch[kAdjColTeams][(F::N1 + 1) * F::N2];
+  __shared__ c32 col[kAdjColTeams][M];
+  const int team = threadIdx.x / F::N2, t = threadIdx.x % F::N2;
+  const int W = 2 * B + 1;
+  const int job = blockIdx.y, img = job / KS;
+  const int x = blockIdx.x * kAdjColTeams + team;
+  const bool active = x < M;
+  const float* g0 = G0g + (size_t)img * M * M;
+  float2* b = buf + (size_t)job * W * M;
+  // E[y][x] = IFFT over ky, times G0_b[y][x]
+  fft_four_step<+1, F::N1, F::N2>(
+      t, scratch[team], tw_m,
+      [&](int h) {
+        const int ky = band_freq(h, M, B);
+        if (!active || ky > B) return c32{0.f, 0.f};
+        const float2 a = b[(size_t)(ky + B) * M + x];
+        return c32{a.x, a.y};
+      },
+      [&](int y, c32 v) { col[team][y] = active ? cscale(v, g0[(size_t)y * M + x]) : c32{0.f, 0.f}; });
+  // forward FFT over y; rows |fy| <= B replace this column of T
+  fft_four_step<-1, F::N1, F::N2>(
+      t, scratch[team], tw_m, [&](int y) { return col[team][y]; },
+      [&](int f, c32 v) {
+        const int fy = band_freq(f, M, B);
+        if (active && fy <= B) b[(size_t)(fy + B) * M + x] = make_float2(v.re, v.im);
+      });
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_ilt_adj_out(
+    const float2* __restrict__ buf, int B, int ncond, int KS, const int* __restrict__ cond_K,
+    const float2* const* __restrict__ cond_kern, const float* const* __restrict__ cond_lam,
+    float lam_scale, const float2* __restrict__ tw_m, float2* __restrict__ Rpart) {
+  using F = AdjFft<M>;
+  __shared__ c32 scratch[F::TEAMS][(F::N1 + 1) * F::N2];
+  const int team = threadIdx.x / F::N2, t = threadIdx.x % F::N2;
+  const int W = 2 * B + 1;
+  const int job = blockIdx.y, img = job / KS, k = job % KS;
+  const int c = img % ncond;
+  const int fyi = blockIdx.x * F::TEAMS + team;  // fy + B
+  const bool active = fyi < W;
+  const bool live = active && k < cond_K[c];
+  const float2* kk = cond_kern[c] + (size_t)(live ? k : 0) * W * W;
+  const float l = live ? cond_lam[c][k] * lam_scale : 0.0f;
+  const float2* row = buf + ((size_t)job * W + (active ? fyi : 0)) * M;
+  float2* ro = Rpart + (size_t)job * W * W;
+  fft_four_step<-1, F::N1, F::N2>(
+      t, scratch[team], tw_m,
+      [&](int x) {
+        if (!live) return c32{0.f, 0.f};
+        const float2 a = row[x];
+        return c32{a.x, a.y};
+      },
+      [&](int f, c32 v) {
+        const int fx = band_freq(f, M, B);
+        if (!active || fx > B) return;
+        c32 r{0.f, 0.f};
+        if (live) {
+          const float2 kv = kk[(size_t)(fx + B) * W + fyi];
+          r = cscale(cconjmul(c32{kv.x, kv.y}, v), l);
+        }
+        ro[(size_t)(fx + B) * W + fyi] = make_float2(r.re, r.im);
+      });
+}
+
 // H[fy][fx] (fx in [0, B], fy in [-B, B]) = sum_c w_c (1-a) (R_c[f] + conj R_c[-f]) / n^2,
 // layout [fy + B][fx] (the Hermitian half band read by band_image, half-width B).
 // R_c = sum over the KS partial bands of image (tgt, c), in split order.
+// With a complex absorber a and c = 1 - a (ilt.rs:672,715), 2 Re[conj(c) Q]
+// has the spectrum conj(c) R(f) + c conj(R(-f)); inv_n2 = 1 / n^2.
 __global__ void k_ilt_hband(const float2* __restrict__ Rin, int B, int ncond, int KS,
-                            const float* __restrict__ weight, float scale,
+                            const float* __restrict__ weight, float cr, float ci, float inv_n2,
                             float2* __restrict__ H) {
   const int tgt = blockIdx.y;
   const int W = 2 * B + 1;
@@ -329,9 +544,48 @@ __global__ void k_ilt_hband(const float2* __restrict__ Rin, int B, int ncond, in
       p = cadd(p, c32{a.x, a.y});
       q = cadd(q, c32{b.x, b.y});
     }
-    acc = cadd(acc, cscale(c32{p.re + q.re, p.im - q.im}, weight[c]));
+    // conj(c) p + c conj(q)
+    const c32 u = cconjmul(c32{cr, ci}, p);
+    const c32 v = cmul(c32{cr, ci}, c32{q.re, -q.im});
+    acc = cadd(acc, cscale(cadd(u, v), weight[c]));
   }
-  H[(size_t)tgt * (B + 1) * W + idx] = make_float2(acc.re * scale, acc.im * scale);
+  H[(size_t)tgt * (B + 1) * W + idx] = make_float2(acc.re * inv_n2, acc.im * inv_n2);
+}
+
+// Local proxy gradient (ilt.rs:719-721): dcdm[t] = sum_c w_c G0_c with
+// G0 = (1 - phi) G + (phi / N) sum G; gsum[img] are the per-image sums of G
+// (k_ilt_gsum partials, summed here in block order).
+__global__ __launch_bounds__(256) void k_ilt_gsum(const float* __restrict__ G, int64_t npix,
+                                                  int bpi, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int img = blockIdx.y;
+  const float* g = G + (size_t)img * npix;
+  const int64_t chunk = (npix + bpi - 1) / bpi;
+  const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
+  double acc = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) acc += (double)g[i];
+  const double s = block_sum<256>(acc, sh);
+  if (threadIdx.x == 0) part[(size_t)img * bpi + blockIdx.x] = s;
+}
+
+__global__ void k_ilt_proxy(const float* __restrict__ G, const double* __restrict__ part,
+                            int ntargets, int ncond, int64_t npix, int bpi,
+                            const float* __restrict__ weight, float flare,
+                            float* __restrict__ dcdm) {
+  const int64_t total = (int64_t)ntargets * npix;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = i / npix, p = i - t * npix;
+    float acc = 0.0f;
+    for (int c = 0; c < ncond; ++c) {
+      const size_t img = (size_t)t * ncond + c;
+      double s = 0.0;
+      for (int b = 0; b < bpi; ++b) s += part[img * bpi + b];
+      const float g0 = (1.0f - flare) * G[img * npix + p] + flare * (float)(s / (double)npix);
+      acc += weight[c] * g0;
+    }
+    dcdm[i] = acc;
+  }
 }
 
 // grad = (dcdm + reg) * beta m (1 - m).
@@ -528,11 +782,74 @@ hipError_t launch_ilt_adjoint(int m, int nimg, int KS, const float2* Mq, const f
   }
 }
 
+template <int M>
+static hipError_t adjoint_big_m(int nimg, int KS, const float2* Mq, const float2* Gb, int B,
+                                int ncond, const int* cond_K, const float2* const* cond_kern,
+                                const float* const* cond_lam, float lam_scale, float g0_scale,
+                                float flare, const float2* tw_m, float2* g0tmp, float* G0g,
+                                float2* buf, float2* Rpart, hipStream_t s) {
+  using F = AdjFft<M>;
+  const int W = 2 * B + 1, W2 = 4 * B + 1, jobs = nimg * KS;
+  auto blocks = [](int count, int teams) { return (unsigned)((count + teams - 1) / teams); };
+  hipLaunchKernelGGL(k_ilt_g0_rows<M>, dim3(blocks(W2, F::TEAMS), nimg), dim3(256), 0, s, Gb, B,
+                     flare, tw_m, g0tmp);
+  hipLaunchKernelGGL(k_ilt_g0_cols<M>, dim3(blocks(M, F::TEAMS), nimg), dim3(256), 0, s, g0tmp, B,
+                     g0_scale, tw_m, G0g);
+  hipLaunchKernelGGL(k_ilt_adj_rows<M>, dim3(blocks(W, F::TEAMS), jobs), dim3(256), 0, s, Mq, B,
+                     ncond, KS, cond_K, cond_kern, tw_m, buf);
+  hipLaunchKernelGGL(k_ilt_adj_cols<M>, dim3(blocks(M, kAdjColTeams), jobs),
+                     dim3(kAdjColTeams * F::N2), 0, s, G0g, B, KS, tw_m, buf);
+  hipLaunchKernelGGL(k_ilt_adj_out<M>, dim3(blocks(W, F::TEAMS), jobs), dim3(256), 0, s, buf, B,
+                     ncond, KS, cond_K, cond_kern, cond_lam, lam_scale, tw_m, Rpart);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_adjoint_big(int m, int nimg, int KS, const float2* Mq, const float2* Gb,
+                                  int B, int ncond, const int* cond_K,
+                                  const float2* const* cond_kern, const float* const* cond_lam,
+                                  float lam_scale, float g0_scale, float flare,
+                                  const float2* tw_m, float2* g0tmp, float* G0g, float2* buf,
+                                  float2* Rpart, hipStream_t s) {
+  if (KS < 1 || nimg < 1 || (int64_t)nimg * KS > 65535 || 4 * B + 1 > m) return hipErrorInvalidValue;
+  switch (m) {
+    case 128: return adjoint_big_m<128>(nimg, KS, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, tw_m, g0tmp, G0g, buf, Rpart, s);
+    case 192: return adjoint_big_m<192>(nimg, KS, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, tw_m, g0tmp, G0g, buf, Rpart, s);
+    case 256: return adjoint_big_m<256>(nimg, KS, Mq, Gb, B, ncond, cond_K, cond_kern, cond_lam, lam_scale, g0_scale, flare, tw_m, g0tmp, G0g, buf, Rpart, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, int KS,
-                            const float* weight, float scale, float2* H, hipStream_t s) {
+                            const float* weight, float cr, float ci, float inv_n2, float2* H,
+                            hipStream_t s) {
   const int cnt = (B + 1) * (2 * B + 1);
   hipLaunchKernelGGL(k_ilt_hband, dim3((cnt + 127) / 128, ntargets), dim3(128), 0, s, Rin, B,
-                     ncond, KS, weight, scale, H);
+                     ncond, KS, weight, cr, ci, inv_n2, H);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_mask_c(const float* theta, float* m, float2* t, int64_t total, float beta,
+                             float ar, float ai, hipStream_t s) {
+  hipLaunchKernelGGL(k_ilt_mask_c, dim3(grid_for(total, 256)), dim3(256), 0, s, theta, m, t,
+                     total, beta, ar, ai);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_filter_band(const float2* X, int Bf, float c, float inv_n2, int count,
+                                  float2* Ib, hipStream_t s) {
+  const int64_t total = (int64_t)count * (Bf + 1) * (2 * Bf + 1);
+  hipLaunchKernelGGL(k_ilt_filter_band, dim3(grid_for(total, 256)), dim3(256), 0, s, X, Bf, c,
+                     inv_n2, count, Ib);
+  return hipGetLastError();
+}
+
+hipError_t launch_ilt_proxy(const float* G, int ntargets, int ncond, int64_t npix, int bpi,
+                            const float* weight, float flare, double* part, float* dcdm,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_ilt_gsum, dim3(bpi, ntargets * ncond), dim3(256), 0, s, G, npix, bpi,
+                     part);
+  hipLaunchKernelGGL(k_ilt_proxy, dim3(grid_for((int64_t)ntargets * npix, 256)), dim3(256), 0, s,
+                     G, part, ntargets, ncond, npix, bpi, weight, flare, dcdm);
   return hipGetLastError();
 }
 

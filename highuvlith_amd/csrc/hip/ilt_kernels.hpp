@@ -25,8 +25,32 @@ hipError_t launch_ilt_adjoint(int m, int nimg, int KS, const float2* Mq, const f
                               int ncond, const int* cond_K, const float2* const* cond_kern,
                               const float* const* cond_lam, float lam_scale, float g0_scale,
                               float flare, float* G0g, float2* Rpart, hipStream_t s);
+// The same adjoint for coarse grids m = 128, 192, 256: team FFTs through HBM
+// (g0tmp: nimg x (4B+1) x m complex; buf: nimg x KS x (2B+1) x m complex;
+// tw_m: the W_m table); Rpart holds one band per kernel (KS = the largest
+// kernel count, zero bands past a condition's own count).
+hipError_t launch_ilt_adjoint_big(int m, int nimg, int KS, const float2* Mq, const float2* Gb,
+                                  int B, int ncond, const int* cond_K,
+                                  const float2* const* cond_kern, const float* const* cond_lam,
+                                  float lam_scale, float g0_scale, float flare,
+                                  const float2* tw_m, float2* g0tmp, float* G0g, float2* buf,
+                                  float2* Rpart, hipStream_t s);
+// H = sum_c w_c (conj(c) R_c(f) + c conj(R_c(-f))) / n^2, c = cr + i ci = 1 - absorber.
 hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, int KS,
-                            const float* weight, float scale, float2* H, hipStream_t s);
+                            const float* weight, float cr, float ci, float inv_n2, float2* H,
+                            hipStream_t s);
+// m = sigma(beta theta) (m may be null) and t = a + (1 - a) m as interleaved
+// complex values (t may be null).
+hipError_t launch_ilt_mask_c(const float* theta, float* m, float2* t, int64_t total, float beta,
+                             float ar, float ai, hipStream_t s);
+// Density-filter band for band_image: Ib[t][gy + Bf][gx] = X[gx + Bf][gy + Bf]
+// exp(-c (gx^2 + gy^2)) inv_n2 for count boxes X of half-width Bf.
+hipError_t launch_ilt_filter_band(const float2* X, int Bf, float c, float inv_n2, int count,
+                                  float2* Ib, hipStream_t s);
+// Local-proxy dC/dm from G (ntargets x ncond images); part: ntargets x ncond x bpi.
+hipError_t launch_ilt_proxy(const float* G, int ntargets, int ncond, int64_t npix, int bpi,
+                            const float* weight, float flare, double* part, float* dcdm,
+                            hipStream_t s);
 hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m, float beta,
                             int64_t total, float* grad, hipStream_t s);
 // Per-target vector operations on `count` target rows listed in `rows`

@@ -122,6 +122,91 @@ __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
 }
 
 // ---------------------------------------------------------------------------
+// Complex transmittance rasters (compute_from_transmittance with an
+// Array2<Complex64> map, aerial.rs:1789-1799): no Hermitian symmetry, so every
+// row is its own complex transform and both signs of kx are column passes.
+// rows_band_c: row r of clip c (interleaved complex FP32), FFT_n forward,
+// keep k in [0, B] and [n - B, n); tmp layout [clip][kk][row], kk as in
+// rows_band (kk <= B -> k = kk, kk > B -> k = n - (kk - B)).
+// ---------------------------------------------------------------------------
+template <int N1, int N2, int TEAMS>
+__global__ __launch_bounds__(TEAMS * N2) void k_rows_band_c(const float2* __restrict__ masks,
+                                                            int B, const float2* __restrict__ tw_n,
+                                                            float2* __restrict__ tmp) {
+  constexpr int N = N1 * N2;
+  __shared__ c32 scratch[TEAMS][(N1 + 1) * N2];
+  const int team = threadIdx.x / N2, t = threadIdx.x % N2;
+  const int clip = blockIdx.y;
+  const int row = blockIdx.x * TEAMS + team;
+  const bool active = row < N;
+  const float* r = reinterpret_cast<const float*>(masks + ((size_t)clip * N + (active ? row : 0)) * N);
+  const int nk = 2 * B + 1;
+  float2* out = tmp + (size_t)clip * nk * N;
+  fft_four_step<-1, N1, N2>(
+      t, scratch[team], tw_n,
+      [&](int i) {
+        return active ? c32{__builtin_nontemporal_load(r + 2 * i),
+                            __builtin_nontemporal_load(r + 2 * i + 1)}
+                      : c32{0.f, 0.f};
+      },
+      [&](int k, c32 v) {
+        if (!active) return;
+        int kk = -1;
+        if (k <= B) kk = k;
+        else if (k >= N - B) kk = B + (N - k);
+        if (kk >= 0) out[(size_t)kk * N + row] = make_float2(v.re, v.im);
+      });
+}
+
+// cols_band_c: box column col = kx + BB for kx in [-BB, BB]: FFT_n forward of
+// the row-spectrum column kx along y, keep ky in [-BB, BB]; the box holds
+// M[ky][kx] * pt for |kx|, |ky| <= B and zero elsewhere (plain and, when Mp
+// is set, column-paired with its zero column in front, as cols_band writes).
+template <int N1, int N2, int TEAMS>
+__global__ __launch_bounds__(TEAMS * N2) void k_cols_band_c(
+    const float2* __restrict__ tmp, int B, int BB, const float2* __restrict__ tw_n,
+    const float* __restrict__ sinc_t, float2* __restrict__ Mq, float2* __restrict__ Mp) {
+  constexpr int N = N1 * N2;
+  __shared__ c32 scratch[TEAMS][(N1 + 1) * N2];
+  const int team = threadIdx.x / N2, t = threadIdx.x % N2;
+  const int clip = blockIdx.y;
+  const int W = 2 * BB + 1;
+  const int col = blockIdx.x * TEAMS + team;
+  const bool active = col < W;
+  const int kx = col - BB;
+  const bool live = active && kx >= -B && kx <= B;
+  const int kk = !live ? 0 : (kx >= 0 ? kx : B - kx);
+  const int nk = 2 * B + 1;
+  const float2* src = tmp + ((size_t)clip * nk + kk) * N;
+  float2* mq = Mq + (size_t)clip * W * W;
+  float2* mp = Mp ? Mp + (size_t)clip * (W + 1) * W : nullptr;
+  auto put_p = [&](int c, int r, float2 v) {
+    mp[((size_t)((c + 1) >> 1) * W + r) * 2 + ((c + 1) & 1)] = v;
+  };
+  fft_four_step<-1, N1, N2>(
+      t, scratch[team], tw_n,
+      [&](int i) {
+        if (!live) return c32{0.f, 0.f};
+        const float2 z = src[i];
+        return c32{z.x, z.y};
+      },
+      [&](int k, c32 v) {
+        if (!active) return;
+        int ky;
+        if (k <= BB) ky = k;
+        else if (k >= N - BB) ky = k - N;
+        else return;
+        if (live && ky >= -B && ky <= B) v = cscale(v, sinc_t[kx + B] * sinc_t[ky + B]);
+        else v = c32{0.f, 0.f};
+        mq[(size_t)col * W + (ky + BB)] = make_float2(v.re, v.im);
+        if (mp) {
+          put_p(col, ky + BB, make_float2(v.re, v.im));
+          if (col == 0) put_p(-1, ky + BB, make_float2(0.f, 0.f));  // the zero column
+        }
+      });
+}
+
+// ---------------------------------------------------------------------------
 // socs_fields: one work-group per (clip, x class q), all planes of the clip.
 // Coarse grid M = 32 Q, Q = R = M / 32 classes and residues, band half-width
 // BB = (M - 1) / 4 at compile time (kernel boxes are zero beyond the set's
@@ -137,7 +222,9 @@ __global__ __launch_bounds__(TEAMS * N2) void k_cols_band(
 //   E[R j' + r][Q j + q] = IFFT_32(d)[j'];  acc += lambda |E|^2
 // The groups run software-pipelined over a double-buffered LDS tile (step s
 // synthesises kernel s while kernel s - 1 is squared and summed), one barrier
-// per step.  Coarse intensity layout: coarse[img][y][x].
+// per step.  Coarse intensity layout: class-major rows, coarse[img][y][q][j]
+// for x = Q j + q, so the 32 lanes of a residue store one contiguous 128-byte
+// run per row (coarse_rows reads the row back in x order).
 // Work-groups are ordered so that the Q classes of one clip share an XCD
 // (consecutive work-group ids are dealt round-robin over the 8 XCDs).
 // ---------------------------------------------------------------------------
@@ -413,7 +500,7 @@ __global__ __launch_bounds__((CoarseCfg<M, XS>::THREADS)) __attribute__((amdgpu_
           const __amdgpu_buffer_rsrc_t orc = __builtin_amdgcn_make_buffer_rsrc(
               coarse + (((size_t)sj * nclips + clip) * nplanes + sp / kc) * M * M, (short)0,
               M * M * (int)sizeof(float), 0x00020000);
-          const int vo = (r * M + Q * j + q) * (int)sizeof(float);
+          const int vo = (r * M + F * q + j) * (int)sizeof(float);
 #pragma unroll
           for (int i = 0; i < F; ++i) {
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, acc[i]), orc, vo,
@@ -459,7 +546,7 @@ template <int M, int TEAMS>
 __global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_rows(
     const float* __restrict__ coarse, int B, const float2* __restrict__ tw_m,
     float2* __restrict__ tmp1) {
-  constexpr int N1 = CoarseFft<M>::N1, N2 = CoarseFft<M>::N2;
+  constexpr int N1 = CoarseFft<M>::N1, N2 = CoarseFft<M>::N2, Q = M / 32;
   __shared__ c32 scratch[TEAMS][(N1 + 1) * N2];
   const int team = threadIdx.x / N2, t = threadIdx.x % N2;
   const int img = blockIdx.y;
@@ -473,7 +560,11 @@ __global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_rows(
   float2* out = tmp1 + (size_t)img * nk * npairs;
   fft_four_step<-1, N1, N2>(
       t, scratch[team], tw_m,
-      [&](int x) { return active ? c32{ra[x], rb[x]} : c32{0.f, 0.f}; },
+      [&](int x) {
+        // class-major rows: x = Q j + q is stored at 32 q + j (socs_fields)
+        const int at = (x % Q) * 32 + x / Q;
+        return active ? c32{ra[at], rb[at]} : c32{0.f, 0.f};
+      },
       [&](int k, c32 v) {
         if (!active) return;
         int kk = -1;
@@ -763,6 +854,35 @@ hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const floa
                             hipStream_t s) {
   if (BB < B || 2 * BB + 1 > n) return hipErrorInvalidValue;
   HUV_BAND_FFT_SWITCH(cols_band_n, B, BB, nclips, tmp, tw_n, sinc_t, Mq, Mp, s)
+}
+
+template <int N1, int N2>
+static hipError_t rows_band_c_n(int B, int nclips, const float2* masks, const float2* tw,
+                                float2* tmp, hipStream_t s) {
+  constexpr int TEAMS = 128 / N2 > 0 ? 128 / N2 : 1;
+  dim3 grid((N1 * N2 + TEAMS - 1) / TEAMS, nclips);
+  HUV_LAUNCH((k_rows_band_c<N1, N2, TEAMS>), grid, dim3(TEAMS * N2), 0, s, masks, B, tw, tmp);
+}
+
+hipError_t launch_rows_band_c(int n, int n1, int B, int nclips, const float2* masks,
+                              const float2* tw_n, float2* tmp, hipStream_t s) {
+  HUV_BAND_FFT_SWITCH(rows_band_c_n, B, nclips, masks, tw_n, tmp, s)
+}
+
+template <int N1, int N2>
+static hipError_t cols_band_c_n(int B, int BB, int nclips, const float2* tmp, const float2* tw_n,
+                                const float* sinc_t, float2* Mq, float2* Mp, hipStream_t s) {
+  constexpr int TEAMS = 128 / N2 > 0 ? 128 / N2 : 1;
+  dim3 grid((2 * BB + 1 + TEAMS - 1) / TEAMS, nclips);
+  HUV_LAUNCH((k_cols_band_c<N1, N2, TEAMS>), grid, dim3(TEAMS * N2), 0, s, tmp, B, BB, tw_n,
+             sinc_t, Mq, Mp);
+}
+
+hipError_t launch_cols_band_c(int n, int n1, int B, int BB, int nclips, const float2* tmp,
+                              const float2* tw_n, const float* sinc_t, float2* Mq, float2* Mp,
+                              hipStream_t s) {
+  if (BB < B || 2 * BB + 1 > n) return hipErrorInvalidValue;
+  HUV_BAND_FFT_SWITCH(cols_band_c_n, B, BB, nclips, tmp, tw_n, sinc_t, Mq, Mp, s)
 }
 
 template <int M, int XNB, int XS = 1>

@@ -38,9 +38,18 @@ hipError_t launch_rows_band(int n, int n1, int B, int nclips, const float* masks
 hipError_t launch_cols_band(int n, int n1, int B, int BB, int nclips, const float2* tmp,
                             const float2* tw_n, const float* sinc_t, float2* Mq, float2* Mp,
                             hipStream_t s);
+// Complex rasters (masks: [nclips][n][n] interleaved complex): every row is
+// transformed (tmp: [nclips][2B+1][n]) and both signs of kx are column
+// passes; Mq / Mp as for launch_cols_band.
+hipError_t launch_rows_band_c(int n, int n1, int B, int nclips, const float2* masks,
+                              const float2* tw_n, float2* tmp, hipStream_t s);
+hipError_t launch_cols_band_c(int n, int n1, int B, int BB, int nclips, const float2* tmp,
+                              const float2* tw_n, const float* sinc_t, float2* Mq, float2* Mp,
+                              hipStream_t s);
 // kern: [Q][nplanes][Kmax] boxes (W = 2 coarse_band(m) + 1, Q = m / 32; copy q
 // multiplied by e^{2 pi i kx q / m}), lam: [nplanes][Kmax]; Mq: one box per
-// clip; coarse: [nclips][nplanes][m][m].  Boxes are W x W ([kx + BB][ky + BB])
+// clip; coarse: [nclips][nplanes][m][Q][32], row y, column x = Q j + q at
+// [y][q][j] (class-major).  Boxes are W x W ([kx + BB][ky + BB])
 // for xlanes = 1 and column-paired (W + 1) x W (launch_cols_band's Mp) for 2.
 // S kernel splits (fields_splits): coarse then holds S partial image sets,
 // [S][nclips][nplanes][m][m], summed by launch_coarse_rows.

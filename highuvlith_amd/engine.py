@@ -403,6 +403,46 @@ def socs_images_device(ctx, masks, planes, out, pixel_transfer=False, graph=Fals
     return out
 
 
+def socs_images_complex_device(ctx, masks, planes, out, pixel_transfer=False, graph=False):
+    """Images of complex transmittance rasters: masks (N, n, n) complex64 CUDA tensor,
+    out (N, P, n, n) f32; on torch's current stream (aerial.rs:1789-1799)."""
+    import torch
+    assert masks.is_cuda and masks.dtype == torch.complex64 and masks.is_contiguous()
+    assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
+    nclips, n = masks.shape[0], masks.shape[-1]
+    if out.shape != (nclips, len(planes), n, n):
+        raise HuvError(2, f"output shape {tuple(out.shape)} != {(nclips, len(planes), n, n)}")
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
+    opts = _lib.huv_socs_opts(int(bool(pixel_transfer)), int(bool(graph)))
+    check(ctx.lib.huv_socs_images_complex(ctx.handle, nclips, n, C.c_void_p(masks.data_ptr()),
+                                          len(planes), arr, C.c_void_p(out.data_ptr()),
+                                          C.byref(opts)))
+    return out
+
+
+def socs_images_spectrum(ctx, spectra, planes, out):
+    """Images from host mask spectra: spectra (N, n, n) complex128 numpy in
+    Fft2D::forward's layout and scale, out (N, P, n, n) f32 CUDA tensor
+    (compute_from_spectrum, aerial.rs:1805-1818).  Synchronises torch's current
+    stream before staging."""
+    import torch
+    spectra = np.ascontiguousarray(spectra, dtype=np.complex128)
+    if spectra.ndim == 2:
+        spectra = spectra[None]
+    nclips, n = spectra.shape[0], spectra.shape[-1]
+    assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous()
+    if out.shape != (nclips, len(planes), n, n) or spectra.shape[1] != n:
+        raise HuvError(2, f"spectrum {spectra.shape} / output {tuple(out.shape)} do not match "
+                          f"{len(planes)} planes")
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    arr = (C.c_void_p * len(planes))(*[p.handle.value for p in planes])
+    check(ctx.lib.huv_socs_images_spectrum(ctx.handle, nclips, n,
+                                           spectra.view(np.float64).ctypes.data_as(_lib.PD),
+                                           len(planes), arr, C.c_void_p(out.data_ptr())))
+    return out
+
+
 def socs_images_batch_device(ctx, groups, pixel_transfer=False, graph=False):
     """Uneven batch: groups = [(masks (N_i, n_i, n_i), planes_i, out (N_i, P_i, n_i, n_i)), ...]
     of torch CUDA f32 tensors, all issued on torch's current stream; with
@@ -816,15 +856,54 @@ class AerialImageEngine:
         return socs_images_features_device(self.context(), self.grid, rect_arrays(clips, bgs),
                                            PolygonArrays(polys), planes, out)
 
+    def compute_batch_complex(self, masks, defocus_nm=(0.0,), pixel_transfer=False, out=None,
+                              graph=False):
+        """compute_batch for complex transmittance maps: masks (N, n, n) complex
+        (torch complex64 CUDA tensor, or numpy, copied).  Returns (N, P, n, n)."""
+        import torch
+        if not isinstance(masks, torch.Tensor):
+            masks = torch.as_tensor(np.ascontiguousarray(masks, dtype=np.complex64),
+                                    device=f"cuda:{self.device}")
+        if masks.dim() == 2:
+            masks = masks[None]
+        n = self.grid.size
+        if masks.shape[-2:] != (n, n):
+            raise HuvError(2, f"transmittance map must match the engine grid ({n}x{n})")
+        masks = masks.to(torch.complex64).contiguous()
+        planes = [self.device_kernel_set(z) for z in defocus_nm]
+        if out is None:
+            out = torch.empty((masks.shape[0], len(planes), n, n), dtype=torch.float32,
+                              device=masks.device)
+        return socs_images_complex_device(self.context(), masks, planes, out, pixel_transfer,
+                                          graph)
+
+    def compute_from_spectrum_batch(self, spectra, defocus_nm=(0.0,)):
+        """compute_from_spectrum (aerial.rs:1805-1818) for (N, n, n) host spectra in
+        Fft2D::forward's layout and scale; returns a torch tensor (N, P, n, n)."""
+        import torch
+        spectra = np.asarray(spectra)
+        if spectra.ndim == 2:
+            spectra = spectra[None]
+        n = self.grid.size
+        if spectra.shape[-2:] != (n, n):
+            raise HuvError(2, f"mask spectrum must match the engine grid ({n}x{n})")
+        planes = [self.device_kernel_set(z) for z in defocus_nm]
+        out = torch.empty((spectra.shape[0], len(planes), n, n), dtype=torch.float32,
+                          device=f"cuda:{self.device}")
+        return socs_images_spectrum(self.context(), spectra, planes, out)
+
+    def compute_from_spectrum(self, spectrum, defocus_nm=0.0):
+        return self.compute_from_spectrum_batch(spectrum, (defocus_nm,))[0, 0].cpu().numpy()
+
     def compute_from_transmittance(self, transmittance, defocus_nm=0.0):
-        t = np.asarray(transmittance)
-        if np.iscomplexobj(t):
-            if np.any(t.imag != 0):
-                raise HuvError(8, "complex transmittance is not supported by the GPU path")
-            t = t.real
-        return self.compute_batch(t, (defocus_nm,))[0, 0].cpu().numpy()
+        """aerial.rs:1789-1799: real maps take the real-raster path, complex ones
+        (any nonzero imaginary part) the complex-raster path."""
+        return self.compute_through_focus_from_transmittance(transmittance, (defocus_nm,))[0]
 
     def compute_through_focus_from_transmittance(self, transmittance, defocus_nm):
-        t = np.asarray(transmittance).real
-        imgs = self.compute_batch(t, tuple(defocus_nm))[0].cpu().numpy()
+        t = np.asarray(transmittance)
+        if np.iscomplexobj(t) and np.any(t.imag != 0):
+            imgs = self.compute_batch_complex(t, tuple(defocus_nm))[0].cpu().numpy()
+        else:
+            imgs = self.compute_batch(np.real(t), tuple(defocus_nm))[0].cpu().numpy()
         return [imgs[i] for i in range(len(defocus_nm))]

@@ -306,8 +306,14 @@ typedef struct huv_ilt huv_ilt;
 /* IltProblem settings (ilt.rs:159-224).  cost_kind 0 = image fidelity,
  * 1 = sigmoid resist contour (threshold, steepness).  beta = mask sigmoid
  * steepness; tv_weight / tv_epsilon / bin_weight = regularisers; absorber =
- * amplitude of the m = 0 state (real values only on the device path).  The
- * Gaussian density filter (min_feature_nm) is not part of the device path. */
+ * complex amplitude of the m = 0 state (0 binary, sqrt(T) e^{i phi} for an
+ * attenuated PSM).  min_feature_nm > 0 adds the Gaussian density filter
+ * m = sigma(beta G_s * theta), s = min_feature_nm / 1.349, on a grid of
+ * pixel_nm pixels (ilt.rs:443-454, 524-535); its spectrum is applied on the
+ * bins where it exceeds 1e-12 (HUV_UNSUPPORTED when that band is wider than
+ * 127 bins).  gradient 0 = exact adjoint, 1 = the local proxy (the flare-adjoint
+ * data gradient dC/dI in place of dC/dm, ilt.rs:719-721; steepest descent,
+ * steps accepted on plain decrease, ilt.rs:789-801). */
 typedef struct {
   int cost_kind;
   double threshold;
@@ -318,11 +324,16 @@ typedef struct {
   double bin_weight;
   double absorber_re;
   double absorber_im;
+  double min_feature_nm;
+  double pixel_nm;
+  int gradient;
 } huv_ilt_config;
 
 /* Bind process conditions (one kernel set per focus, from huv_kset_upload
  * with eigenvalues divided by the clear field) with dose and weight
- * (replaces IltProblem::new, ilt.rs:389-478).  Coarse grids up to 64. */
+ * (replaces IltProblem::new, ilt.rs:389-478).  Coarse grids up to 256 (band
+ * half-widths up to 63); conditions x targets x kernels per evaluation up to
+ * 65535 on coarse grids above 64. */
 huv_status huv_ilt_create(huv_ctx* ctx, int n, int nconds, huv_kset* const* conds,
                           const double* dose, const double* weight, const huv_ilt_config* cfg,
                           huv_ilt** out);
@@ -349,6 +360,28 @@ huv_status huv_ilt_optimize_to(huv_ilt* ilt, int ntargets, const float* targets,
                                int max_iterations, double learning_rate, double convergence_tol,
                                int use_cg, const double* stop_cost, double* cost_hist,
                                int* iterations, int* termination);
+/* run_stage with its trace (ilt.rs:755-866 with IltTrace, ilt.rs:745-750):
+ * as huv_ilt_optimize_to with stop_cost nullable, plus gnorm_hist (host,
+ * nullable, ntargets x (max_iterations + 1): the L2 norm of dC/dtheta at the
+ * start and after each iteration, NaN after a target stops) and mask
+ * snapshots: with snapshot_every > 0, target t's mask sigma(beta G_s * theta)
+ * after its global iteration g = iteration_offset[t] + i is written to
+ * snapshots (DEVICE, ntargets x nslots x n x n) slot g / snapshot_every - 1
+ * whenever g is a multiple of snapshot_every and that slot is < nslots.
+ * iteration_offset (host, nullable = zeros) holds the iterations each target
+ * ran in earlier stages: max_iterations is then the whole budget, and target t
+ * runs at most max_iterations - iteration_offset[t] iterations here. */
+huv_status huv_ilt_run_stage(huv_ilt* ilt, int ntargets, const float* targets, float* theta,
+                             int max_iterations, double learning_rate, double convergence_tol,
+                             int use_cg, const double* stop_cost, double* cost_hist,
+                             double* gnorm_hist, int snapshot_every, const int* iteration_offset,
+                             int nslots, float* snapshots, int* iterations, int* termination);
+/* Binarisation weight of later evaluations (the continuation of optimize_ilt,
+ * ilt.rs:894-915, switches it between stages). */
+huv_status huv_ilt_set_binarization(huv_ilt* ilt, double bin_weight);
+/* m = sigma(beta G_s * theta) (IltProblem::mask, ilt.rs:537-542): theta and m
+ * DEVICE, ntargets x n x n. */
+huv_status huv_ilt_mask(huv_ilt* ilt, int ntargets, const float* theta, float* m);
 
 /* ======================================================================== */
 /* U-Net warm start for ILT                                                  */

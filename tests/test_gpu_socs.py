@@ -200,3 +200,83 @@ def test_one_context_shared_by_threads_is_deterministic():
         assert np.array_equal(o[0][1], seq), z
     ref = _oracle_images(eng, masks[:1], (90.0,))
     assert np.abs(outs[3][0][1][0, 0] - ref[0, 0]).max() <= TOL
+
+
+def test_complex_transmittance_rasters_match_oracle():
+    """Complex transmittance maps (compute_from_transmittance with an
+    Array2<Complex64>, aerial.rs:1789-1799): an attenuated phase-shift mask
+    (absorber amplitude 0.245 e^{i pi}) with a smooth random phase ramp, two
+    focus planes, with and without the pixel transfer, eager and graph
+    replays; a real map sent through the complex path equals the real path."""
+    import torch
+    eng = _engine(512, 2.0, max_kernels=24)
+    rng = np.random.default_rng(31)
+    n = 512
+    yy, xx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    masks = []
+    for c in range(3):
+        binary = _manhattan(n, 10, rng, 16, 128)
+        amp = np.where(binary > 0, 1.0 + 0j, 0.245 * np.exp(1j * np.pi))
+        phase = np.exp(1j * 0.3 * np.sin(2 * np.pi * (xx * (c + 1) + yy * 2) / n))
+        masks.append(amp * phase)
+    masks = np.stack(masks).astype(np.complex64)
+    zs = (0.0, 40.0)
+    for pt in (False, True):
+        got = eng.compute_batch_complex(masks, zs, pixel_transfer=pt).cpu().numpy()
+        worst = 0.0
+        for c in range(3):
+            spec = oracle.fft2(masks[c].astype(np.complex128))
+            if pt:
+                spec = oracle.pixel_transfer(spec)
+            for p, z in enumerate(zs):
+                ref = oracle.socs_image(eng.kernel_set(z), spec, eng.flare_fraction(), threads=8)
+                worst = max(worst, float(np.abs(got[c, p] - ref).max()))
+        print("complex raster, pixel transfer", pt, "max err", worst)
+        assert worst <= TOL
+    # graph replays equal eager calls
+    dm = torch.from_numpy(masks).cuda()
+    planes = [eng.device_kernel_set(z) for z in zs]
+    ctx = eng.context()
+    eager = torch.empty((3, 2, n, n), device="cuda")
+    H.engine.socs_images_complex_device(ctx, dm, planes, eager)
+    for _ in range(3):
+        outg = torch.full_like(eager, float("nan"))
+        H.engine.socs_images_complex_device(ctx, dm, planes, outg, graph=True)
+        torch.cuda.synchronize()
+        assert (outg - eager).abs().max().item() == 0.0
+    # a real map through the complex path equals the real path
+    real = np.stack([_manhattan(n, 10, rng, 16, 128) for _ in range(2)])
+    a = eng.compute_batch(real, zs).cpu().numpy()
+    b = eng.compute_batch_complex(real.astype(np.complex64), zs).cpu().numpy()
+    print("real map, complex vs real path", np.abs(a - b).max())
+    assert np.abs(a - b).max() <= 1e-6
+    # the engine's transmittance entry routes complex maps to this path (no pixel transfer)
+    img = eng.compute_from_transmittance(masks[1].astype(np.complex128), 40.0)
+    assert np.abs(img - eager[1, 1].cpu().numpy()).max() <= 1e-6
+
+
+def test_spectrum_entry_matches_oracle():
+    """compute_from_spectrum (aerial.rs:1805-1818): host FP64 spectra in
+    Fft2D::forward's layout, from an FFT of a complex raster and from the exact
+    spectrum of the single-clip lines-and-rectangles workload, against the
+    oracle's SOCS image of the same spectra; the exact spectrum reproduces
+    compute(mask) of the feature path."""
+    from highuvlith_amd import workloads as W
+    src, opt, grid, K = W.arf_lines_256_setup()
+    eng = H.AerialImageEngine(src, opt, grid, max_kernels=K)
+    n = grid.size
+    rng = np.random.default_rng(5)
+    raster = _manhattan(n, 8, rng) * np.exp(1j * 0.2) + 0.1j
+    specs = [oracle.fft2(raster.astype(np.complex128)),
+             oracle.mask_spectrum(n, grid.pixel_nm, W.arf_lines_256_features(3), dark_field=False)]
+    got = eng.compute_from_spectrum_batch(np.stack(specs), (0.0, 50.0)).cpu().numpy()
+    for c, spec in enumerate(specs):
+        for p, z in enumerate((0.0, 50.0)):
+            ref = oracle.socs_image(eng.kernel_set(z), spec, eng.flare_fraction(), threads=8)
+            err = float(np.abs(got[c, p] - ref).max())
+            print("spectrum entry", c, z, err)
+            assert err <= TOL
+    feat = eng.compute_features([W.arf_lines_256_features(3)], (0.0, 50.0)).cpu().numpy()
+    assert np.abs(feat[0] - got[1]).max() <= 1e-6
+    with pytest.raises(H.HuvError):
+        eng.compute_from_spectrum(np.zeros((128, 128), np.complex128))
