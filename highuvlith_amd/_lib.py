@@ -148,6 +148,8 @@ SIGNATURES = {
                                   C.POINTER(huv_socs_opts)]),
     "huv_socs_images_host": (C.c_int, [P, C.c_int, C.c_int, PF, C.c_int, C.POINTER(P), PF,
                                        C.POINTER(huv_socs_opts)]),
+    "huv_socs_images_noalloc": (C.c_int, [P, C.c_int, C.c_int, PF, C.c_int, C.POINTER(P), PF,
+                                          C.c_int]),
     "huv_socs_images_complex": (C.c_int, [P, C.c_int, C.c_int, PF, C.c_int, C.POINTER(P), PF,
                                           C.POINTER(huv_socs_opts)]),
     "huv_socs_images_spectrum": (C.c_int, [P, C.c_int, C.c_int, PD, C.c_int, C.POINTER(P),

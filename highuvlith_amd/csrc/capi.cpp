@@ -23,6 +23,35 @@ namespace {
 thread_local std::string g_err;
 }
 
+// Device scratch of the no-allocation entry point (huv_socs_images_noalloc):
+// one copy per GPU, held until the process ends.  Host code never uses this
+// name as an address: each device's copy is resolved once with
+// hipGetSymbolAddress (scratch_address below) and that pointer is what the
+// kernels receive.
+constexpr size_t kScratchBytes = (size_t)HUV_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, char*> g_scratch_ptr;  // device -> its copy of `scratch`
+
+// The device's copy of `scratch`; `device` must be current.
+hipError_t scratch_address(int device, char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+  g_scratch_ptr[device] = (char*)p;
+  *out = (char*)p;
+  return hipSuccess;
+}
+}  // namespace
+
 namespace huvi {
 
 huv_status fail(huv_status code, const std::string& msg) {
@@ -771,6 +800,9 @@ huv_status huv_ctx_create(int device, huv_ctx** out) {
                                      "; this backend is built for gfx950 (MI355X)");
   auto c = std::make_unique<huv_ctx>();
   c->device = device;
+  e = scratch_address(device, &c->fixed);
+  if (e != hipSuccess) return hip_fail(e, "scratch symbol address");
+  c->fixed_cap = kScratchBytes;
   // A blocking stream: it orders with work a caller puts on the null stream.
   e = hipStreamCreate(&c->own_stream);
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
@@ -1373,6 +1405,56 @@ huv_status huv_socs_images_spectrum(huv_ctx* c, int nclips, int n, const double*
   src.spectra = spectra;
   return socs_forward(c, nclips, n, src, nplanes, planes, out, false, nullptr,
                       it != c->tuned.end() ? &it->second : nullptr);
+}
+
+huv_status huv_socs_images_noalloc(huv_ctx* c, int nclips, int n, const float* masks,
+                                   int nplanes, huv_kset* const* planes, float* out,
+                                   int pixel_transfer) {
+  if (!c || !masks || !planes || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
+  if (nclips < 0 || nplanes < 1) return fail(HUV_INVALID_PARAMETER, "need nclips >= 0 and nplanes >= 1");
+  huv_status st = validate_planes(c, n, nplanes, planes);
+  if (st != HUV_OK) return st;
+  if (nclips == 0) return HUV_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  if (!c->fixed) return fail(HUV_INTERNAL_ERROR, "no scratch array on this device");
+  const bool pt = pixel_transfer != 0;
+  // Launch shape: the context's table entry for this shape class, else the
+  // defaults (no autotuning here: it times extra launches).
+  huv_socs_tuning t{0, 0, 0, 0};
+  auto tit = c->tuned.find(tuning_key(n, planes, nplanes));
+  if (tit != c->tuned.end()) t = tit->second;
+  const int B = planes[0]->B;
+  const int m = t.m ? t.m : planes[0]->m;
+  const int xlanes = t.xlanes ? t.xlanes : huvk::fields_lanes_default(m);
+  if (!c->tw.count(n) || !c->tw.count(m) || !c->sinc.count(sinc_key(n, B, pt)))
+    return fail(HUV_INVALID_PARAMETER,
+                "transform tables for this grid are not built yet: call huv_socs_images once "
+                "with the same kernel sets before the no-allocation entry point");
+  int Kmax = 0;
+  for (int p = 0; p < nplanes; ++p) Kmax = std::max(Kmax, planes[p]->K);
+  // As many kernel splits as huv_socs_images would use, fewer if the scratch
+  // array is too small for their partial images.
+  int S = huvk::fields_splits(m, nclips, Kmax);
+  size_t need = 0;
+  for (;; --S) {
+    const int ks = (Kmax + S - 1) / S, s_eff = (Kmax + ks - 1) / ks;
+    need = socs_sizes(nclips, n, B, m, xlanes == 2, nplanes, s_eff, true).total();
+    if (need <= c->fixed_cap || S == 1) break;
+  }
+  if (need > c->fixed_cap)
+    return fail(HUV_INVALID_PARAMETER,
+                "this batch needs " + std::to_string((need + (1 << 20) - 1) >> 20) +
+                    " MiB of scratch; the no-allocation entry point has " +
+                    std::to_string(c->fixed_cap >> 20) + " MiB per GPU: pass fewer clips per call");
+  const bool prof = c->profiling;
+  c->profiling = false;  // per-stage timing creates events
+  c->fixed_mode = true;
+  c->fixed_used = 0;
+  st = socs_forward(c, nclips, n, MaskSource::real(masks), nplanes, planes, out, pt, nullptr, &t, S);
+  c->fixed_mode = false;
+  c->profiling = prof;
+  return st;
 }
 
 huv_status huv_socs_images_features(huv_ctx* c, int nclips, int n, double pixel_nm,

@@ -211,6 +211,42 @@ huv_status huv_ctx_tuning_set(huv_ctx* ctx, int n, int band, int nplanes, int km
  * out + (c * nplanes + p) * n * n.  Asynchronous on the context stream. */
 huv_status huv_socs_images(huv_ctx* ctx, int nclips, int n, const float* masks, int nplanes,
                            huv_kset* const* planes, float* out, const huv_socs_opts* opts);
+/* huv_socs_images without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * intermediate buffers (row spectra, band boxes, coarse images, intensity
+ * bands) are carved from a fixed array of HUV_NOALLOC_SCRATCH_MIB MiB per GPU
+ * that the library holds until the process ends; a batch that needs more is
+ * refused with HUV_INVALID_PARAMETER (pass fewer clips per call).  Set-up:
+ * the transform tables and kernel stacks are not built here, so call
+ * huv_socs_images once with the same kernel sets (any batch size) first;
+ * otherwise HUV_INVALID_PARAMETER.  Uses the context's tuning-table entry for
+ * the shape class if present (no autotuning) and records no graph, memset or
+ * event.  All calls on one GPU share its scratch array: calls that overlap in
+ * time on one GPU (two contexts, two streams) race and must be ordered by the
+ * caller.  Asynchronous on the context stream. */
+#define HUV_NOALLOC_SCRATCH_MIB 64
+huv_status huv_socs_images_noalloc(huv_ctx* ctx, int nclips, int n, const float* masks,
+                                   int nplanes, huv_kset* const* planes, float* out,
+                                   int pixel_transfer);
+
+/* Complex transmittance rasters (compute_from_transmittance /
+ * compute_through_focus_from_transmittance with an Array2<Complex64> map,
+ * aerial.rs:1789-1799,1829-1842): masks is a DEVICE pointer to nclips x n x n
+ * interleaved complex FP32 (re, im) values; planes, out and opts as in
+ * huv_socs_images (pixel_transfer and graph apply). */
+huv_status huv_socs_images_complex(huv_ctx* ctx, int nclips, int n, const float* masks,
+                                   int nplanes, huv_kset* const* planes, float* out,
+                                   const huv_socs_opts* opts);
+/* Images from mask spectra (compute_from_spectrum, aerial.rs:1805-1818):
+ * spectra is a HOST pointer to nclips x n x n interleaved complex FP64 values
+ * in Fft2D::forward's layout and scale (row ky, column kx, FFT index order,
+ * unnormalised; fft2d.rs:11-13); only the |kx|, |ky| <= band bins are read
+ * (rounded to FP32).  planes and out as in huv_socs_images.  Synchronises the
+ * context stream before staging the band (pinned upload). */
+huv_status huv_socs_images_spectrum(huv_ctx* ctx, int nclips, int n, const double* spectra,
+                                    int nplanes, huv_kset* const* planes, float* out);
+
 /* One group of equally sized clips of an uneven batch. */
 typedef struct {
   int n;
