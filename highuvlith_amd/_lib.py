@@ -127,6 +127,9 @@ SIGNATURES = {
     "huv_kernels_build": (C.c_int, [C.POINTER(huv_optics), PD, C.c_int64, C.c_int, C.c_double,
                                     C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
                                     C.POINTER(P)]),
+    "huv_kernels_build_device": (C.c_int, [P, C.POINTER(huv_optics), PD, C.c_int64, C.c_int,
+                                           C.c_double, C.c_int, PD, C.c_double, C.c_int,
+                                           C.c_double, C.c_int, C.POINTER(P)]),
     "huv_kernels_info_get": (C.c_int, [P, C.POINTER(huv_kernels_info)]),
     "huv_kernels_copy": (C.c_int, [P, PI64, PI64, PD, PD]),
     "huv_kernels_free": (None, [P]),

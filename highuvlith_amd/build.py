@@ -26,7 +26,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 
 HOST_SRC = ["host/optics_source.cpp", "host/linalg.cpp", "host/kernels_build.cpp"]
 HIP_SRC = ["hip/socs_kernels.hip", "hip/ilt_kernels.hip", "hip/unet_kernels.hip",
-           "hip/mask_kernels.hip", "capi.cpp",
+           "hip/mask_kernels.hip", "hip/kbuild_kernels.hip", "capi.cpp", "capi_kbuild.cpp",
            "capi_ilt.cpp", "capi_unet.cpp"]
 
 

@@ -619,11 +619,24 @@ using namespace huvi;
 // ===========================================================================
 // Host engine
 // ===========================================================================
-struct huv_host_kernels {
-  huv::KernelSet ks;
-};
+huv_status huvi::check_build_args(const huv_optics* optics, int n, double pixel_nm,
+                                  double wavelength_nm, int max_kernels,
+                                  double energy_fraction) {
+  if (max_kernels < 1) return fail(HUV_INVALID_PARAMETER, "max_kernels: must be at least 1");
+  if (!(energy_fraction > 0.0 && energy_fraction <= 1.0))
+    return fail(HUV_INVALID_PARAMETER, "kernel_energy_fraction: must be in (0, 1]");
+  if (n < 2 || !(std::isfinite(pixel_nm) && pixel_nm > 0.0))
+    return fail(HUV_INVALID_PARAMETER, "grid: needs size >= 2 and a positive finite pixel_nm");
+  if (!(std::isfinite(wavelength_nm) && wavelength_nm > 0.0))
+    return fail(HUV_INVALID_PARAMETER, "wavelength_nm: must be positive and finite");
+  const double medium = optics->kind == 0 ? optics->immersion_index : 1.0;
+  if (!(std::isfinite(medium) && medium >= 1.0 && optics->na > 0.0 && optics->na < medium))
+    return fail(HUV_INVALID_PARAMETER,
+                "numerical_aperture: must be positive and below the image-space index");
+  return HUV_OK;
+}
 
-static huv::Optics to_optics(const huv_optics* o) {
+huv::Optics huvi::to_optics(const huv_optics* o) {
   huv::Optics r;
   r.kind = o->kind;
   r.na = o->na;
@@ -695,17 +708,9 @@ huv_status huv_kernels_build(const huv_optics* optics, const double* xyw, int64_
                              int max_kernels, double energy_fraction, int method,
                              huv_host_kernels** out) {
   if (!optics || !xyw || !out) return fail(HUV_INVALID_PARAMETER, "null argument");
-  if (max_kernels < 1) return fail(HUV_INVALID_PARAMETER, "max_kernels: must be at least 1");
-  if (!(energy_fraction > 0.0 && energy_fraction <= 1.0))
-    return fail(HUV_INVALID_PARAMETER, "kernel_energy_fraction: must be in (0, 1]");
-  if (n < 2 || !(std::isfinite(pixel_nm) && pixel_nm > 0.0))
-    return fail(HUV_INVALID_PARAMETER, "grid: needs size >= 2 and a positive finite pixel_nm");
-  if (!(std::isfinite(wavelength_nm) && wavelength_nm > 0.0))
-    return fail(HUV_INVALID_PARAMETER, "wavelength_nm: must be positive and finite");
-  const double medium = optics->kind == 0 ? optics->immersion_index : 1.0;
-  if (!(std::isfinite(medium) && medium >= 1.0 && optics->na > 0.0 && optics->na < medium))
-    return fail(HUV_INVALID_PARAMETER,
-                "numerical_aperture: must be positive and below the image-space index");
+  const huv_status chk =
+      check_build_args(optics, n, pixel_nm, wavelength_nm, max_kernels, energy_fraction);
+  if (chk != HUV_OK) return chk;
   try {
     std::vector<huv::SourcePoint> pts((size_t)npoints);
     for (int64_t i = 0; i < npoints; ++i) pts[i] = {xyw[3 * i], xyw[3 * i + 1], xyw[3 * i + 2]};

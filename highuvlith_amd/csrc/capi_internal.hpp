@@ -10,8 +10,20 @@
 #include <vector>
 
 #include "../../include/huv.h"
+#include "host/engine.hpp"
+
+// A kernel set built on the host (huv_kernels_build) or with the device's
+// help (huv_kernels_build_device).
+struct huv_host_kernels {
+  huv::KernelSet ks;
+};
 
 namespace huvi {
+
+huv::Optics to_optics(const huv_optics* o);
+// The argument checks huv_kernels_build and huv_kernels_build_device share.
+huv_status check_build_args(const huv_optics* optics, int n, double pixel_nm,
+                            double wavelength_nm, int max_kernels, double energy_fraction);
 
 huv_status fail(huv_status code, const std::string& msg);
 huv_status hip_fail(hipError_t e, const char* what);

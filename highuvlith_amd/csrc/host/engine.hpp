@@ -151,6 +151,26 @@ int build_kernel_set(const Optics& optics, const std::vector<SourcePoint>& point
                      int n, double pixel_nm, double defocus_nm, double wavelength_nm,
                      const BuildSettings& settings, KernelSet& out, std::string& err);
 
+// The stages of build_kernel_set that the device build shares with it.
+// Candidate mask frequencies |f| <= (1 + sigma_max) NA/lambda in row-major
+// grid order (candidate 0 is DC), in units of the cutoff NA/lambda
+// (aerial.rs:1053-1075).
+struct Candidates {
+  std::vector<int64_t> idx;     // flat grid index i * n + j
+  std::vector<double> fx, fy;   // frequency / cutoff
+  bool exceeds_nyquist = false;
+};
+int kernel_candidates(const Optics& optics, const std::vector<SourcePoint>& points, int n,
+                      double pixel_nm, double wavelength_nm, Candidates& out, std::string& err);
+// Decomposition path for an n_cand x n_cols factor (aerial.rs:1118-1128).
+int choose_build_method(int64_t n_cand, int64_t n_cols, const BuildSettings& settings);
+// Fill `out` from the chosen kernels (values K x support, kernel-major).
+int assemble_kernel_set(int n, double pixel_nm, double defocus_nm, double wavelength_nm,
+                        std::vector<int64_t> support, std::vector<double> eig,
+                        std::vector<cd> values, double trace, double clear_field,
+                        double pupil_peak, int method, int64_t n_eff, bool exceeds_nyquist,
+                        KernelSet& out, std::string& err);
+
 // ---------------------------------------------------------------------------
 // Mask rasterisation: axis-aligned rectangles painted in order over a
 // background, exact area coverage per pixel (mask.rs:590, 1365-1571).

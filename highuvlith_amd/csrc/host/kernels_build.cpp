@@ -66,9 +66,8 @@ int select_kernel_count(const double* values, int64_t nvals, double trace,
   return (int)k;
 }
 
-int build_kernel_set(const Optics& optics, const std::vector<SourcePoint>& points, int n,
-                     double pixel_nm, double defocus_nm, double wavelength_nm,
-                     const BuildSettings& settings, KernelSet& out, std::string& err) {
+int kernel_candidates(const Optics& optics, const std::vector<SourcePoint>& points, int n,
+                      double pixel_nm, double wavelength_nm, Candidates& out, std::string& err) {
   const double df = 1.0 / (n * pixel_nm);
   const double cutoff = optics.na / wavelength_nm;
   if (!(std::isfinite(cutoff) && cutoff > 0.0)) {
@@ -79,21 +78,21 @@ int build_kernel_set(const Optics& optics, const std::vector<SourcePoint>& point
   for (const auto& p : points) sigma_max = std::max(sigma_max, std::hypot(p.sx, p.sy));
   const double radius = cutoff * (1.0 + sigma_max);
   const double r2 = std::pow(radius / df, 2) * (1.0 + 1e-12);
-  const bool exceeds_nyquist = radius > (double)(n / 2) * df;
-
-  struct Cand { int64_t idx; double fx, fy; };
-  std::vector<Cand> cand;
+  out = Candidates();
+  out.exceeds_nyquist = radius > (double)(n / 2) * df;
   for (int64_t i = 0; i < n; ++i) {
     const double ky = signed_index(i, n);
     if (ky * ky > r2) continue;
     for (int64_t j = 0; j < n; ++j) {
       const double kx = signed_index(j, n);
-      if (kx * kx + ky * ky <= r2) cand.push_back({i * n + j, kx * df / cutoff, ky * df / cutoff});
+      if (kx * kx + ky * ky <= r2) {
+        out.idx.push_back(i * n + j);
+        out.fx.push_back(kx * df / cutoff);
+        out.fy.push_back(ky * df / cutoff);
+      }
     }
   }
-  const int64_t n_cand = (int64_t)cand.size();
-  const int64_t n_pts = (int64_t)points.size();
-  const int64_t n_cols = n_pts;
+  const int64_t n_cand = (int64_t)out.idx.size(), n_cols = (int64_t)points.size();
   if (n_cand == 0 || n_cols == 0) {
     err = "no diffraction orders pass the pupil";
     return NO_DIFFRACTION_ORDERS;
@@ -103,18 +102,38 @@ int build_kernel_set(const Optics& optics, const std::vector<SourcePoint>& point
           std::to_string(MAX_FACTOR_BYTES / (sizeof(cd) * n_cols));
     return PUPIL_SAMPLING_TOO_DENSE;
   }
+  return OK;
+}
+
+int choose_build_method(int64_t n_cand, int64_t n_cols, const BuildSettings& settings) {
+  const int64_t min_dim = std::min(n_cand, n_cols);
+  const int64_t subspace = settings.max_kernels + RANDOMIZED_OVERSAMPLE;
+  const bool dense = min_dim <= DENSE_FAST_DIM || (min_dim <= DENSE_MAX_DIM && subspace * 3 >= min_dim);
+  if (settings.method != METHOD_AUTO) return settings.method;
+  return !dense ? METHOD_RANDOMIZED : (n_cand <= n_cols ? METHOD_DENSE_TCC : METHOD_DENSE_GRAM);
+}
+
+int build_kernel_set(const Optics& optics, const std::vector<SourcePoint>& points, int n,
+                     double pixel_nm, double defocus_nm, double wavelength_nm,
+                     const BuildSettings& settings, KernelSet& out, std::string& err) {
+  Candidates cs;
+  const int cst = kernel_candidates(optics, points, n, pixel_nm, wavelength_nm, cs, err);
+  if (cst != OK) return cst;
+  struct Cand { int64_t idx; double fx, fy; };
+  std::vector<Cand> cand(cs.idx.size());
+  for (size_t i = 0; i < cand.size(); ++i) cand[i] = {cs.idx[i], cs.fx[i], cs.fy[i]};
+  const bool exceeds_nyquist = cs.exceeds_nyquist;
+  const int64_t n_cand = (int64_t)cand.size();
+  const int64_t n_pts = (int64_t)points.size();
+  const int64_t n_cols = n_pts;
   auto eval = [&](const Cand& c, const SourcePoint& p) {
     return pupil_function(optics, c.fx + p.sx, c.fy + p.sy, defocus_nm, wavelength_nm) *
            std::sqrt(p.weight);
   };
   const int64_t max_k = settings.max_kernels;
   const double fraction = settings.kernel_energy_fraction;
-  const int64_t min_dim = std::min(n_cand, n_cols);
   const int64_t subspace = max_k + RANDOMIZED_OVERSAMPLE;
-  const bool dense = min_dim <= DENSE_FAST_DIM || (min_dim <= DENSE_MAX_DIM && subspace * 3 >= min_dim);
-  int method = settings.method;
-  if (method == METHOD_AUTO)
-    method = !dense ? METHOD_RANDOMIZED : (n_cand <= n_cols ? METHOD_DENSE_TCC : METHOD_DENSE_GRAM);
+  const int method = choose_build_method(n_cand, n_cols, settings);
 
   std::vector<int64_t> support;
   std::vector<double> eig;
@@ -229,6 +248,16 @@ int build_kernel_set(const Optics& optics, const std::vector<SourcePoint>& point
       values.assign(u.data.begin(), u.data.begin() + (size_t)k * n_cand);
     }
   }
+  return assemble_kernel_set(n, pixel_nm, defocus_nm, wavelength_nm, std::move(support),
+                             std::move(eig), std::move(values), trace, clear_field, pupil_peak,
+                             method, n_eff, exceeds_nyquist, out, err);
+}
+
+int assemble_kernel_set(int n, double pixel_nm, double defocus_nm, double wavelength_nm,
+                        std::vector<int64_t> support, std::vector<double> eig,
+                        std::vector<cd> values, double trace, double clear_field,
+                        double pupil_peak, int method, int64_t n_eff, bool exceeds_nyquist,
+                        KernelSet& out, std::string& err) {
   if (eig.empty() || std::isnan(trace) || trace <= 0.0) {
     err = "no diffraction orders pass the pupil";
     return NO_DIFFRACTION_ORDERS;

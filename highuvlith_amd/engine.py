@@ -273,6 +273,29 @@ def build_kernel_set(optics, points, grid, defocus_nm, wavelength_nm, max_kernel
     check(lib.huv_kernels_build(C.byref(o), dptr(pts), len(pts), grid.size, grid.pixel_nm,
                                 defocus_nm, wavelength_nm, max_kernels, kernel_energy_fraction,
                                 m, C.byref(h)))
+    return _take_kernel_set(lib, h, grid, defocus_nm, wavelength_nm)
+
+
+def build_kernel_sets_device(ctx, optics, points, grid, defocus_nm, wavelength_nm, max_kernels,
+                             kernel_energy_fraction=1.0, method=None):
+    """Kernel sets of several focus planes with the TCC factor, its Gram product
+    and the kernels computed on the GPU in FP64 and the eigendecompositions on
+    host threads (dense TCC / dense Gram paths; huv_kernels_build_device)."""
+    lib = _lib.load()
+    o = optics.to_c()
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    zs = np.ascontiguousarray(np.atleast_1d(defocus_nm), dtype=np.float64)
+    m = -1 if method is None else METHODS[method]
+    hs = (C.c_void_p * len(zs))()
+    check(lib.huv_kernels_build_device(ctx.handle, C.byref(o), dptr(pts), len(pts), grid.size,
+                                       grid.pixel_nm, len(zs), dptr(zs), wavelength_nm,
+                                       max_kernels, kernel_energy_fraction, m, hs))
+    return [_take_kernel_set(lib, C.c_void_p(hs[i]), grid, float(zs[i]), wavelength_nm)
+            for i in range(len(zs))]
+
+
+def _take_kernel_set(lib, h, grid, defocus_nm, wavelength_nm):
+    """SparseKernelSet copied out of a huv_host_kernels handle, which is freed."""
     try:
         info = _lib.huv_kernels_info()
         check(lib.huv_kernels_info_get(h, C.byref(info)))
@@ -754,6 +777,29 @@ class AerialImageEngine:
                 while len(self._cache) > self.settings.kernel_cache_capacity:
                     self._cache.popitem(last=False)
         return ks
+
+    def build_planes_device(self, defocus_nm):
+        """Build the kernel sets of several focus planes at once with the GPU's help
+        (huv_kernels_build_device) and keep them in the cache; planes already
+        cached are kept.  Returns the sets in order."""
+        zs = [self._key(z) for z in defocus_nm]
+        with self._lock:
+            todo = sorted({z for z in zs if z != 0.0 and z not in self._cache})
+        if todo:
+            s = self.settings
+            sets = build_kernel_sets_device(self.context(), self.optics, self._points, self.grid,
+                                            todo, self.source.wavelength_nm, s.max_kernels,
+                                            s.kernel_energy_fraction, s.method)
+            with self._lock:
+                for z, ks in zip(todo, sets):
+                    if self.settings.kernel_cache_capacity > 0:
+                        self._cache[z] = ks
+                while len(self._cache) > self.settings.kernel_cache_capacity:
+                    self._cache.popitem(last=False)
+            built = dict(zip(todo, sets))
+        else:
+            built = {}
+        return [built[z] if z in built else self.kernel_set(z) for z in zs]
 
     def kernels(self, defocus_nm=0.0):
         """Dense kernel set (eigenvalues / clear field) for ILT (aerial.rs:1912-1942)."""

@@ -176,10 +176,13 @@ class IltProblem:
         return out["theta"], out["cost_history"], out["iterations"], out["termination"]
 
     def run_stage_traced(self, targets, theta, max_iterations, stop_cost=None,
-                         snapshot_every=0, iteration_offset=None, nslots=0, snapshots=None):
+                         snapshot_every=0, iteration_offset=None, nslots=0, snapshots=None,
+                         convergence_tol=None):
         """run_stage with its trace: per-target cost and gradient-norm histories
         and, with snapshot_every > 0, the masks at every global iteration that
-        is a multiple of it written to snapshots (torch (T, nslots, n, n))."""
+        is a multiple of it written to snapshots (torch (T, nslots, n, n)).
+        iteration_offset: iterations each target ran before (max_iterations is
+        then the whole budget); convergence_tol overrides the config's."""
         import torch
         tg, th = self._dev(targets), self._dev(theta)
         T = tg.shape[0]
@@ -198,9 +201,10 @@ class IltProblem:
         if snapshot_every > 0:
             assert snapshots is not None and snapshots.shape == (T, nslots, self.n, self.n)
         self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        tol = cfg.convergence_tol if convergence_tol is None else convergence_tol
         check(self.ctx.lib.huv_ilt_run_stage(
             self.handle, T, C.c_void_p(tg.data_ptr()), C.c_void_p(th.data_ptr()),
-            max_iterations, cfg.learning_rate, cfg.convergence_tol, int(cfg.optimizer == "cg"),
+            max_iterations, cfg.learning_rate, tol, int(cfg.optimizer == "cg"),
             dptr(stop) if stop is not None else None, dptr(hist), dptr(gn), int(snapshot_every),
             off, int(nslots), C.c_void_p(snapshots.data_ptr()) if snapshot_every > 0 else None,
             iters, term))

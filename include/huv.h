@@ -168,6 +168,22 @@ void huv_kset_destroy(huv_kset* ks);
 /* Band half-width B and coarse grid m chosen for this set. */
 huv_status huv_kset_shape(const huv_kset* ks, int* band, int* coarse);
 
+/* Kernel sets of nplanes focus planes (defocus_nm[p]) built with the device's
+ * help (replaces KernelBuilder::build, aerial.rs:1041-1317, for the dense TCC
+ * and dense Gram paths): the TCC factor A[f, s] = sqrt(w_s) P(f/f_c + s; z,
+ * lambda), its statistics (clear field, pupil peak, trace), its Gram product
+ * (linalg.rs:181-232) and the kernels U = A V / sqrt(lambda) run on the GPU in
+ * FP64; each plane's Hermitian eigendecomposition (linalg.rs:324-551) runs on
+ * its own host thread.  Arguments as huv_kernels_build; out receives nplanes
+ * sets (free each with huv_kernels_free).  A method that resolves to the
+ * randomized path returns HUV_UNSUPPORTED (build it with huv_kernels_build).
+ * Synchronises the context stream. */
+huv_status huv_kernels_build_device(huv_ctx* ctx, const huv_optics* optics, const double* xyw,
+                                    int64_t npoints, int n, double pixel_nm, int nplanes,
+                                    const double* defocus_nm, double wavelength_nm,
+                                    int max_kernels, double energy_fraction, int method,
+                                    huv_host_kernels** out);
+
 typedef struct {
   /* Multiply the mask spectrum by sinc(pi kx/n) sinc(pi ky/n) (mask.rs:804-815):
    * with a pixel-aligned raster this reproduces AerialImageEngine::compute(mask). */

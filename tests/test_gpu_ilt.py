@@ -26,7 +26,9 @@ def oracle_problem(eng, cfg, target):
                              threshold=cfg.threshold, steepness=cfg.steepness,
                              beta=cfg.sigmoid_steepness, tv_weight=cfg.regularization,
                              tv_eps=cfg.tv_epsilon, bin_weight=cfg.binarization_weight,
-                             absorber=cfg.absorber, pixel_nm=eng.grid.pixel_nm)
+                             absorber=cfg.absorber, pixel_nm=eng.grid.pixel_nm,
+                             min_feature_nm=cfg.min_feature_nm,
+                             proxy=cfg.gradient == "local_proxy")
 
 
 def perturbed_theta(target, seed):
@@ -122,10 +124,14 @@ def test_invalid_ilt_configs_rejected():
     eng = quasar_engine(256)
     with pytest.raises(H.HuvError):
         L.IltProblem(eng, L.ILTConfig(conditions=[]))
+    with pytest.raises(H.HuvError):   # filter band wider than 127 bins
+        L.IltProblem(eng, L.ILTConfig(min_feature_nm=1.0))
     with pytest.raises(H.HuvError):
-        L.IltProblem(eng, L.ILTConfig(min_feature_nm=40.0))
+        L.IltProblem(eng, L.ILTConfig(absorber=complex(float("nan"), 0.0)))
     with pytest.raises(H.HuvError):
-        L.IltProblem(eng, L.ILTConfig(absorber=0.2j))
+        L.IltProblem(eng, L.ILTConfig(gradient="finite_differences"))
+    with pytest.raises(H.HuvError):
+        L.IltProblem(eng, L.ILTConfig(learning_rate=0.0))
     prob = L.IltProblem(eng, L.ILTConfig())
     with pytest.raises(H.HuvError):
         prob.cost(np.zeros((1, 128, 128)), np.zeros((1, 128, 128)))
@@ -162,3 +168,129 @@ def test_benchmark_ilt_run_matches_oracle_run_stage():
         assert np.all(np.abs(g[:k] - hist[:k]) / hist[:k] < 1e-3)
         assert abs(at_gpu - g[-1]) / g[-1] < 1e-4
         assert abs(g[-1] - hist[-1]) / hist[-1] < 0.1
+
+
+def _annular_engine(n, px, K, kind="arf"):
+    if kind == "arf":
+        src = H.Source.arf(H.IlluminationShape.Annular(0.7, 0.9))
+        opt = H.ProjectionOptics.immersion_193i()
+    else:
+        src = H.Source(13.5, H.IlluminationShape.SectorPoles(0.7, 0.9, 35.0, 0.0, 2))
+        opt = H.EuvProjectionOptics.nxe_033()
+    return H.AerialImageEngine(src, opt, H.GridConfig(n, px), max_kernels=K)
+
+
+ILT_CASES = {
+    # coarse grids above 64: the adjoint's team-FFT passes through HBM
+    "coarse128": (lambda: _annular_engine(1024, 2.0, 8),
+                  dict(cost="resist", regularization=0.0)),
+    "coarse192": (lambda: _annular_engine(1024, 1.0, 6, "euv"),
+                  dict(cost="image", regularization=0.02)),
+    "coarse256": (lambda: _annular_engine(2048, 2.0, 4),
+                  dict(cost="resist", regularization=0.0)),
+    # attenuated phase-shift absorber with a complex amplitude (ilt.rs:544-552, 672)
+    "complex_absorber": (lambda: quasar_engine(256),
+                         dict(cost="image", regularization=0.05, binarization_weight=0.02,
+                              absorber=0.245 * np.exp(1j * 2.2))),
+    # Gaussian density filter (ilt.rs:443-454, 524-535, 733-737)
+    "density_filter": (lambda: quasar_engine(512),
+                       dict(cost="resist", regularization=0.0, min_feature_nm=40.0)),
+    # local-proxy gradient (ilt.rs:719-721)
+    "local_proxy": (lambda: quasar_engine(256),
+                    dict(cost="resist", regularization=0.01, gradient="local_proxy")),
+}
+
+
+@pytest.mark.parametrize("case", list(ILT_CASES))
+def test_ilt_extensions_match_oracle(case):
+    """Cost and dC/dtheta of the device ILT against the FP64 oracle (cost 1e-5,
+    gradient 1e-3 of its largest entry) on coarse grids 128 / 192 / 256, with a
+    complex absorber, with the density filter and with the local proxy."""
+    make, kw = ILT_CASES[case]
+    eng = make()
+    cfg = L.ILTConfig(**kw)
+    ds = eng.device_kernel_set(0.0)
+    targets = W.contact_array_targets(2, seed=41, grid=eng.grid)
+    prob = L.IltProblem(eng, cfg)
+    thetas = np.stack([perturbed_theta(t, s) for s, t in enumerate(targets)])
+    cost, grad = prob.cost_and_gradient(targets, thetas)
+    grad = grad.cpu().numpy()
+    for t in range(len(targets)):
+        orc = oracle_problem(eng, cfg, targets[t])
+        c_ref, g_ref = orc.cost_and_gradient(thetas[t].astype(np.float32).astype(np.float64))
+        rel_c = abs(cost[t] - c_ref) / abs(c_ref)
+        rel_g = np.abs(grad[t] - g_ref).max() / np.abs(g_ref).max()
+        print(case, "band", ds.band, "coarse", ds.coarse, t, "cost", cost[t], c_ref, rel_c,
+              "grad rel", rel_g)
+        assert rel_c <= 1e-5
+        assert rel_g <= 1e-3
+    if case == "density_filter":
+        m = prob.mask(thetas).cpu().numpy()
+        for t in range(len(targets)):
+            ref = oracle_problem(eng, cfg, targets[t]).mask(
+                thetas[t].astype(np.float32).astype(np.float64))
+            assert np.abs(m[t] - ref).max() <= 1e-5
+
+
+def test_local_proxy_stage_tracks_oracle():
+    """Steepest descent on the local proxy with plain-decrease acceptance
+    (ilt.rs:767-801) against the oracle's run_stage on the same objective."""
+    eng = quasar_engine(256)
+    cfg = L.ILTConfig(cost="resist", regularization=0.01, gradient="local_proxy",
+                      max_iterations=8)
+    targets = W.contact_array_targets(2, seed=51, grid=eng.grid)
+    prob = L.IltProblem(eng, cfg)
+    _, hist, its, term = prob.run_stage(targets, prob.initial_theta(targets), 8)
+    for t in range(2):
+        orc = oracle_problem(eng, cfg, targets[t])
+        _, ref, _ = orc.run_stage(targets[t] - 0.5, 8, use_cg=False)
+        k = min(len(hist[t]), len(ref), 5)
+        print("proxy", t, hist[t][:k], ref[:k], term[t])
+        assert np.all(np.abs(hist[t][:k] - ref[:k]) / ref[:k] < 1e-3)
+        assert np.all(np.diff(hist[t]) < 0)
+
+
+def test_optimize_ilt_stages_and_result_fields():
+    """optimize_ilt with binarisation continuation (ilt.rs:881-951): a first
+    stage of 4 iterations without the penalty, then the rest of the 10-iteration
+    budget with it; stage starts, per-stage monotone costs, gradient norms,
+    snapshots every 3 iterations, the images of the continuous and the binary
+    mask and their pattern errors, against the oracle run the same way."""
+    eng = quasar_engine(256)
+    cfg = L.ILTConfig(cost="resist", regularization=0.0, binarization_weight=0.05,
+                      binarization_after=4, max_iterations=10, snapshot_every=3)
+    targets = W.contact_array_targets(2, seed=61, grid=eng.grid)
+    res = L.optimize_ilt(eng, targets, cfg)
+    for t in range(2):
+        h, starts = res.cost_history[t], res.stage_starts[t]
+        assert starts[0] == 0 and len(starts) == 2
+        k1 = starts[1]
+        assert np.all(np.diff(h[:k1]) <= 0) and np.all(np.diff(h[k1:]) <= 0)
+        assert res.iterations[t] == len(h) - len(starts) <= 10
+        assert len(res.gradient_norm_history[t]) == len(h)
+        # the oracle: same two stages
+        orc = oracle_problem(eng, L.ILTConfig(cost="resist", regularization=0.0,
+                                              binarization_weight=0.0), targets[t])
+        th1, r1, _ = orc.run_stage(targets[t] - 0.5, 4)
+        orc.set_bin_weight(0.05)
+        th2, r2, _ = orc.run_stage(th1, 10 - (len(r1) - 1))
+        print("stages", t, h[:k1], r1, h[k1:], r2)
+        assert np.all(np.abs(h[:k1] - r1[:k1]) / r1[:k1] < 1e-3)
+        assert abs(h[k1] - r2[0]) / r2[0] < 1e-3
+        assert abs(h[-1] - r2[-1]) / r2[-1] < 2e-2
+        # snapshots at 0, 3, 6, 9 and the final iteration
+        its = [s[0] for s in res.snapshots[t]]
+        done = int(res.iterations[t])
+        assert its == sorted(set([0] + list(range(3, done + 1, 3)) + [done]))
+        assert np.array_equal(res.snapshots[t][-1][1].cpu().numpy(),
+                              res.mask_transmittance[t].cpu().numpy())
+        # result images and pattern errors (ilt.rs:918-950)
+        m = res.mask_transmittance[t].cpu().numpy().astype(np.float64)
+        img_ref = orc.image(m, 0)
+        assert np.abs(res.aerial_image[t].cpu().numpy() - img_ref).max() <= 1e-5
+        binary = (m > 0.5).astype(np.float64)
+        assert np.array_equal(res.binary_mask[t].cpu().numpy(), binary.astype(np.float32))
+        bimg_ref = orc.image(binary, 0)
+        assert np.abs(res.binary_aerial_image[t].cpu().numpy() - bimg_ref).max() <= 1e-5
+        pe = int(np.count_nonzero((res.aerial_image[t].cpu().numpy() > 0.25) != (targets[t] > 0.5)))
+        assert res.pattern_error[t] == pe

@@ -28,8 +28,12 @@ def test_warm_start_reaches_the_plain_cost_in_fewer_iterations():
     print("labels", round(lab_s, 3), "s; train loss", [round(h, 4) for h in hist[::5]], hist[-1])
     print("plain cost", res.plain_cost, "iters", res.plain_iterations, "wall", res.plain_wall_s)
     print("warm  cost", res.warm_cost, "iters", res.warm_iterations, "wall", res.warm_wall_s)
-    print("reached", res.reached, "fallback", res.fallback)
+    print("reached", res.reached, "fallback", res.fallback, "by refinement",
+          res.reached_by_refinement, "continued", res.continued, "plain fallback",
+          res.plain_fallback)
     assert np.all(np.isfinite(res.warm_cost))
     assert hist[-1] < 0.5 * hist[0]
-    assert res.reached.mean() >= 0.75
-    assert res.warm_iterations.mean() < 0.8 * res.plain_iterations.mean()
+    # every target ends at its plain cost (fallbacks included, their iterations counted)
+    assert res.reached.mean() >= 0.95
+    assert res.reached_by_refinement.mean() >= 0.5
+    assert res.warm_iterations.mean() < res.plain_iterations.mean()
