@@ -8,8 +8,11 @@ sectors sigma 0.7-0.9 with a 35 deg opening, 5 focus planes
 (-60/-30/0/+30/+60 nm), K = 32 kernels per plane built on the host in FP64
 (2560 images per step).
 
-Also reported in the same JSON line (each skipped with --no-rects,
---no-uneven, --no-ilt, --no-warm):
+Also reported in the same JSON line (each skipped with --no-single,
+--no-rects, --no-uneven, --no-ilt, --no-warm):
+  single   one 256^2 @ 4 nm ArF clip (65/130 nm lines truncated at the field
+           edge + 8 rectangles), K = 24: latency of the exact-spectrum and
+           raster paths, and (with the CPU leg) the oracle on the same clip.
   rects    the headline optics and focus planes on masks given as rectangle
            lists (512 clips of 8 x 8 contacts with sub-pixel size, pitch and
            offset, 64 disjoint cells each): exact band spectrum in FP64 on the
@@ -494,6 +497,56 @@ def bench_ilt(args, H, W, np, torch):
             }
 
 
+def bench_single(args, H, W, np, torch):
+    """One 256^2 @ 4 nm ArF clip (65 nm lines on a 130 nm pitch truncated at the
+    field edge, plus 8 rectangles), K = 24, one focus plane: GPU latency of
+    the exact-spectrum (feature list) and raster paths, median of 50 calls;
+    with the CPU leg, the oracle (exact spectrum + SOCS image, FP64) on the
+    same clip, on 16 threads and on one, and the GPU's error against it."""
+    src, opt, grid, K = W.arf_lines_256_setup()
+    eng = H.AerialImageEngine(src, opt, grid, max_kernels=K)
+    feats = W.arf_lines_256_features(0)
+    raster = torch.as_tensor(W.arf_lines_256_raster(0)[None].astype(np.float32), device="cuda")
+
+    def median_ms(fn):
+        fn()
+        ts = []
+        for _ in range(50):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)) * 1e3
+
+    img = {}
+    ms_feat = median_ms(lambda: img.__setitem__("f", eng.compute_features([feats], (0.0,))))
+    ms_rast = median_ms(lambda: img.__setitem__("r", eng.compute_batch(raster, (0.0,))))
+    out = {"workload": "one clip 256x256 @ 4 nm, ArF 193i annular 0.7/0.9, K=24, 65/130 nm "
+                       "lines + 8 rectangles, one focus plane",
+           "features_ms": round(ms_feat, 4), "raster_ms": round(ms_rast, 4),
+           "features_images_per_s": round(1e3 / ms_feat, 1),
+           "raster_images_per_s": round(1e3 / ms_rast, 1),
+           "latency": "host call to device completion, median of 50"}
+    if not args.no_cpu:
+        import oracle
+        ks = eng.kernel_set(0.0)
+        n = grid.size
+
+        def cpu(threads):
+            t = time.perf_counter()
+            spec = oracle.mask_spectrum(n, grid.pixel_nm, feats, dark_field=False)
+            ref = oracle.socs_image(ks, spec, eng.flare_fraction(), threads=threads)
+            return ref, (time.perf_counter() - t) * 1e3
+        cpu(16)
+        ref, ms16 = cpu(16)
+        _, ms1 = cpu(1)
+        out["cpu_oracle_ms_16_threads"] = round(ms16, 3)
+        out["cpu_oracle_ms_1_thread"] = round(ms1, 3)
+        out["max_abs_err_vs_oracle"] = float(np.abs(img["f"][0, 0].cpu().numpy() - ref).max())
+    return out
+
+
 def bench_warm(args, H, W, np, torch):
     from highuvlith_amd import ilt as L
     from highuvlith_amd import unet as UN
@@ -533,6 +586,7 @@ def bench_warm(args, H, W, np, torch):
             "unit": "clips/s to the plain 200-iteration cost",
             "warm_over_plain_wall": round(res.warm_wall_s / res.plain_wall_s, 4),
             "warm_wall_includes": "inference, refinement, continuation and plain-ILT fallback",
+            "warm_stages_s": res.stages_s,
             "plain_mean_iterations": float(np.mean(res.plain_iterations)),
             "warm_mean_iterations": float(np.mean(res.warm_iterations)),
             "reached_fraction": float(np.mean(res.reached)),
@@ -557,6 +611,7 @@ def main():
     ap.add_argument("--cpu-clips", type=int, default=2,
                     help="clips in the CPU-baseline sample (x 5 planes)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-rects", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-ilt", action="store_true")
@@ -587,7 +642,8 @@ def main():
 
     result = bench_fixed(args, H, W, np, torch)
     print("fixed batch done", file=sys.stderr, flush=True)
-    for name, fn, skip in (("rects", bench_rects, args.no_rects),
+    for name, fn, skip in (("single", bench_single, args.no_single),
+                           ("rects", bench_rects, args.no_rects),
                            ("uneven", bench_uneven, args.no_uneven),
                            ("ilt", bench_ilt, args.no_ilt),
                            ("warm", bench_warm, args.no_warm)):

@@ -1067,7 +1067,7 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
 // Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
 huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                         huv_kset* const* planes, float* out, bool pt, float2** Mq_out,
-                        const huv_socs_tuning* tune, int max_splits) {
+                        const huv_socs_tuning* tune, int max_splits, int split_batch) {
   const int B = planes[0]->B;
   const int m = tune && tune->m ? tune->m : planes[0]->m;
   const int BB = huvk::coarse_band(m);
@@ -1110,7 +1110,7 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int np
   int Kmax;
   st = plane_pack(c, nplanes, planes, m, xlanes == 2, &kern, &lam, &Kmax);
   if (st != HUV_OK) return st;
-  int S = huvk::fields_splits(m, nclips, Kmax);
+  int S = huvk::fields_splits(m, split_batch > 0 ? split_batch : nclips, Kmax);
   if (max_splits > 0 && S > max_splits) {
     // every split must own at least one kernel (its partials are summed)
     const int ks = (Kmax + max_splits - 1) / max_splits;

@@ -173,7 +173,10 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const float* theta,
   float2* Mq = nullptr;
   MaskSource src = MaskSource::real(b.t);
   if (p->complex_abs) src.kind = MaskSource::COMPLEX;
-  st = socs_forward(c, T, n, src, C, p->conds.data(), b.image, false, &Mq);
+  // The kernel sum is split as for a single target at every batch size, so a
+  // target's images, and with them its whole optimisation trajectory, do not
+  // depend on which other targets are still live in the batch.
+  st = socs_forward(c, T, n, src, C, p->conds.data(), b.image, false, &Mq, nullptr, 0, 1);
   if (st != HUV_OK) return st;
   c->stage_begin(ST_ILT_POINTWISE, &ev);
   e = huvk::launch_ilt_pointwise(b.image, targets, T * C, C, N, kBpi, p->cfg.cost_kind,

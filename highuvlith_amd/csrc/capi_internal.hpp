@@ -167,9 +167,13 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
 // Forward images; tune = NULL uses the kernel sets' own coarse grid and the
 // default launch shapes (the ILT adjoint relies on that band layout).
 // pixel_transfer applies to rasters only.
+// split_batch > 0 chooses the field kernel's split of the kernel sum as for a
+// batch of that many clips, whatever nclips is: images then do not depend on
+// which other clips share the call (the ILT relies on it).
 huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                         huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out,
-                        const huv_socs_tuning* tune = nullptr, int max_splits = 0);
+                        const huv_socs_tuning* tune = nullptr, int max_splits = 0,
+                        int split_batch = 0);
 
 }  // namespace huvi
 

@@ -613,6 +613,29 @@ __global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_cols(
       });
 }
 
+// v_permlane32_swap on two floats: afterwards the upper 32 lanes of a hold
+// the lower lanes of b and the lower 32 lanes of b the upper lanes of a (the
+// other halves keep their values).  The results are taken out of the
+// returned vector as integers before the bit cast: a bit cast applied to a
+// vector element directly reads element 0.
+__device__ __forceinline__ void swap32(float& a, float& b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(int, a),
+                                                  __builtin_bit_cast(int, b), false, false);
+  const int r0 = r[0], r1 = r[1];
+  a = __builtin_bit_cast(float, r0);
+  b = __builtin_bit_cast(float, r1);
+}
+
+// u[k] *= e^{SIGN 2 pi i k / (2 L)}, k < L, in the lanes where `on` holds.
+template <int SIGN, int L, int K = 0>
+__device__ __forceinline__ void tw_half_if(c32* u, bool on) {
+  if constexpr (K < L) {
+    const c32 y = twiddle<SIGN, K, 2 * L>(u[K]);
+    u[K] = on ? y : u[K];
+    tw_half_if<SIGN, L, K + 1>(u, on);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // band_image: the n x n real image of a Hermitian half band Ib[gy + G][gx]
 // (gx in [0, G]), I[y][x] = sum_{g} Ib[g] e^{2 pi i (gx x + gy y) / n}.
@@ -624,7 +647,9 @@ __global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_cols(
 //       X[C i + c] = IFFT_32(d)[i],
 //       d[h] = e^{2 pi i h c / n} sum_t z[h + 32 t] e^{2 pi i t c / C}
 //   with z[g] = Ua[g] + i Ub[g] (g >= 0), conj Ua[-g] + i conj Ub[-g] (g < 0);
-//   row a takes Re X, row b Im X.  Writes are coalesced across classes.
+//   row a takes Re X, row b Im X.  Writes are coalesced across classes.  The
+//   classes c and c + C/2 share their fold (even and odd t, two lanes per
+//   pair of classes, one permlane32 swap to regroup by class).
 // Folding ranges t in [TMIN, TMAX] cover |g| <= G with 32 (TMAX) <= G <=
 // 32 (TMAX + 1) - 1 (TMIN = -TMAX - 1), so only the outermost t of either
 // sign can fall outside the band; U rows are zero-padded to |g| <= 32 |TMIN|
@@ -660,47 +685,66 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
   }
   __syncthreads();
   const int nk = 2 * G + 1;
-  // ---- phase 1: two threads per column gx, each folding half of the t range ----
-  constexpr int TMID = TMIN + (TMAX - TMIN + 1) / 2;
-  const int half = tid / (G + 1), gx = tid % (G + 1);
-  c32 u[F];
-  if (half < 2) {
+  // ---- phase 1: lanes l and l + 32 of wave v take column gx = 32 v + l; lane
+  // half hi folds the bins h = 16 hi + h' (h' < 16) over every t.  One radix-2
+  // step across the pair (a permlane32 swap) and an FFT_16 per lane then give
+  // the rows j = 2 j' + hi of U. ----
+  {
+    constexpr int H = F / 2;
+    const int lane = tid & 63, hi = lane >> 5;
+    const int gx = 32 * (tid >> 6) + (lane & 31);
+    const bool col = gx <= G;
+    // lanes past the band fold a valid column and store nothing, so every
+    // lane of a wave takes part in the swap
+    const int gxa = col ? gx : G;
     const __amdgpu_buffer_rsrc_t br = box_rsrc(Ib + (size_t)img * nk * (G + 1), nk * (G + 1));
-    int vo = gx * (int)sizeof(float2);
+    int vo = (gxa + H * hi * (G + 1)) * (int)sizeof(float2);
+    c32 u[H];
 #pragma unroll
-    for (int h = 0; h < F; ++h) u[h] = c32{0.f, 0.f};
+    for (int h = 0; h < H; ++h) u[h] = c32{0.f, 0.f};
 #pragma unroll
     for (int t = TMIN; t <= TMAX; ++t) {
-      if ((t < TMID) != (half == 0)) continue;
-      // Opaque to the optimiser: keeps each band row block's loads after the
-      // previous block instead of hoisting every load to the top.
-      asm volatile("" : "+v"(vo));
+      // one block of 16 loads in flight beside the accumulators
+      __builtin_amdgcn_sched_barrier(0);
       const c32 w = twt[(t - TMIN) * C + p];  // e^{2 pi i t p / P}
-      c32 e[F];
+      c32 e[H];
 #pragma unroll
-      for (int h = 0; h < F; ++h) {
-        const int gy = h + F * t;
-        e[h] = c32{0.f, 0.f};
-        // only the outermost folds can leave the band
-        const bool inb = t == TMIN ? gy >= -G : (t == TMAX ? gy <= G : true);
-        if (inb) e[h] = load_c32(br, vo, (gy + G) * (G + 1) * (int)sizeof(float2));
+      for (int h = 0; h < H; ++h) {
+        // band row H hi + h + F t at byte offset vo + ro
+        const int ro = (h + F * t + G) * (G + 1) * (int)sizeof(float2);
+        // Only the outermost folds can leave the band, and there the whole
+        // offset goes through the lane's register with none in the scalar
+        // offset: a row below the band gives a negative offset, which wraps
+        // past the buffer's end, and a row above it an offset past the end;
+        // the buffer range check returns zero for both.
+        if (t == TMIN || t == TMAX) e[h] = load_c32(br, vo + ro, 0);
+        else e[h] = load_c32(br, vo, ro);
       }
 #pragma unroll
-      for (int h = 0; h < F; ++h) u[h] = t == 0 ? cadd(u[h], e[h]) : cmac(u[h], e[h], w);
+      for (int h = 0; h < H; ++h) u[h] = t == 0 ? cadd(u[h], e[h]) : cmac(u[h], e[h], w);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int h = 0; h < F; ++h) u[h] = cmul(u[h], twh[h * C + p]);
-    fft_reg<+1, F>(u);
-    if (half == 0) {
+    for (int h = 0; h < H; ++h) u[h] = cmul(u[h], twh[(H * hi + h) * C + p]);
+    // radix-2 step of the inverse FFT_32 on x = bin h (lane hi = 0) and
+    // bin h + 16 (lane hi = 1): lane 0 keeps x[h] + x[h + 16] (even rows),
+    // lane 1 (x[h] - x[h + 16]) e^{2 pi i h / 32} (odd rows).  Swapping a = x
+    // with b = x negated in the upper lanes leaves a = x[h] in every lane and
+    // b = x[h + 16] (lower) or -x[h + 16] (upper): a + b is the butterfly.
+    const float sg = hi ? -1.f : 1.f;
 #pragma unroll
-      for (int j = 0; j < F; ++j) U[j * UP + gx] = u[j];
+    for (int h = 0; h < H; ++h) {
+      float ar = u[h].re, br = sg * u[h].re, ai = u[h].im, bi = sg * u[h].im;
+      swap32(ar, br);
+      swap32(ai, bi);
+      u[h] = c32{ar + br, ai + bi};
     }
-  }
-  __syncthreads();
-  if (half == 1) {
+    tw_half_if<+1, H>(u, hi != 0);
+    fft_reg<+1, H>(u);
+    if (col) {
 #pragma unroll
-    for (int j = 0; j < F; ++j) U[j * UP + gx] = cadd(U[j * UP + gx], u[j]);
+      for (int j = 0; j < H; ++j) U[(2 * j + hi) * UP + gx] = u[j];
+    }
   }
   __syncthreads();
   // ---- pair packing, in place: Z[pr][g + GP] = z_pr[g] for |g| <= GP ----
@@ -731,25 +775,50 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
   // ---- phase 2 ----
   const __amdgpu_buffer_rsrc_t orc = __builtin_amdgcn_make_buffer_rsrc(
       out + (size_t)img * n * n, (short)0, n * n * (int)sizeof(float), 0x00020000);
-  for (int task = tid; task < (F / 2) * C; task += 256) {
-    const int pr = task / C, c = task % C;
-    const c32* zr = U + pr * ZW + GP;  // zr[g], |g| <= GP
-    c32 d[F];
+  // Lanes l and l + 32 of a wave share a task (row pair pr, classes c and
+  // c + C/2, c < C/2): e^{2 pi i t (c + C/2) / C} = (-1)^t e^{2 pi i t c / C},
+  // so the even-t and odd-t partial folds E, O give d_c = E + O and
+  // d_{c+C/2} = E - O at once.  Lane l folds the bins h < 16, lane l + 32 the
+  // bins h >= 16; one permlane32 swap per register then hands lane l all 32
+  // bins of class c and lane l + 32 those of class c + C/2.  Every wave is
+  // wholly active or wholly idle in each pass: the 8 C tasks (C = n/32 a
+  // multiple of 4) fill whole waves of 32 tasks.
+  constexpr int H = F / 2;
+  const int lane = tid & 63, hi = lane >> 5;
+  const int ntask = (F / 2) * (C / 2);
+  for (int base = (tid >> 6) * 32; base < ntask; base += 128) {
+    const int task = base + (lane & 31);
+    const int pr = task / (C / 2);
+    int c = task % (C / 2);
+    const c32* zr = U + pr * ZW + GP + H * hi;  // zr[g - H hi], |g| <= GP
+    c32 ev[H], od[H];
 #pragma unroll
-    for (int h = 0; h < F; ++h) d[h] = c32{0.f, 0.f};
+    for (int h = 0; h < H; ++h) ev[h] = od[h] = c32{0.f, 0.f};
 #pragma unroll
     for (int t = TMIN; t <= TMAX; ++t) {
       __builtin_amdgcn_sched_barrier(0);
       const c32 w = twt[(t - TMIN) * C + c];  // e^{2 pi i t c / C}
 #pragma unroll
-      for (int h = 0; h < F; ++h) {
-        const int g = h + F * t;
+      for (int h = 0; h < H; ++h) {
         // Z is zero beyond G, so every fold term is read unconditionally
-        const c32 z = zr[g];
-        d[h] = t == 0 ? cadd(d[h], z) : cmac(d[h], z, w);
+        const c32 z = zr[h + F * t];
+        if ((t & 1) == 0) ev[h] = t == 0 ? cadd(ev[h], z) : cmac(ev[h], z, w);
+        else od[h] = cmac(od[h], z, w);
       }
     }
     __builtin_amdgcn_sched_barrier(0);
+    c32 d[F];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      // a = d_c (bins of this lane), b = d_{c + C/2}; after the swap the lower
+      // lane holds d_c at bins h, h + 16 and the upper lane d_{c + C/2}
+      c32 a = cadd(ev[h], od[h]), b = csub(ev[h], od[h]);
+      swap32(a.re, b.re);
+      swap32(a.im, b.im);
+      d[h] = a;
+      d[h + H] = b;
+    }
+    c += hi * (C / 2);  // this lane's class from here on
 #pragma unroll
     for (int h = 0; h < F; ++h) d[h] = cmul(d[h], twh[h * C + c]);
     fft_reg<+1, F>(d);

@@ -120,6 +120,31 @@ def test_gpu_optimizer_tracks_oracle_run_stage():
         assert abs(g[-1] - hist[-1]) / hist[-1] < 2e-2
 
 
+def test_trajectory_does_not_depend_on_the_batch():
+    """A target's CG / Armijo run is the same alone, in a batch of five, and in
+    a batch whose other targets stop early: bit for bit, theta and history."""
+    import torch
+    eng = quasar_engine(256)
+    cfg = L.ILTConfig(cost="resist", regularization=0.0)
+    prob = L.IltProblem(eng, cfg)
+    targets = torch.as_tensor(W.contact_array_targets(5, seed=8, grid=eng.grid), device="cuda")
+    th5 = prob.initial_theta(targets)
+    _, h5, it5, _ = prob.run_stage(targets, th5, 25)
+    # the others stop after 3 iterations, target 2 runs on alone
+    out = prob.run_stage_traced(targets, prob.initial_theta(targets), 25,
+                                iteration_offset=[22, 22, 0, 22, 22])
+    for t in (0, 2, 4):
+        th1 = prob.initial_theta(targets[t:t + 1])
+        _, h1, it1, _ = prob.run_stage(targets[t:t + 1], th1, 25)
+        print("target", t, "iterations", it1[0], it5[t], "final", h1[0][-1], h5[t][-1])
+        assert it1[0] == it5[t]
+        assert np.array_equal(h1[0], h5[t])
+        assert torch.equal(th1[0], th5[t])
+    assert out["iterations"][2] == it5[2]
+    assert np.array_equal(out["cost_history"][2], h5[2])
+    assert torch.equal(out["theta"][2], th5[2])
+
+
 def test_invalid_ilt_configs_rejected():
     eng = quasar_engine(256)
     with pytest.raises(H.HuvError):
