@@ -33,8 +33,9 @@ Also reported in the same JSON line (each skipped with --no-single,
            cores): trained on (target, 200-iteration ILT theta) pairs from
            the GPU ILT (seeded contact and line/space targets, 512^2 at 2 nm),
            then on held-out targets plain ILT (200 iterations from the
-           target) vs warm start refined to the same per-target cost, with
-           continuation and plain-ILT fallback for targets that stop above
+           target) vs warm start refined to the same per-target cost (no
+           convergence stop above it) within the 200-iteration budget, with
+           plain ILT from the target as the fallback for targets still above
            it (all timed); clips/s to that fixed cost for both.  The default
            sizes are a reduced version of the stated 32/64/128/256-channel,
            20,000-pair, 20-epoch training (--warm-width, --warm-pairs,
@@ -585,7 +586,7 @@ def bench_warm(args, H, W, np, torch):
             "warm_clips_per_s": round(T / res.warm_wall_s, 3),
             "unit": "clips/s to the plain 200-iteration cost",
             "warm_over_plain_wall": round(res.warm_wall_s / res.plain_wall_s, 4),
-            "warm_wall_includes": "inference, refinement, continuation and plain-ILT fallback",
+            "warm_wall_includes": "inference, refinement and plain-ILT fallback",
             "warm_stages_s": res.stages_s,
             "plain_mean_iterations": float(np.mean(res.plain_iterations)),
             "warm_mean_iterations": float(np.mean(res.warm_iterations)),

@@ -702,12 +702,8 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     c32 u[H];
 #pragma unroll
     for (int h = 0; h < H; ++h) u[h] = c32{0.f, 0.f};
-#pragma unroll
-    for (int t = TMIN; t <= TMAX; ++t) {
-      // one block of 16 loads in flight beside the accumulators
-      __builtin_amdgcn_sched_barrier(0);
-      const c32 w = twt[(t - TMIN) * C + p];  // e^{2 pi i t p / P}
-      c32 e[H];
+    // The 16 band rows of fold t, this lane's column.
+    auto load_block = [&](int t, c32* e) {
 #pragma unroll
       for (int h = 0; h < H; ++h) {
         // band row H hi + h + F t at byte offset vo + ro
@@ -720,6 +716,15 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
         if (t == TMIN || t == TMAX) e[h] = load_c32(br, vo + ro, 0);
         else e[h] = load_c32(br, vo, ro);
       }
+    };
+    // One block of 16 loads in flight beside the accumulators (requesting
+    // fold t + 1's rows before fold t is accumulated measured no faster).
+#pragma unroll
+    for (int t = TMIN; t <= TMAX; ++t) {
+      __builtin_amdgcn_sched_barrier(0);
+      c32 e[H];
+      load_block(t, e);
+      const c32 w = twt[(t - TMIN) * C + p];  // e^{2 pi i t p / P}
 #pragma unroll
       for (int h = 0; h < H; ++h) u[h] = t == 0 ? cadd(u[h], e[h]) : cmac(u[h], e[h], w);
     }

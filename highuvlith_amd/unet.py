@@ -115,9 +115,9 @@ class WarmStartResult:
     warm_iterations: np.ndarray  # all iterations spent on the warm side, per target
     reached: np.ndarray          # final cost at or below the plain cost
     fallback: np.ndarray         # warm start worse than the target at iteration 0
-    continued: np.ndarray        # refinement stopped above the plain cost, continued
+    continued: np.ndarray        # the convergence test would have stopped it above the plain cost
     plain_fallback: np.ndarray   # still above after the budget: plain ILT from the target
-    reached_by_refinement: np.ndarray  # reached in the first refinement stage
+    reached_by_refinement: np.ndarray  # reached before the convergence test would have fired
     stages_s: dict = None        # wall seconds of each warm-side stage
 
 
@@ -128,6 +128,21 @@ class WarmStartResult:
 REACH_RTOL = 0.0
 
 
+def convergence_iteration(history, tol):
+    """First iteration at which the driver's convergence test (three
+    consecutive relative decreases below tol, ilt.rs:755-866) fires on this
+    cost history, or -1.  A history that ends at its stop cost is scanned up
+    to its last step only, where the stop cost takes precedence."""
+    small = 0
+    for i in range(1, len(history) - 1):
+        prev = history[i - 1]
+        rel = (prev - history[i]) / prev if prev > 0.0 else 0.0
+        small = small + 1 if rel < tol else 0
+        if small >= 3:
+            return i
+    return -1
+
+
 def compare_warm_start(engine, net, targets, config, budget=200):
     """Plain ILT from the target for `budget` iterations vs the network warm
     start, each target refined until it reaches its plain cost, with plain ILT
@@ -135,12 +150,13 @@ def compare_warm_start(engine, net, targets, config, budget=200):
 
     1. inference; a target whose warm cost exceeds its target-initialised cost
        starts from the target instead (`fallback`);
-    2. refinement (same optimiser and convergence test) until the plain cost;
-    3. a target that stopped above the plain cost continues without the
-       convergence test until it reaches it or the budget is spent
-       (`continued`);
-    4. a target still above the plain cost is re-run as plain ILT from the
-       target until it reaches it (`plain_fallback`).
+    2. refinement with the same optimiser until the plain cost or the budget,
+       all targets in one batched run: the convergence test, which would stop
+       a target that stalls above the plain cost, is off (`continued` counts
+       the targets it would have stopped there; `reached_by_refinement` those
+       that reach the plain cost before it would have fired);
+    3. a target still above the plain cost after the budget is re-run as plain
+       ILT from the target until it reaches it (`plain_fallback`).
     """
     import torch
     from .ilt import IltProblem
@@ -178,23 +194,14 @@ def compare_warm_start(engine, net, targets, config, budget=200):
         sel = torch.as_tensor(fallback, device="cuda")
         theta_w[sel] = prob.initial_theta(tg)[sel]
     t1 = lap("inference", t0)
-    _, hist_w, its_w, _ = prob.run_stage(tg, theta_w, budget, stop_cost=goal)
+    out = prob.run_stage_traced(tg, theta_w, budget, stop_cost=goal, convergence_tol=0.0)
     t1 = lap("refinement", t1)
-    its = np.asarray(its_w).copy()
-    cost = np.array([h[-1] for h in hist_w])
-    by_refinement = cost <= goal
-    # continuation without the convergence test, within the remaining budget
-    continued = (cost > goal) & (its < budget)
-    if continued.any():
-        idx = np.flatnonzero(continued)
-        sub = torch.as_tensor(idx, device="cuda")
-        th = theta_w[sub].contiguous()
-        out = prob.run_stage_traced(tg[sub], th, budget, stop_cost=goal[idx],
-                                    iteration_offset=its[idx], convergence_tol=0.0)
-        theta_w[sub] = out["theta"]
-        its[idx] += out["iterations"]
-        cost[idx] = [h[-1] for h in out["cost_history"]]
-    t1 = lap("continuation", t1)
+    its = np.asarray(out["iterations"]).copy()
+    cost = np.array([h[-1] for h in out["cost_history"]])
+    stall = np.array([convergence_iteration(h, config.convergence_tol)
+                      for h in out["cost_history"]])
+    continued = stall >= 0   # the convergence test would have stopped it above the goal
+    by_refinement = (cost <= goal) & ~continued
     # plain ILT from the target for whatever is still above
     plain_fb = cost > goal
     if plain_fb.any():
