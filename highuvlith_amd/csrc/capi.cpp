@@ -1067,7 +1067,8 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
 // Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
 huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                         huv_kset* const* planes, float* out, bool pt, float2** Mq_out,
-                        const huv_socs_tuning* tune, int max_splits, int split_batch) {
+                        const huv_socs_tuning* tune, int max_splits, int split_batch,
+                        const huvk::FidelityEpi* fid) {
   const int B = planes[0]->B;
   const int m = tune && tune->m ? tune->m : planes[0]->m;
   const int BB = huvk::coarse_band(m);
@@ -1129,20 +1130,32 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int np
   const float2* tw_n = c->twiddles(n, e);
   if (!tw_n) return hip_fail(e, "twiddle table");
   const int G = 2 * B;
-  float2* tmp1 = (float2*)c->buf(Workspace::TMP_COARSE, (size_t)nimg * (2 * G + 1) * (m / 2) * sizeof(float2), e);
-  if (!tmp1) return hip_fail(e, "workspace");
+  float2* tmp1 = nullptr;
+  const bool fused = huvk::coarse_fused_supported(m);
+  if (!fused) {
+    tmp1 = (float2*)c->buf(Workspace::TMP_COARSE, (size_t)nimg * (2 * G + 1) * (m / 2) * sizeof(float2), e);
+    if (!tmp1) return hip_fail(e, "workspace");
+  }
   float2* Ib = (float2*)c->buf(Workspace::IBAND, (size_t)nimg * (G + 1) * (2 * G + 1) * sizeof(float2), e);
   if (!Ib) return hip_fail(e, "workspace");
-  c->stage_begin(ST_COARSE_ROWS, &ev);
-  e = huvk::launch_coarse_rows(m, B, nimg, S, coarse, tw_m, tmp1, c->stream());
-  if (e != hipSuccess) return hip_fail(e, "coarse_rows launch");
-  c->stage_end(ST_COARSE_ROWS, ev);
-  c->stage_begin(ST_COARSE_COLS, &ev);
-  e = huvk::launch_coarse_cols(m, B, nimg, (float)flare, tmp1, tw_m, Ib, c->stream());
-  if (e != hipSuccess) return hip_fail(e, "coarse_cols launch");
-  c->stage_end(ST_COARSE_COLS, ev);
+  if (fused) {
+    c->stage_begin(ST_COARSE, &ev);
+    e = huvk::launch_coarse_fused(m, B, nimg, S, coarse, (float)flare, tw_m, Ib, c->stream());
+    if (e != hipSuccess) return hip_fail(e, "coarse launch");
+    c->stage_end(ST_COARSE, ev);
+  } else {
+    c->stage_begin(ST_COARSE_ROWS, &ev);
+    e = huvk::launch_coarse_rows(m, B, nimg, S, coarse, tw_m, tmp1, c->stream());
+    if (e != hipSuccess) return hip_fail(e, "coarse_rows launch");
+    c->stage_end(ST_COARSE_ROWS, ev);
+    c->stage_begin(ST_COARSE_COLS, &ev);
+    e = huvk::launch_coarse_cols(m, B, nimg, (float)flare, tmp1, tw_m, Ib, c->stream());
+    if (e != hipSuccess) return hip_fail(e, "coarse_cols launch");
+    c->stage_end(ST_COARSE_COLS, ev);
+  }
   c->stage_begin(ST_BAND_IMAGE, &ev);
-  e = huvk::launch_band_image(n, G, nimg, Ib, tw_n, out, c->stream());
+  e = fid ? huvk::launch_band_image_fidelity(n, G, nimg, Ib, tw_n, *fid, c->stream())
+          : huvk::launch_band_image(n, G, nimg, Ib, tw_n, out, c->stream());
   if (e != hipSuccess) return hip_fail(e, "band_image launch");
   c->stage_end(ST_BAND_IMAGE, ev);
   return HUV_OK;

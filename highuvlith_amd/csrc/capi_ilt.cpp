@@ -41,7 +41,7 @@ namespace {
 constexpr int kBpi = 64;  // reduction blocks per image
 
 struct IltBuffers {
-  float *m, *t, *image, *G, *dcdm, *reg_grad, *G0g;
+  float *m, *t, *G, *dcdm, *reg_grad, *G0g;
   float2 *Gb, *Rout, *H;
   float2* tc = nullptr;                       // complex transmittance (complex absorber)
   float *thf = nullptr, *gm = nullptr;        // filtered theta, chain-rule temporary (filter)
@@ -71,9 +71,7 @@ huv_status ilt_buffers(Ctx* c, const huv_ilt* p, int T, IltBuffers& b) {
   b.m = (float*)a;
   b.t = b.m + (size_t)T * N;
   b.tc = p->complex_abs ? (float2*)b.t : nullptr;
-  b.image = (float*)c->ws.get(Workspace::ILT1, TC * N * sizeof(float), e);
-  if (!b.image) return hip_fail(e, "ILT workspace");
-  b.G = (float*)c->ws.get(Workspace::ILT2, TC * N * sizeof(float), e);
+  b.G =(float*)c->ws.get(Workspace::ILT2, TC * N * sizeof(float), e);
   if (!b.G) return hip_fail(e, "ILT workspace");
   a = (char*)c->ws.get(Workspace::ILT3, 2 * (size_t)T * N * sizeof(float), e);
   if (!a) return hip_fail(e, "ILT workspace");
@@ -135,61 +133,99 @@ huv_status ilt_filter(huv_ilt* p, int count, const float* in, float* out, IltBuf
   return e == hipSuccess ? HUV_OK : hip_fail(e, "ilt filter image launch");
 }
 
-// m (and the transmittance t) of count targets from theta: the density
-// filter first when configured.
-huv_status ilt_mask_t(huv_ilt* p, int count, const float* theta, float* m, void* t,
+// Where the theta rows of an evaluation come from: row j is
+// theta[rows[j]] + coef[j] dir[rows[j]] (rows null: row j; coef null: theta
+// alone), read in place.  stage: room for count compact rows, written only
+// when the density filter needs the rows as fields.
+struct ThetaRows {
+  const float* theta = nullptr;
+  const int* rows = nullptr;
+  const float* dir = nullptr;
+  const double* coef = nullptr;
+  float* stage = nullptr;
+};
+
+// m (may be null) and the transmittance t (may be null; complex with a
+// complex absorber) of count evaluation rows: the density filter first when
+// configured.
+huv_status ilt_mask_t(huv_ilt* p, int count, const ThetaRows& src, float* m, void* t,
                       IltBuffers& b) {
   Ctx* c = p->ctx;
-  const int64_t total = (int64_t)count * p->n * p->n;
-  const float* th = theta;
-  if (p->Bf > 0) {
-    huv_status st = ilt_filter(p, count, theta, b.thf, b);
-    if (st != HUV_OK) return st;
-    th = b.thf;
-  }
+  hipStream_t s = c->stream();
+  const int64_t N = (int64_t)p->n * p->n;
+  const float beta = (float)p->cfg.beta, ar = (float)p->cfg.absorber_re;
+  const float ai = (float)p->cfg.absorber_im;
+  float* tr = p->complex_abs ? nullptr : (float*)t;
+  float2* tc = p->complex_abs ? (float2*)t : nullptr;
   hipError_t e;
-  if (p->complex_abs || !t)
-    e = huvk::launch_ilt_mask_c(th, m, (float2*)t, total, (float)p->cfg.beta,
-                                (float)p->cfg.absorber_re, (float)p->cfg.absorber_im, c->stream());
-  else
-    e = huvk::launch_ilt_mask(th, m, (float*)t, total, (float)p->cfg.beta,
-                              (float)p->cfg.absorber_re, c->stream());
+  if (p->Bf > 0) {
+    // the filter acts on theta as a field: materialise the rows first
+    const float* th = src.theta;
+    if (src.coef || src.rows) {
+      if (!src.stage || (src.coef && !src.rows))
+        return fail(HUV_INVALID_PARAMETER, "filtered evaluation rows need a staging buffer");
+      e = src.coef ? huvk::launch_vec_axpy_gather(src.theta, src.dir, src.coef, N, count, src.rows,
+                                                  src.stage, s)
+                   : huvk::launch_rows_gather(src.theta, src.rows, N, count, src.stage, s);
+      if (e != hipSuccess) return hip_fail(e, "ilt theta rows");
+      th = src.stage;
+    }
+    huv_status st = ilt_filter(p, count, th, b.thf, b);
+    if (st != HUV_OK) return st;
+    e = huvk::launch_ilt_mask_rows(b.thf, nullptr, nullptr, nullptr, N, count, beta, ar, ai, m, tr,
+                                   tc, s);
+  } else {
+    e = huvk::launch_ilt_mask_rows(src.theta, src.rows, src.dir, src.coef, N, count, beta, ar, ai,
+                                   m, tr, tc, s);
+  }
   return e == hipSuccess ? HUV_OK : hip_fail(e, "ilt mask launch");
 }
 
-// Cost (into b.cost_d, device) and optionally the theta-gradient.
-huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const float* theta, float* grad,
-                    IltBuffers& b) {
+// Cost of T evaluation rows (into b.cost_d, device) and optionally the
+// theta-gradient.  Row j compares with target row trows[j] (null: j) and its
+// gradient goes to grad row grows[j] (null: j).
+huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const int* trows,
+                    const ThetaRows& src, float* grad, const int* grows, IltBuffers& b) {
   Ctx* c = p->ctx;
   hipStream_t s = c->stream();
   hipError_t e;
   const int n = p->n, C = p->ncond, B = p->B, m = p->m;
   const int64_t N = (int64_t)n * n;
+  const bool reg = p->cfg.tv_weight > 0.0 || p->cfg.bin_weight > 0.0;
   hipEvent_t ev;
   c->stage_begin(ST_ILT_POINTWISE, &ev);
-  huv_status st = ilt_mask_t(p, T, theta, b.m, p->complex_abs ? (void*)b.tc : (void*)b.t, b);
+  huv_status st = ilt_mask_t(p, T, src, grad || reg ? b.m : nullptr,
+                             p->complex_abs ? (void*)b.tc : (void*)b.t, b);
   if (st != HUV_OK) return st;
   c->stage_end(ST_ILT_POINTWISE, ev);
   float2* Mq = nullptr;
-  MaskSource src = MaskSource::real(b.t);
-  if (p->complex_abs) src.kind = MaskSource::COMPLEX;
+  MaskSource mask = MaskSource::real(b.t);
+  if (p->complex_abs) mask.kind = MaskSource::COMPLEX;
   // The kernel sum is split as for a single target at every batch size, so a
   // target's images, and with them its whole optimisation trajectory, do not
-  // depend on which other targets are still live in the batch.
-  st = socs_forward(c, T, n, src, C, p->conds.data(), b.image, false, &Mq, nullptr, 0, 1);
+  // depend on which other targets are still live in the batch.  The images
+  // are never stored: the last stage evaluates the fidelity (and G) of every
+  // pixel where it would have written it, with per-row-class partial sums.
+  huvk::FidelityEpi fid;
+  fid.target = targets;
+  fid.trows = trows;
+  fid.ncond = C;
+  fid.cost_kind = p->cfg.cost_kind;
+  fid.threshold = (float)p->cfg.threshold;
+  fid.steepness = (float)p->cfg.steepness;
+  fid.inv_n = 1.0f / (float)N;
+  fid.dose = p->dose_d;
+  fid.G = grad ? b.G : nullptr;
+  fid.partial = b.fid_part;
+  st = socs_forward(c, T, n, mask, C, p->conds.data(), nullptr, false, &Mq, nullptr, 0, 1, &fid);
   if (st != HUV_OK) return st;
   c->stage_begin(ST_ILT_POINTWISE, &ev);
-  e = huvk::launch_ilt_pointwise(b.image, targets, T * C, C, N, kBpi, p->cfg.cost_kind,
-                                 (float)p->cfg.threshold, (float)p->cfg.steepness, p->dose_d,
-                                 grad ? b.G : nullptr, b.fid_part, s);
-  if (e != hipSuccess) return hip_fail(e, "ilt pointwise launch");
-  const bool reg = p->cfg.tv_weight > 0.0 || p->cfg.bin_weight > 0.0;
   if (reg) {
     e = huvk::launch_ilt_reg(b.m, n, T, kBpi, (float)p->cfg.tv_weight, (float)p->cfg.tv_epsilon,
                              (float)p->cfg.bin_weight, b.reg_grad, b.reg_part, s);
     if (e != hipSuccess) return hip_fail(e, "ilt reg launch");
   }
-  e = huvk::launch_ilt_cost_reduce(b.fid_part, reg ? b.reg_part : nullptr, T, C, kBpi, kBpi,
+  e = huvk::launch_ilt_cost_reduce(b.fid_part, reg ? b.reg_part : nullptr, T, C, n / 32, kBpi,
                                    p->weight_d, 1.0 / (double)N, b.cost_d, s);
   if (e != hipSuccess) return hip_fail(e, "ilt cost reduce launch");
   c->stage_end(ST_ILT_POINTWISE, ev);
@@ -242,12 +278,18 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const float* theta,
   }
   // chain through the sigmoid, then the (self-adjoint) density filter
   c->stage_begin(ST_ILT_OTHER, &ev);
-  e = huvk::launch_ilt_chain(b.dcdm, reg ? b.reg_grad : nullptr, b.m, (float)p->cfg.beta,
-                             (int64_t)T * N, p->Bf > 0 ? b.gm : grad, s);
+  const bool filt = p->Bf > 0;
+  e = huvk::launch_ilt_chain(b.dcdm, reg ? b.reg_grad : nullptr, b.m, (float)p->cfg.beta, N, T,
+                             filt ? nullptr : grows, filt ? b.gm : grad, s);
   if (e != hipSuccess) return hip_fail(e, "ilt chain launch");
-  if (p->Bf > 0) {
-    st = ilt_filter(p, T, b.gm, grad, b);
+  if (filt) {
+    // the filter writes compact rows; those of a row subset go out through thf
+    st = ilt_filter(p, T, b.gm, grows ? b.thf : grad, b);
     if (st != HUV_OK) return st;
+    if (grows) {
+      e = huvk::launch_rows_scatter(b.thf, grows, N, T, grad, s);
+      if (e != hipSuccess) return hip_fail(e, "ilt gradient scatter");
+    }
   }
   c->stage_end(ST_ILT_OTHER, ev);
   return HUV_OK;
@@ -340,7 +382,9 @@ huv_status huv_ilt_cost_grad(huv_ilt* p, int T, const float* targets, const floa
   IltBuffers b;
   huv_status st = ilt_buffers(c, p, T, b);
   if (st != HUV_OK) return st;
-  st = ilt_eval(p, T, targets, theta, grad, b);
+  ThetaRows src;
+  src.theta = theta;
+  st = ilt_eval(p, T, targets, nullptr, src, grad, nullptr, b);
   if (st != HUV_OK) return st;
   hipError_t e = hipMemcpyAsync(cost, b.cost_d, T * sizeof(double), hipMemcpyDeviceToHost, c->stream());
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream());
@@ -458,20 +502,14 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     e = huvk::launch_vec_dir(g, ca, cb, N, (int)rows.size(), rd, dir, s);
     return e == hipSuccess;
   };
-  auto copy_rows = [&](const float* src, float* dst, const std::vector<int>& rows) -> bool {
-    if (rows.empty()) return true;
-    const int* rd = stage_rows(rows);
-    if (e != hipSuccess || !flush()) return false;
-    e = huvk::launch_vec_copy_rows(src, N, (int)rows.size(), rd, dst, s);
-    return e == hipSuccess;
-  };
-  // Targets that are done cost nothing: evaluations gather the rows of the
-  // targets in `which` into compact buffers (all targets: no copies).
-  char* cmp = (char*)c->ws.get(Workspace::ILT8, 3 * vb, e);
+  // Targets that are done cost nothing: evaluations read the rows of the
+  // targets in `which` in place through a row list and write their gradients
+  // back the same way.  th_c: compact theta rows for the density filter;
+  // g_c: compact masks for snapshots.
+  char* cmp = (char*)c->ws.get(Workspace::ILT8, 2 * vb, e);
   if (!cmp) return hip_fail(e, "ILT optimizer workspace");
   float* th_c = (float*)cmp;
-  float* tg_c = (float*)(cmp + vb);
-  float* g_c = (float*)(cmp + 2 * vb);
+  float* g_c = (float*)(cmp + vb);
   std::vector<double> cost_c(T);
   auto eval_sel = [&](const float* th, float* g, std::vector<double>& out,
                       const std::vector<int>& which) -> huv_status {
@@ -479,7 +517,9 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     if (Tc == 0) return HUV_OK;
     if (Tc == T) {
       if (!flush()) return hip_fail(e, "staging copy");
-      huv_status r = ilt_eval(p, T, targets, th, g, b);
+      ThetaRows src;
+      src.theta = th;
+      huv_status r = ilt_eval(p, T, targets, nullptr, src, g, nullptr, b);
       if (r != HUV_OK) return r;
       e = hipMemcpyAsync(out.data(), b.cost_d, T * sizeof(double), hipMemcpyDeviceToHost, s);
       if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -488,15 +528,12 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     }
     const int* idx = stage_rows(which);
     if (!idx || !flush()) return hip_fail(e, "row staging");
-    e = huvk::launch_rows_gather(th, idx, N, Tc, th_c, s);
-    if (e == hipSuccess) e = huvk::launch_rows_gather(targets, idx, N, Tc, tg_c, s);
-    if (e != hipSuccess) return hip_fail(e, "target gather");
-    huv_status r = ilt_eval(p, Tc, tg_c, th_c, g ? g_c : nullptr, b);
+    ThetaRows src;
+    src.theta = th;
+    src.rows = idx;
+    src.stage = th_c;
+    huv_status r = ilt_eval(p, Tc, targets, idx, src, g, idx, b);
     if (r != HUV_OK) return r;
-    if (g) {
-      e = huvk::launch_rows_scatter(g_c, idx, N, Tc, g, s);
-      if (e != hipSuccess) return hip_fail(e, "gradient scatter");
-    }
     e = hipMemcpyAsync(cost_c.data(), b.cost_d, Tc * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     synced();
@@ -615,10 +652,14 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         const int* prow_d = (const int*)put(prow.data(), np * sizeof(int));
         const double* pcoef_d = (const double*)put(pcoef.data(), np * sizeof(double));
         if (!prow_d || !pcoef_d || !flush()) return fail_e("trial staging");
-        e = huvk::launch_vec_axpy_gather(theta, dir, pcoef_d, N, np, prow_d, th_c, s);
-        if (e == hipSuccess) e = huvk::launch_rows_gather(targets, prow_d, N, np, tg_c, s);
-        if (e != hipSuccess) return fail_e("trial build");
-        st = ilt_eval(p, np, tg_c, th_c, nullptr, b);
+        // trial rows are read in place: theta + coef dir of their target
+        ThetaRows trial;
+        trial.theta = theta;
+        trial.rows = prow_d;
+        trial.dir = dir;
+        trial.coef = pcoef_d;
+        trial.stage = th_c;
+        st = ilt_eval(p, np, targets, prow_d, trial, nullptr, nullptr, b);
         if (st != HUV_OK) return st;
         e = hipMemcpyAsync(cost_c.data(), b.cost_d, np * sizeof(double), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -648,10 +689,14 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
           }
         }
         if (!acc_src.empty()) {
-          const int* sr = (const int*)put(acc_src.data(), acc_src.size() * sizeof(int));
+          // theta += coef dir in place, rounded exactly as the trial was
+          std::vector<double> acc_coef(acc_src.size());
+          for (size_t k = 0; k < acc_src.size(); ++k) acc_coef[k] = pcoef[acc_src[k]];
           const int* dr = (const int*)put(acc_dst.data(), acc_dst.size() * sizeof(int));
-          if (!sr || !dr || !flush()) return fail_e("accept staging");
-          e = huvk::launch_rows_copy_map(th_c, sr, theta, dr, N, (int)acc_src.size(), s);
+          const double* cf =
+              (const double*)put(acc_coef.data(), acc_coef.size() * sizeof(double));
+          if (!dr || !cf || !flush()) return fail_e("accept staging");
+          e = huvk::launch_vec_axpy_rows(theta, dir, cf, N, (int)acc_src.size(), dr, s);
           if (e != hipSuccess) return fail_e("accept copy");
         }
       }
@@ -714,7 +759,10 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       if (active[t] && it[t] >= limit[t]) active[t] = 0;  // MaxIterations
     }
     if (!dir_rows(gnew, which)) return fail_e("dir");
-    if (!copy_rows(gnew, grad, which)) return fail_e("gradient copy");
+    // The new gradients become the current ones by exchanging the buffers:
+    // every target still active next iteration is in `which`, so the rows
+    // read later are all current; the other rows are never read again.
+    std::swap(grad, gnew);
     // snapshots of the masks of targets whose global iteration is due
     snap_src.clear();
     snap_dst.clear();
@@ -730,9 +778,11 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       const int* sr = (const int*)put(snap_src.data(), cnt * sizeof(int));
       const int* dr = (const int*)put(snap_dst.data(), cnt * sizeof(int));
       if (!sr || !dr || !flush()) return fail_e("snapshot staging");
-      e = huvk::launch_rows_gather(theta, sr, N, cnt, th_c, s);
-      if (e != hipSuccess) return fail_e("snapshot gather");
-      st = ilt_mask_t(p, cnt, th_c, g_c, nullptr, b);
+      ThetaRows snap;
+      snap.theta = theta;
+      snap.rows = sr;
+      snap.stage = th_c;
+      st = ilt_mask_t(p, cnt, snap, g_c, nullptr, b);
       if (st != HUV_OK) return st;
       e = huvk::launch_rows_scatter(g_c, dr, N, cnt, tr.snapshots, s);
       if (e != hipSuccess) return fail_e("snapshot scatter");
@@ -800,7 +850,9 @@ huv_status huv_ilt_mask(huv_ilt* p, int T, const float* theta, float* m) {
   IltBuffers b;
   huv_status st = ilt_buffers(c, p, T, b);
   if (st != HUV_OK) return st;
-  return ilt_mask_t(p, T, theta, m, nullptr, b);
+  ThetaRows src;
+  src.theta = theta;
+  return ilt_mask_t(p, T, src, m, nullptr, b);
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/huv.h"
+#include "hip/socs_kernels.hpp"
 #include "host/engine.hpp"
 
 // A kernel set built on the host (huv_kernels_build) or with the device's
@@ -113,7 +114,7 @@ SocsSizes socs_sizes(int nclips, int n, int B, int m, bool paired, int nplanes, 
 
 enum Stage { ST_ROWS_BAND = 0, ST_COLS_BAND, ST_FIELDS, ST_COARSE_ROWS, ST_COARSE_COLS,
              ST_BAND_IMAGE, ST_ILT_POINTWISE, ST_ILT_ADJOINT, ST_ILT_OTHER, ST_RECT_TABLES,
-             ST_RECT_BAND, ST_POLY_BAND, ST_COUNT };
+             ST_RECT_BAND, ST_POLY_BAND, ST_COARSE, ST_COUNT };
 
 // Band spectrum of nclips real n x n masks: bins |k| <= B computed, stored
 // in boxes of half-width BB >= B (zero beyond B).  Layout Mq[clip][kx + BB][ky + BB];
@@ -169,11 +170,13 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
 // pixel_transfer applies to rasters only.
 // split_batch > 0 chooses the field kernel's split of the kernel sum as for a
 // batch of that many clips, whatever nclips is: images then do not depend on
-// which other clips share the call (the ILT relies on it).
+// which other clips share the call (the ILT relies on it).  fid: the last
+// stage evaluates the ILT fidelity of each image instead of writing it (out
+// unused).
 huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                         huv_kset* const* planes, float* out, bool pixel_transfer, float2** Mq_out,
                         const huv_socs_tuning* tune = nullptr, int max_splits = 0,
-                        int split_batch = 0);
+                        int split_batch = 0, const huvk::FidelityEpi* fid = nullptr);
 
 }  // namespace huvi
 

@@ -21,27 +21,50 @@
 
 namespace huvk {
 
-// m = sigma(beta theta), t = a + (1 - a) m.
-__global__ void k_ilt_mask(const float* __restrict__ theta, float* __restrict__ m,
-                           float* __restrict__ t, int64_t total, float beta, float a) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+// m and t of evaluation rows read in place from the optimiser's vectors:
+// row j's theta is theta[rows[j]] + coef[j] dir[rows[j]] (rows null: row j;
+// coef null: theta alone), computed in FP64 and rounded as the line search's
+// trial points are, so no compact copy of the rows is written.  m, t, tc may
+// each be null; tc (complex absorber ar + i ai) replaces t when given.
+// Grid: x over float4 pixel groups, y over rows.
+__global__ void k_ilt_mask_rows(const float* __restrict__ theta, const int* __restrict__ rows,
+                                const float* __restrict__ dir, const double* __restrict__ coef,
+                                int64_t npix, float beta, float ar, float ai,
+                                float* __restrict__ m, float* __restrict__ t,
+                                float2* __restrict__ tc) {
+  const int j = blockIdx.y;
+  const size_t src = (size_t)(rows ? rows[j] : j) * npix, dst = (size_t)j * npix;
+  const double c = coef ? coef[j] : 0.0;
+  const int64_t n4 = npix / 4;
+  const float4* th4 = reinterpret_cast<const float4*>(theta + src);
+  const float4* d4 = coef ? reinterpret_cast<const float4*>(dir + src) : nullptr;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float mm = 1.0f / (1.0f + expf(-beta * theta[i]));
-    m[i] = mm;
-    t[i] = a + (1.0f - a) * mm;
-  }
-}
-
-// Complex absorber a (attenuated PSM): m = sigma(beta theta), t = a + (1 - a) m
-// as interleaved complex values (ilt.rs:544-552).
-__global__ void k_ilt_mask_c(const float* __restrict__ theta, float* __restrict__ m,
-                             float2* __restrict__ t, int64_t total, float beta, float ar,
-                             float ai) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const float mm = 1.0f / (1.0f + expf(-beta * theta[i]));
-    if (m) m[i] = mm;
-    if (t) t[i] = make_float2(ar + (1.0f - ar) * mm, ai - ai * mm);
+    float v[4];
+    const float4 a = th4[i];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    if (d4) {
+      const float4 d = d4[i];
+      v[0] = (float)((double)v[0] + c * (double)d.x);
+      v[1] = (float)((double)v[1] + c * (double)d.y);
+      v[2] = (float)((double)v[2] + c * (double)d.z);
+      v[3] = (float)((double)v[3] + c * (double)d.w);
+    }
+    float mm[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mm[k] = 1.0f / (1.0f + expf(-beta * v[k]));
+    if (m) reinterpret_cast<float4*>(m + dst)[i] = make_float4(mm[0], mm[1], mm[2], mm[3]);
+    if (tc) {
+      float4* o = reinterpret_cast<float4*>(tc + dst);
+      o[2 * i] = make_float4(ar + (1.0f - ar) * mm[0], ai - ai * mm[0], ar + (1.0f - ar) * mm[1],
+                             ai - ai * mm[1]);
+      o[2 * i + 1] = make_float4(ar + (1.0f - ar) * mm[2], ai - ai * mm[2],
+                                 ar + (1.0f - ar) * mm[3], ai - ai * mm[3]);
+    } else if (t) {
+      reinterpret_cast<float4*>(t + dst)[i] =
+          make_float4(ar + (1.0f - ar) * mm[0], ar + (1.0f - ar) * mm[1], ar + (1.0f - ar) * mm[2],
+                      ar + (1.0f - ar) * mm[3]);
+    }
   }
 }
 
@@ -78,41 +101,10 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// Fidelity term of image img = target * ncond + cond: cost partials per block
-// and G = dC/dI (unweighted by the condition weight).
 constexpr int PW_THREADS = 256;
-__global__ __launch_bounds__(PW_THREADS) void k_ilt_pointwise(
-    const float* __restrict__ image, const float* __restrict__ target, int ncond, int64_t npix,
-    int blocks_per_img, int cost_kind, float threshold, float steepness,
-    const float* __restrict__ dose, float* __restrict__ G, double* __restrict__ partial) {
-  __shared__ double sh[PW_THREADS];
-  const int img = blockIdx.y;
-  const int tgt = img / ncond, c = img % ncond;
-  const float d = dose[c];
-  const float inv_n = 1.0f / (float)npix;
-  const float* I = image + (size_t)img * npix;
-  const float* Z = target + (size_t)tgt * npix;
-  float* g = G ? G + (size_t)img * npix : nullptr;
-  double acc = 0.0;
-  const int64_t chunk = (npix + blocks_per_img - 1) / blocks_per_img;
-  const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
-  for (int64_t i = lo + threadIdx.x; i < hi; i += PW_THREADS) {
-    const float di = d * I[i];
-    float r, gi;
-    if (cost_kind == 0) {
-      r = di - Z[i];
-      gi = 2.0f * r * d * inv_n;
-    } else {
-      const float rr = 1.0f / (1.0f + expf(-steepness * (di - threshold)));
-      r = rr - Z[i];
-      gi = 2.0f * r * steepness * rr * (1.0f - rr) * d * inv_n;
-    }
-    acc += (double)r * (double)r;
-    if (g) g[i] = gi;
-  }
-  const double s = block_sum<PW_THREADS>(acc, sh);
-  if (threadIdx.x == 0) partial[(size_t)img * blocks_per_img + blockIdx.x] = s;
-}
+
+// (The fidelity term of each image and G = dC/dI are evaluated by the
+// band-image stage itself, launch_band_image_fidelity.)
 
 // Regularisers on m (ilt.rs:606-645): cost partials per block and gradient.
 __global__ __launch_bounds__(PW_THREADS) void k_ilt_reg(
@@ -589,14 +581,17 @@ __global__ void k_ilt_proxy(const float* __restrict__ G, const double* __restric
 }
 
 // grad = (dcdm + reg) * beta m (1 - m).
+// grad row orows[j] (null: j) of evaluation row j.  Grid: x over pixels, y over rows.
 __global__ void k_ilt_chain(const float* __restrict__ dcdm, const float* __restrict__ reg,
-                            const float* __restrict__ m, float beta, int64_t total,
-                            float* __restrict__ grad) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+                            const float* __restrict__ m, float beta, int64_t npix,
+                            const int* __restrict__ orows, float* __restrict__ grad) {
+  const int j = blockIdx.y;
+  const size_t a = (size_t)j * npix, o = (size_t)(orows ? orows[j] : j) * npix;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float mm = m[i];
-    const float r = reg ? reg[i] : 0.0f;
-    grad[i] = (dcdm[i] + r) * beta * mm * (1.0f - mm);
+    const float mm = m[a + i];
+    const float r = reg ? reg[a + i] : 0.0f;
+    grad[o + i] = (dcdm[a + i] + r) * beta * mm * (1.0f - mm);
   }
 }
 
@@ -660,15 +655,6 @@ __global__ void k_vec_dir(const float* __restrict__ g, const double* __restrict_
     d[base + i] = (float)(ca * (double)g[base + i] + cb * (double)d[base + i]);
 }
 
-// dst[t] = src[t] for the listed rows
-__global__ void k_vec_copy_rows(const float* __restrict__ src, int64_t npix,
-                                const int* __restrict__ rows, float* __restrict__ dst) {
-  const size_t base = (size_t)row_of(rows, blockIdx.y) * npix;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
-       i += (int64_t)gridDim.x * blockDim.x)
-    dst[base + i] = src[base + i];
-}
-
 // Line-search trials: out[j] = x[rows[j]] + coef[j] d[rows[j]] (compact rows).
 __global__ void k_vec_axpy_gather(const float* __restrict__ x, const float* __restrict__ d,
                                   const double* __restrict__ coef, int64_t npix,
@@ -681,15 +667,17 @@ __global__ void k_vec_axpy_gather(const float* __restrict__ x, const float* __re
     out[dst + i] = (float)((double)x[src + i] + c * (double)d[src + i]);
 }
 
-// dst[drows[j]] = src[srows[j]] for j < count
-__global__ void k_rows_copy_map(const float* __restrict__ src, const int* __restrict__ srows,
-                                float* __restrict__ dst, const int* __restrict__ drows,
-                                int64_t npix) {
+// Accepted line-search steps, in place: x[rows[j]] += coef[j] d[rows[j]], in
+// FP64 and rounded exactly as the trial point was (k_ilt_mask_rows).
+__global__ void k_vec_axpy_rows(float* __restrict__ x, const float* __restrict__ d,
+                                const double* __restrict__ coef, int64_t npix,
+                                const int* __restrict__ rows) {
   const int j = blockIdx.y;
-  const size_t a = (size_t)srows[j] * npix, b = (size_t)drows[j] * npix;
+  const size_t r = (size_t)rows[j] * npix;
+  const double c = coef[j];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix;
        i += (int64_t)gridDim.x * blockDim.x)
-    dst[b + i] = src[a + i];
+    x[r + i] = (float)((double)x[r + i] + c * (double)d[r + i]);
 }
 
 // Row gather / scatter of per-target images: dst[j] = src[idx[j]] or
@@ -715,19 +703,14 @@ static int grid_for(int64_t total, int threads) {
   return (int)std::min<int64_t>(b, 1 << 16);
 }
 
-hipError_t launch_ilt_mask(const float* theta, float* m, float* t, int64_t total, float beta,
-                           float a, hipStream_t s) {
-  hipLaunchKernelGGL(k_ilt_mask, dim3(grid_for(total, 256)), dim3(256), 0, s, theta, m, t, total,
-                     beta, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_ilt_pointwise(const float* image, const float* target, int nimg, int ncond,
-                                int64_t npix, int bpi, int cost_kind, float threshold,
-                                float steepness, const float* dose, float* G, double* partial,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(k_ilt_pointwise, dim3(bpi, nimg), dim3(PW_THREADS), 0, s, image, target,
-                     ncond, npix, bpi, cost_kind, threshold, steepness, dose, G, partial);
+hipError_t launch_ilt_mask_rows(const float* theta, const int* rows, const float* dir,
+                                const double* coef, int64_t npix, int count, float beta, float ar,
+                                float ai, float* m, float* t, float2* tc, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (npix % 4 != 0 || (coef && !dir)) return hipErrorInvalidValue;
+  const int64_t bx = std::min<int64_t>((npix / 4 + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_ilt_mask_rows, dim3((unsigned)bx, (unsigned)count), dim3(256), 0, s, theta,
+                     rows, dir, coef, npix, beta, ar, ai, m, t, tc);
   return hipGetLastError();
 }
 
@@ -828,13 +811,6 @@ hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, i
   return hipGetLastError();
 }
 
-hipError_t launch_ilt_mask_c(const float* theta, float* m, float2* t, int64_t total, float beta,
-                             float ar, float ai, hipStream_t s) {
-  hipLaunchKernelGGL(k_ilt_mask_c, dim3(grid_for(total, 256)), dim3(256), 0, s, theta, m, t,
-                     total, beta, ar, ai);
-  return hipGetLastError();
-}
-
 hipError_t launch_ilt_filter_band(const float2* X, int Bf, float c, float inv_n2, int count,
                                   float2* Ib, hipStream_t s) {
   const int64_t total = (int64_t)count * (Bf + 1) * (2 * Bf + 1);
@@ -853,16 +829,26 @@ hipError_t launch_ilt_proxy(const float* G, int ntargets, int ncond, int64_t npi
   return hipGetLastError();
 }
 
-hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m, float beta,
-                            int64_t total, float* grad, hipStream_t s) {
-  hipLaunchKernelGGL(k_ilt_chain, dim3(grid_for(total, 256)), dim3(256), 0, s, dcdm, reg, m, beta,
-                     total, grad);
-  return hipGetLastError();
-}
-
 static dim3 rows_grid(int64_t npix, int count) {
   const int64_t bx = std::min<int64_t>((npix + 255) / 256, 1024);
   return dim3((unsigned)bx, (unsigned)count);
+}
+
+hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m, float beta,
+                            int64_t npix, int count, const int* orows, float* grad,
+                            hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ilt_chain, rows_grid(npix, count), dim3(256), 0, s, dcdm, reg, m, beta,
+                     npix, orows, grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_axpy_rows(float* x, const float* d, const double* coef, int64_t npix,
+                                int count, const int* rows, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vec_axpy_rows, rows_grid(npix, count), dim3(256), 0, s, x, d, coef, npix,
+                     rows);
+  return hipGetLastError();
 }
 
 hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int count,
@@ -902,22 +888,6 @@ hipError_t launch_vec_axpy_gather(const float* x, const float* d, const double* 
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_vec_axpy_gather, rows_grid(npix, count), dim3(256), 0, s, x, d, coef, npix,
                      rows, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_rows_copy_map(const float* src, const int* srows, float* dst, const int* drows,
-                                int64_t npix, int count, hipStream_t s) {
-  if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rows_copy_map, rows_grid(npix, count), dim3(256), 0, s, src, srows, dst,
-                     drows, npix);
-  return hipGetLastError();
-}
-
-hipError_t launch_vec_copy_rows(const float* src, int64_t npix, int count, const int* rows,
-                                float* dst, hipStream_t s) {
-  if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vec_copy_rows, rows_grid(npix, count), dim3(256), 0, s, src, npix, rows,
-                     dst);
   return hipGetLastError();
 }
 

@@ -7,12 +7,12 @@
 
 namespace huvk {
 
-hipError_t launch_ilt_mask(const float* theta, float* m, float* t, int64_t total, float beta,
-                           float a, hipStream_t s);
-hipError_t launch_ilt_pointwise(const float* image, const float* target, int nimg, int ncond,
-                                int64_t npix, int bpi, int cost_kind, float threshold,
-                                float steepness, const float* dose, float* G, double* partial,
-                                hipStream_t s);
+// m and t (real absorber ar, or tc for a complex one ar + i ai; each may be
+// null) of count evaluation rows read in place: row j's theta is
+// theta[rows[j]] + coef[j] dir[rows[j]] (rows null: j; coef null: theta only).
+hipError_t launch_ilt_mask_rows(const float* theta, const int* rows, const float* dir,
+                                const double* coef, int64_t npix, int count, float beta, float ar,
+                                float ai, float* m, float* t, float2* tc, hipStream_t s);
 hipError_t launch_ilt_reg(const float* m, int n, int ntargets, int bpi, float tv_w, float tv_eps,
                           float bin_w, float* reg_grad, double* partial, hipStream_t s);
 hipError_t launch_ilt_cost_reduce(const double* fid, const double* reg, int ntargets, int ncond,
@@ -39,10 +39,6 @@ hipError_t launch_ilt_adjoint_big(int m, int nimg, int KS, const float2* Mq, con
 hipError_t launch_ilt_hband(const float2* Rin, int B, int ntargets, int ncond, int KS,
                             const float* weight, float cr, float ci, float inv_n2, float2* H,
                             hipStream_t s);
-// m = sigma(beta theta) (m may be null) and t = a + (1 - a) m as interleaved
-// complex values (t may be null).
-hipError_t launch_ilt_mask_c(const float* theta, float* m, float2* t, int64_t total, float beta,
-                             float ar, float ai, hipStream_t s);
 // Density-filter band for band_image: Ib[t][gy + Bf][gx] = X[gx + Bf][gy + Bf]
 // exp(-c (gx^2 + gy^2)) inv_n2 for count boxes X of half-width Bf.
 hipError_t launch_ilt_filter_band(const float2* X, int Bf, float c, float inv_n2, int count,
@@ -51,8 +47,10 @@ hipError_t launch_ilt_filter_band(const float2* X, int Bf, float c, float inv_n2
 hipError_t launch_ilt_proxy(const float* G, int ntargets, int ncond, int64_t npix, int bpi,
                             const float* weight, float flare, double* part, float* dcdm,
                             hipStream_t s);
+// dC/dtheta of count evaluation rows into grad rows orows[j] (null: j).
 hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m, float beta,
-                            int64_t total, float* grad, hipStream_t s);
+                            int64_t npix, int count, const int* orows, float* grad,
+                            hipStream_t s);
 // Per-target vector operations on `count` target rows listed in `rows`
 // (device; null = rows 0..count-1); other rows are left untouched.
 // mode 0: per-target sum x.y ; mode 1: per-target max |x|.  part: ntargets x nblk,
@@ -71,11 +69,8 @@ hipError_t launch_rows_scatter(const float* src, const int* idx, int64_t npix, i
 // out[j] = x[rows[j]] + coef[j] d[rows[j]], j < count (compact line-search trials)
 hipError_t launch_vec_axpy_gather(const float* x, const float* d, const double* coef, int64_t npix,
                                   int count, const int* rows, float* out, hipStream_t s);
-// dst[drows[j]] = src[srows[j]], j < count
-hipError_t launch_rows_copy_map(const float* src, const int* srows, float* dst, const int* drows,
-                                int64_t npix, int count, hipStream_t s);
-// dst[t] = src[t] for the listed rows
-hipError_t launch_vec_copy_rows(const float* src, int64_t npix, int count, const int* rows,
-                                float* dst, hipStream_t s);
+// x[rows[j]] += coef[j] d[rows[j]] in place (accepted line-search steps)
+hipError_t launch_vec_axpy_rows(float* x, const float* d, const double* coef, int64_t npix,
+                                int count, const int* rows, hipStream_t s);
 
 }  // namespace huvk

@@ -613,6 +613,117 @@ __global__ __launch_bounds__(TEAMS * CoarseFft<M>::N2) void k_coarse_cols(
       });
 }
 
+// ---------------------------------------------------------------------------
+// coarse_fused: coarse_rows and coarse_cols of one image in one work-group,
+// the row spectra kept in LDS ((M/2) x (M + 1) complex; M = 64, 128, 192).
+// Transforms M = N1 x N2 run in place in the LDS rows, teams of N2 threads:
+//  row pass, pair y2 (z = row 2 y2 + i row 2 y2 + 1, x = Q j + q read from
+//    the class-major image): thread n1 < N1 takes x = n1 + N1 n2, FFT_N2,
+//    twiddle W_M^{n1 k2}, to slot n1 + N1 k2; after a barrier thread k2
+//    takes slots n1 + N1 k2 (n1 < N1), FFT_N1, back to the same slots: the
+//    pair's frequency k = k2 + N2 k1 sits at slot k1 + N1 k2.
+//  column pass, column gx in [0, G] (G = 2B): c[2 y2] = A, c[2 y2 + 1] = B
+//    separated from Z[gx] and Z[M - gx] of pair y2, FFT_M over y in the same
+//    steps with the column's two slots per pair as the exchange area (slots
+//    of frequency M/2, never kept since G < M/2, for gx = 0); gy in [-G, G]
+//    kept: Ib[img][gy + G][gx] = v / M^2, times (1 - flare) except at the
+//    origin.  Every read of a column's slots precedes the barrier after
+//    which they are rewritten; columns never share slots.
+// ---------------------------------------------------------------------------
+template <int M, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_coarse_fused(const float* __restrict__ coarse, int B,
+                                                          float flare,
+                                                          const float2* __restrict__ tw_m,
+                                                          float2* __restrict__ Ib) {
+  constexpr int N1 = CoarseFft<M>::N1, N2 = CoarseFft<M>::N2, Q = M / 32;
+  constexpr int NP = M / 2, PS = M + 1, TEAMS = THREADS / N2;
+  extern __shared__ c32 zs[];  // [NP][PS]
+  const int img = blockIdx.x;
+  const int team = threadIdx.x / N2, t = threadIdx.x % N2;
+  const int G = 2 * B;
+  auto slot = [](int k) { return k / N2 + N1 * (k % N2); };
+  const float* base = coarse + (size_t)img * M * M;
+  for (int r0 = 0; r0 < NP; r0 += TEAMS) {
+    const int y2 = r0 + team;
+    const bool act = y2 < NP;
+    c32* row = zs + (act ? y2 : 0) * PS;
+    if (act && t < N1) {
+      const float* ra = base + (size_t)(2 * y2) * M;
+      const float* rb = ra + M;
+      c32 v[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) {
+        const int x = t + N1 * n2;
+        const int at = (x % Q) * 32 + x / Q;  // class-major position of x
+        v[n2] = c32{ra[at], rb[at]};
+      }
+      fft_small<-1, N2>(v);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        if (k2 > 0) v[k2] = cmul(v[k2], tw_lookup<-1>(tw_m, (t * k2) % M));
+        row[t + N1 * k2] = v[k2];
+      }
+    }
+    __syncthreads();
+    if (act) {
+      c32 u[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) u[n1] = row[n1 + N1 * t];
+      fft_small<-1, N1>(u);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) row[k1 + N1 * t] = u[k1];
+    }
+    __syncthreads();
+  }
+  const float inv = 1.0f / ((float)M * (float)M);
+  const int nk = 2 * G + 1;
+  float2* out = Ib + (size_t)img * nk * (G + 1);
+  for (int c0 = 0; c0 <= G; c0 += TEAMS) {
+    const int gx = c0 + team;
+    const bool act = gx <= G;
+    const int gxa = act ? gx : 0;
+    const int sp = slot(gxa), sm = slot((M - gxa) % M), sw = gxa == 0 ? slot(M / 2) : sm;
+    // exchange-area slot of column value y
+    auto cs = [&](int y) -> c32* { return zs + (y >> 1) * PS + ((y & 1) ? sw : sp); };
+    c32 v[N2];
+    if (act && t < N1) {
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) {
+        const int y = t + N1 * n2;
+        const c32 zp = zs[(y >> 1) * PS + sp], zn = zs[(y >> 1) * PS + sm];
+        // A = (Z[k] + conj Z[M-k]) / 2 ; B = (Z[k] - conj Z[M-k]) / (2i)
+        v[n2] = (y & 1) == 0 ? c32{0.5f * (zp.re + zn.re), 0.5f * (zp.im - zn.im)}
+                             : c32{0.5f * (zp.im + zn.im), -0.5f * (zp.re - zn.re)};
+      }
+      fft_small<-1, N2>(v);
+#pragma unroll
+      for (int k2 = 1; k2 < N2; ++k2) v[k2] = cmul(v[k2], tw_lookup<-1>(tw_m, (t * k2) % M));
+    }
+    __syncthreads();
+    if (act && t < N1) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) *cs(t + N1 * k2) = v[k2];
+    }
+    __syncthreads();
+    if (act) {
+      c32 u[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) u[n1] = *cs(n1 + N1 * t);
+      fft_small<-1, N1>(u);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        const int k = t + N2 * k1;
+        int gy;
+        if (k <= G) gy = k;
+        else if (k >= M - G) gy = k - M;
+        else continue;
+        const float s = (gx == 0 && gy == 0) ? inv : inv * (1.0f - flare);
+        out[(size_t)(gy + G) * (G + 1) + gx] = make_float2(u[k1].re * s, u[k1].im * s);
+      }
+    }
+  }
+}
+
 // v_permlane32_swap on two floats: afterwards the upper 32 lanes of a hold
 // the lower lanes of b and the lower 32 lanes of b the upper lanes of a (the
 // other halves keep their values).  The results are taken out of the
@@ -656,10 +767,67 @@ __device__ __forceinline__ void tw_half_if(c32* u, bool on) {
 // so phase 2 needs no band test at all.
 // Work-groups of one image are dealt to one XCD so the band stays in its L2.
 // ---------------------------------------------------------------------------
-template <int TMIN, int TMAX>
+// Fidelity of one pixel (ilt.rs:564-605): the residual of the dosed image
+// against the target, sigmoid-resist (KIND 1) or plain (0); returns r^2 and
+// sets g = dC/dI (unweighted by the condition weight).
+template <int KIND>
+__device__ __forceinline__ double fidelity_term(const FidelityEpi& e, float dose, float v, float z,
+                                                float& g) {
+  const float di = dose * v;
+  float r;
+  if constexpr (KIND == 0) {
+    r = di - z;
+    g = 2.0f * r * dose * e.inv_n;
+  } else {
+    const float rr = 1.0f / (1.0f + expf(-e.steepness * (di - e.threshold)));
+    r = rr - z;
+    g = 2.0f * r * e.steepness * rr * (1.0f - rr) * dose * e.inv_n;
+  }
+  return (double)r * (double)r;
+}
+
+// The fidelity of one band_image task's 64 pixels (rows a and b at byte
+// offsets va, vb, column offsets C i): target values read 8 columns at a
+// time, G written when WITH_G; returns the task's sum of r^2.
+template <int KIND, bool WITH_G>
+__device__ __forceinline__ double fidelity_task(const FidelityEpi& e, float dose, const c32* d,
+                                                int C, int va, int vb,
+                                                __amdgpu_buffer_rsrc_t zrc,
+                                                __amdgpu_buffer_rsrc_t grc) {
+  double acc = 0.0;
+#pragma unroll
+  for (int i0 = 0; i0 < 32; i0 += 8) {
+    __builtin_amdgcn_sched_barrier(0);
+    float za[8], zb[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int so = C * (i0 + k) * (int)sizeof(float);
+      za[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(zrc, va, so, 0));
+      zb[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(zrc, vb, so, 0));
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      float ga, gb;
+      acc += fidelity_term<KIND>(e, dose, d[i0 + k].re, za[k], ga);
+      acc += fidelity_term<KIND>(e, dose, d[i0 + k].im, zb[k], gb);
+      if constexpr (WITH_G) {
+        const int so = C * (i0 + k) * (int)sizeof(float);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, ga), grc, va, so, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, gb), grc, vb, so, 0);
+      }
+    }
+  }
+  return acc;
+}
+
+// FID: instead of the image, the ILT fidelity of every pixel against the
+// target (epi): per-work-group FP64 partial sums of r^2 at
+// epi.partial[img][p] and, when epi.G is set, G = dC/dI written where the
+// image would be; image img is (target img / ncond, condition img % ncond).
+template <int TMIN, int TMAX, bool FID>
 __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ Ib, int n, int G,
                                                     int nimg, const float2* __restrict__ tw_n,
-                                                    float* __restrict__ out) {
+                                                    float* __restrict__ out, FidelityEpi epi) {
   constexpr int F = 32, NT = TMAX - TMIN + 1;
   extern __shared__ c32 smem[];
   constexpr int UP = 32 * (-TMIN) + 1;  // U row: g = 0 .. 32 |TMIN|, zero beyond G
@@ -778,8 +946,16 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     __syncthreads();
   }
   // ---- phase 2 ----
+  float* dst = FID ? epi.G : out;
   const __amdgpu_buffer_rsrc_t orc = __builtin_amdgcn_make_buffer_rsrc(
-      out + (size_t)img * n * n, (short)0, n * n * (int)sizeof(float), 0x00020000);
+      dst ? dst + (size_t)img * n * n : nullptr, (short)0, n * n * (int)sizeof(float), 0x00020000);
+  const int tgt = FID ? img / epi.ncond : 0;
+  const float dose = FID ? epi.dose[img % epi.ncond] : 1.0f;
+  const __amdgpu_buffer_rsrc_t zrc = __builtin_amdgcn_make_buffer_rsrc(
+      FID ? const_cast<float*>(epi.target) + (size_t)(epi.trows ? epi.trows[tgt] : tgt) * n * n
+          : nullptr,
+      (short)0, n * n * (int)sizeof(float), 0x00020000);
+  double fid_acc = 0.0;
   // Lanes l and l + 32 of a wave share a task (row pair pr, classes c and
   // c + C/2, c < C/2): e^{2 pi i t (c + C/2) / C} = (-1)^t e^{2 pi i t c / C},
   // so the even-t and odd-t partial folds E, O give d_c = E + O and
@@ -833,12 +1009,34 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     // inputs from L2 and the MALL.
     const int va = ((P * (2 * pr) + p) * n + c) * (int)sizeof(float);
     const int vb = va + P * n * (int)sizeof(float);
+    if constexpr (!FID) {
 #pragma unroll
-    for (int i = 0; i < F; ++i) {
-      const int so = C * i * (int)sizeof(float);
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].re), orc, va, so, 2);
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].im), orc, vb, so, 2);
+      for (int i = 0; i < F; ++i) {
+        const int so = C * i * (int)sizeof(float);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].re), orc, va, so, 2);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, d[i].im), orc, vb, so, 2);
+      }
+    } else {
+      // the fidelity term of each pixel instead of the pixel (G = dC/dI in
+      // place of the image when asked for); one branch per task on the cost
+      // kind and on G, none per pixel
+      if (epi.cost_kind == 0) {
+        if (epi.G) fid_acc += fidelity_task<0, true>(epi, dose, d, C, va, vb, zrc, orc);
+        else fid_acc += fidelity_task<0, false>(epi, dose, d, C, va, vb, zrc, orc);
+      } else {
+        if (epi.G) fid_acc += fidelity_task<1, true>(epi, dose, d, C, va, vb, zrc, orc);
+        else fid_acc += fidelity_task<1, false>(epi, dose, d, C, va, vb, zrc, orc);
+      }
     }
+  }
+  if constexpr (FID) {
+    // work-group sum in a fixed order: lanes by a butterfly, then the waves
+    __shared__ double wsum[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) fid_acc += __shfl_xor(fid_acc, off);
+    if (lane == 0) wsum[tid >> 6] = fid_acc;
+    __syncthreads();
+    if (tid == 0) epi.partial[(size_t)img * P + p] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
   }
 }
 
@@ -1054,6 +1252,38 @@ hipError_t launch_coarse_rows(int m, int B, int nimg, int S, const float* coarse
   }
 }
 
+template <int M, int THREADS>
+static hipError_t coarse_fused_m(int B, int nimg, int S, const float* coarse, float flare,
+                                 const float2* tw_m, float2* Ib, hipStream_t s) {
+  if (S > 1) {
+    const int64_t count4 = (int64_t)nimg * M * M / 4;
+    const int blocks = (int)std::min<int64_t>((count4 + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_sum_splits, dim3(blocks), dim3(256), 0, s, const_cast<float*>(coarse), S,
+                       count4);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  constexpr size_t shmem = (size_t)(M / 2) * (M + 1) * sizeof(c32);
+  auto kern = k_coarse_fused<M, THREADS>;
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  if (attr != hipSuccess) return attr;
+  HUV_LAUNCH(kern, dim3(nimg), dim3(THREADS), shmem, s, coarse, B, flare, tw_m, Ib);
+}
+
+hipError_t launch_coarse_fused(int m, int B, int nimg, int S, const float* coarse, float flare,
+                               const float2* tw_m, float2* Ib, hipStream_t s) {
+  if (S < 1 || 4 * B + 1 > m) return hipErrorInvalidValue;
+  switch (m) {
+    case 64: return coarse_fused_m<64, 512>(B, nimg, S, coarse, flare, tw_m, Ib, s);
+    case 128: return coarse_fused_m<128, 512>(B, nimg, S, coarse, flare, tw_m, Ib, s);
+    case 192: return coarse_fused_m<192, 512>(B, nimg, S, coarse, flare, tw_m, Ib, s);
+    default: return hipErrorNotSupported;
+  }
+}
+
+bool coarse_fused_supported(int m) { return m == 64 || m == 128 || m == 192; }
+
 template <int M>
 static hipError_t coarse_cols_m(int B, int nimg, float flare, const float2* tmp1,
                                 const float2* tw_m, float2* Ib, hipStream_t s) {
@@ -1074,31 +1304,44 @@ hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2*
   }
 }
 
-template <int TMIN, int TMAX>
+template <int TMIN, int TMAX, bool FID>
 static hipError_t band_image_t(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
-                               float* out, hipStream_t s) {
+                               float* out, const FidelityEpi& epi, hipStream_t s) {
   constexpr int UP = 32 * (-TMIN) + 1, NT = TMAX - TMIN + 1;
   auto bytes = [](int nn) { return ((size_t)32 * UP + (size_t)(32 + NT) * (nn / 32)) * sizeof(c32); };
   const size_t shmem = bytes(n);
-  auto kern = k_band_image<TMIN, TMAX>;
+  auto kern = k_band_image<TMIN, TMAX, FID>;
   // Sized for the largest grid (n = 2048), once.
   static const hipError_t attr = hipFuncSetAttribute(
       (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes(2048));
   if (attr != hipSuccess) return attr;
   const int groups = (nimg + kXcds - 1) / kXcds;
   HUV_LAUNCH(kern, dim3(groups * kXcds * (n / 32)), dim3(256), shmem, s, Ib, n, G, nimg, tw_n,
-             out);
+             out, epi);
+}
+
+template <bool FID>
+static hipError_t band_image_g(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
+                               float* out, const FidelityEpi& epi, hipStream_t s) {
+  if (n < 256 || n > 2048 || (n & (n - 1)) != 0 || G < 0 || G > 255 || 2 * G + 1 > n)
+    return hipErrorInvalidValue;
+  if (G <= 31) return band_image_t<-1, 0, FID>(n, G, nimg, Ib, tw_n, out, epi, s);
+  if (G <= 63) return band_image_t<-2, 1, FID>(n, G, nimg, Ib, tw_n, out, epi, s);
+  if (G <= 95) return band_image_t<-3, 2, FID>(n, G, nimg, Ib, tw_n, out, epi, s);
+  if (G <= 127) return band_image_t<-4, 3, FID>(n, G, nimg, Ib, tw_n, out, epi, s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_band_image(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
                              float* out, hipStream_t s) {
-  if (n < 256 || n > 2048 || (n & (n - 1)) != 0 || G < 0 || G > 255 || 2 * G + 1 > n)
+  return band_image_g<false>(n, G, nimg, Ib, tw_n, out, FidelityEpi{}, s);
+}
+
+hipError_t launch_band_image_fidelity(int n, int G, int nimg, const float2* Ib,
+                                      const float2* tw_n, const FidelityEpi& epi, hipStream_t s) {
+  if (!epi.target || !epi.dose || !epi.partial || epi.ncond < 1 || nimg % epi.ncond != 0)
     return hipErrorInvalidValue;
-  if (G <= 31) return band_image_t<-1, 0>(n, G, nimg, Ib, tw_n, out, s);
-  if (G <= 63) return band_image_t<-2, 1>(n, G, nimg, Ib, tw_n, out, s);
-  if (G <= 95) return band_image_t<-3, 2>(n, G, nimg, Ib, tw_n, out, s);
-  if (G <= 127) return band_image_t<-4, 3>(n, G, nimg, Ib, tw_n, out, s);
-  return hipErrorInvalidValue;
+  return band_image_g<true>(n, G, nimg, Ib, tw_n, nullptr, epi, s);
 }
 
 }  // namespace huvk

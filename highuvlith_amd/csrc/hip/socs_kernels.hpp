@@ -61,8 +61,32 @@ hipError_t launch_coarse_rows(int m, int B, int nimg, int S, const float* coarse
                               const float2* tw_m, float2* tmp1, hipStream_t s);
 hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2* tmp1,
                               const float2* tw_m, float2* Ib, hipStream_t s);
+// coarse_rows + coarse_cols of each image in one work-group (row spectra in
+// LDS), for m = 64, 128, 192; the same Ib as the two launches.
+bool coarse_fused_supported(int m);
+hipError_t launch_coarse_fused(int m, int B, int nimg, int S, const float* coarse, float flare,
+                               const float2* tw_m, float2* Ib, hipStream_t s);
 // Ib: [nimg][2G+1][G+1] Hermitian half band (gy + G, gx).
 hipError_t launch_band_image(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
                              float* out, hipStream_t s);
+
+// The ILT fidelity term evaluated where band_image would store the image
+// (images img = target * ncond + condition): r = sigma(steepness (dose I -
+// threshold)) - Z (cost_kind 1) or dose I - Z (0), Z = target row
+// trows[target] (trows null: target); partial[img][p] = sum of r^2 over the
+// image's row class p (n / 32 per image, FP64, fixed order); G (optional) =
+// 2 r dr/dI inv_n per pixel.
+struct FidelityEpi {
+  const float* target = nullptr;
+  const int* trows = nullptr;
+  int ncond = 1;
+  int cost_kind = 0;
+  float threshold = 0.f, steepness = 0.f, inv_n = 0.f;
+  const float* dose = nullptr;
+  float* G = nullptr;
+  double* partial = nullptr;
+};
+hipError_t launch_band_image_fidelity(int n, int G, int nimg, const float2* Ib,
+                                      const float2* tw_n, const FidelityEpi& epi, hipStream_t s);
 
 }  // namespace huvk

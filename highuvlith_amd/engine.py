@@ -338,7 +338,7 @@ class DeviceContext:
 
     STAGES = ("rows_band", "cols_band", "socs_fields", "coarse_rows", "coarse_cols",
               "band_image", "ilt_pointwise", "ilt_adjoint", "ilt_other", "rect_tables",
-              "rect_band", "poly_band")
+              "rect_band", "poly_band", "coarse")
 
     def set_profiling(self, on=True):
         check(self.lib.huv_ctx_set_profiling(self.handle, int(bool(on))))

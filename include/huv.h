@@ -150,8 +150,10 @@ huv_status huv_ctx_synchronize(huv_ctx* ctx);
 /* Per-stage timing with HIP events recorded on the active stream around each
  * kernel launch (off by default).  Stages: 0 rows_band, 1 cols_band,
  * 2 socs_fields, 3 coarse_rows, 4 coarse_cols, 5 band_image, 6 ilt_pointwise,
- * 7 ilt_adjoint, 8 ilt_other, 9 rect_tables, 10 rect_band, 11 poly_band (the
- * last three: feature-list masks).  huv_ctx_stage_times synchronises, returns the
+ * 7 ilt_adjoint, 8 ilt_other, 9 rect_tables, 10 rect_band, 11 poly_band (9-11:
+ * feature-list masks), 12 coarse (coarse_rows and coarse_cols fused in one
+ * launch where the coarse grid allows it; 3 and 4 are then not used).
+ * huv_ctx_stage_times synchronises, returns the
  * accumulated milliseconds and launch counts, and optionally resets them. */
 huv_status huv_ctx_set_profiling(huv_ctx* ctx, int on);
 huv_status huv_ctx_stage_times(huv_ctx* ctx, double* ms, int64_t* calls, int nstages, int reset);
