@@ -488,14 +488,31 @@ def bench_ilt(args, H, W, np, torch):
     L.optimize_ilt(eng, targets[:2], L.ILTConfig(cost="resist", regularization=0.0,
                                                  max_iterations=2))  # warm-up
     res = L.optimize_ilt(eng, targets, cfg)
-    return {"workload": f"{len(targets)} contact-array targets 512x512 @ 2 nm, ArF 193i quasar, "
-                        f"K={K}, resist cost, {args.ilt_iterations} CG/Armijo iterations",
-            "value": round(len(targets) / res.wall_s, 3), "unit": "clips/s",
-            "wall_s": round(res.wall_s, 3),
-            "mean_iterations": float(np.mean(res.iterations)),
-            "mean_initial_cost": float(np.mean([h[0] for h in res.cost_history])),
-            "mean_final_cost": float(np.mean(res.final_cost)),
-            }
+    out = {"workload": f"{len(targets)} contact-array targets 512x512 @ 2 nm, ArF 193i quasar, "
+                       f"K={K}, resist cost, {args.ilt_iterations} CG/Armijo iterations",
+           "value": round(len(targets) / res.wall_s, 3), "unit": "clips/s",
+           "wall_s": round(res.wall_s, 3),
+           "ms_per_iteration": round(res.wall_s / args.ilt_iterations * 1e3, 3),
+           "mean_iterations": float(np.mean(res.iterations)),
+           "mean_initial_cost": float(np.mean([h[0] for h in res.cost_history])),
+           "mean_final_cost": float(np.mean(res.final_cost)),
+           }
+    if not args.no_cpu:
+        # the oracle's run_stage (FP64, OpenMP over kernels) on the first target,
+        # 3 iterations from the same start
+        import oracle
+        conds = [(c.dose, c.weight, eng.kernels(c.defocus_nm)) for c in cfg.conditions]
+        orc = oracle.IltProblem(grid.size, targets[0], conds, eng.flare_fraction(), cost=cfg.cost,
+                                threshold=cfg.threshold, steepness=cfg.steepness,
+                                beta=cfg.sigmoid_steepness, tv_weight=cfg.regularization,
+                                tv_eps=cfg.tv_epsilon, bin_weight=cfg.binarization_weight)
+        t0 = time.perf_counter()
+        _, hist, _ = orc.run_stage(targets[0] - 0.5, 3)
+        per_it = (time.perf_counter() - t0) / max(len(hist) - 1, 1)
+        out["cpu_oracle_s_per_target_iteration"] = round(per_it, 4)
+        out["cpu_oracle_clips_per_s"] = round(1.0 / (per_it * args.ilt_iterations), 4)
+        out["cpu_oracle_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return out
 
 
 def bench_single(args, H, W, np, torch):

@@ -71,7 +71,7 @@ huv_status ilt_buffers(Ctx* c, const huv_ilt* p, int T, IltBuffers& b) {
   b.m = (float*)a;
   b.t = b.m + (size_t)T * N;
   b.tc = p->complex_abs ? (float2*)b.t : nullptr;
-  b.G =(float*)c->ws.get(Workspace::ILT2, TC * N * sizeof(float), e);
+  b.G = (float*)c->ws.get(Workspace::ILT2, TC * N * sizeof(float), e);
   if (!b.G) return hip_fail(e, "ILT workspace");
   a = (char*)c->ws.get(Workspace::ILT3, 2 * (size_t)T * N * sizeof(float), e);
   if (!a) return hip_fail(e, "ILT workspace");
@@ -472,20 +472,19 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     if ((int)rows.size() == T || rows.empty()) return nullptr;
     return (const int*)put(rows.data(), rows.size() * sizeof(int));
   };
-  // Up to kRed per-target reductions over `rows`, one synchronisation;
+  // Up to kRed reductions of two vectors a, b over `rows` in one pass (op codes of
+  // launch_vec_reduce: 0 a.a, 1 a.b, 2 b.b, 3 max |a|), one synchronisation;
   // h_red[r * T + t] holds reduction r for each listed t.
-  struct Red { const float* x; const float* y; int mode; };
-  auto reduce = [&](std::initializer_list<Red> ops, const std::vector<int>& rows) -> bool {
+  enum { AA = 0, AB = 1, BB = 2, MAXA = 3 };
+  auto reduce = [&](const float* a, const float* bv, std::initializer_list<int> ops,
+                    const std::vector<int>& rows) -> bool {
     if (rows.empty()) return true;
     const int* rd = stage_rows(rows);
     if (e != hipSuccess || !flush()) return false;
-    int r = 0;
-    for (const Red& op : ops) {
-      e = huvk::launch_vec_reduce(op.x, op.y, N, (int)rows.size(), rd, nblk, op.mode,
-                                  part + (size_t)r * T * nblk, red + (size_t)r * T, s);
-      if (e != hipSuccess) return false;
-      ++r;
-    }
+    int code = 0, r = 0;
+    for (int op : ops) code |= op << (4 * r++);
+    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, part, red, s);
+    if (e != hipSuccess) return false;
     e = hipMemcpyAsync(h_red.data(), red, (size_t)r * T * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return false;
     e = hipStreamSynchronize(s);
@@ -574,7 +573,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       term[t] = 3;
     }
   }
-  if (!reduce({{grad, grad, 0}}, all_t)) return fail_e("dot");
+  if (!reduce(grad, nullptr, {AA}, all_t)) return fail_e("dot");
   for (int t = 0; t < T; ++t) {
     gnorm[t] = std::sqrt(h_red[t]);
     if (tr.gnorm_hist) tr.gnorm_hist[(size_t)t * H] = gnorm[t];
@@ -603,7 +602,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       lst.clear();
       for (int t = 0; t < T; ++t)
         if (active[t] && !accepted[t] && attempt[t] == att) lst.push_back(t);
-      if (!reduce({{dir, nullptr, 1}, {grad, dir, 0}}, lst)) return fail_e("maxabs / dot");
+      if (!reduce(dir, grad, {MAXA, AB}, lst)) return fail_e("maxabs / dot");
       rst.clear();
       for (int t : lst) {
         dmax[t] = h_red[t];
@@ -731,7 +730,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     st = eval_sel(theta, gnew, cnew, which);
     if (st != HUV_OK) return st;
     // CG update: beta = max(0, (g'.g' - g'.g) / g.g)
-    if (!reduce({{grad, grad, 0}, {gnew, grad, 0}, {gnew, gnew, 0}}, which)) return fail_e("dot");
+    if (!reduce(grad, gnew, {AA, AB, BB}, which)) return fail_e("dot");
     const double* gg = h_red.data();
     const double* gng = gg + T;
     const double* gngn = gng + T;

@@ -606,41 +606,75 @@ __device__ __forceinline__ int row_of(const int* __restrict__ rows, int j) {
   return rows ? rows[j] : j;
 }
 
-__global__ __launch_bounds__(PW_THREADS) void k_vec_reduce(const float* __restrict__ x,
-                                                           const float* __restrict__ y,
-                                                           int64_t npix, int nblk, int mode,
-                                                           const int* __restrict__ rows,
-                                                           double* __restrict__ out) {
-  __shared__ double sh[PW_THREADS];
-  const int t = row_of(rows, blockIdx.y);
-  const float* xx = x + (size_t)t * npix;
-  const float* yy = y ? y + (size_t)t * npix : nullptr;
-  const int64_t chunk = (npix + nblk - 1) / nblk;
-  const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
-  double acc = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += PW_THREADS) {
-    if (mode == 0) acc += (double)xx[i] * (double)yy[i];
-    else acc = fmax(acc, (double)fabsf(xx[i]));
+// Up to three per-target reductions over two vectors a, b in one pass; op r
+// (4 bits of ops each, r < nops): 0 a.a, 1 a.b, 2 b.b, 3 max |a|.  Each
+// reduction keeps the fixed order of a single one: per-thread FP64 sums over
+// the block's strided pixels, then a tree over the block.
+// part[(r * T + t) * nblk + block].
+__device__ __forceinline__ double reduce_term(int op, float av, float bv) {
+  switch (op) {
+    case 0: return (double)av * (double)av;
+    case 1: return (double)av * (double)bv;
+    case 2: return (double)bv * (double)bv;
+    default: return (double)fabsf(av);
   }
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = PW_THREADS / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      sh[threadIdx.x] = mode == 0 ? sh[threadIdx.x] + sh[threadIdx.x + s]
-                                  : fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[(size_t)t * nblk + blockIdx.x] = sh[0];
 }
 
-__global__ void k_vec_finish(const double* __restrict__ part, int count, int nblk, int mode,
-                             const int* __restrict__ rows, double* __restrict__ out) {
+template <int NOPS>
+__global__ __launch_bounds__(PW_THREADS) void k_vec_reduce(const float* __restrict__ a,
+                                                           const float* __restrict__ b,
+                                                           int64_t npix, int nblk, int ops, int T,
+                                                           const int* __restrict__ rows,
+                                                           double* __restrict__ part) {
+  __shared__ double sh[NOPS][PW_THREADS];
+  const int t = row_of(rows, blockIdx.y);
+  const float* aa = a + (size_t)t * npix;
+  const float* bb = b ? b + (size_t)t * npix : nullptr;
+  const int64_t chunk = (npix + nblk - 1) / nblk;
+  const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
+  int op[NOPS];
+#pragma unroll
+  for (int r = 0; r < NOPS; ++r) op[r] = (ops >> (4 * r)) & 15;
+  double acc[NOPS];
+#pragma unroll
+  for (int r = 0; r < NOPS; ++r) acc[r] = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += PW_THREADS) {
+    const float av = aa[i], bv = bb ? bb[i] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < NOPS; ++r) {
+      const double v = reduce_term(op[r], av, bv);
+      acc[r] = op[r] == 3 ? fmax(acc[r], v) : acc[r] + v;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NOPS; ++r) sh[r][threadIdx.x] = acc[r];
+  __syncthreads();
+  for (int s = PW_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int r = 0; r < NOPS; ++r)
+        sh[r][threadIdx.x] = op[r] == 3 ? fmax(sh[r][threadIdx.x], sh[r][threadIdx.x + s])
+                                        : sh[r][threadIdx.x] + sh[r][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < NOPS)
+    part[((size_t)threadIdx.x * T + t) * nblk + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// out[r * T + t] = sum (or max) over the blocks of part, for the listed rows.
+__global__ void k_vec_finish(const double* __restrict__ part, int count, int nblk, int ops,
+                             int nops, int T, const int* __restrict__ rows,
+                             double* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= count) return;
+  const int r = blockIdx.y;
+  if (j >= count || r >= nops) return;
   const int t = row_of(rows, j);
-  double a = 0.0;
-  for (int b = 0; b < nblk; ++b) a = mode == 0 ? a + part[(size_t)t * nblk + b] : fmax(a, part[(size_t)t * nblk + b]);
-  out[t] = a;
+  const bool mx = ((ops >> (4 * r)) & 15) == 3;
+  const double* p = part + ((size_t)r * T + t) * nblk;
+  double acc = 0.0;
+  for (int k = 0; k < nblk; ++k) acc = mx ? fmax(acc, p[k]) : acc + p[k];
+  out[(size_t)r * T + t] = acc;
 }
 
 // d = a[t] * g + b[t] * d
@@ -851,14 +885,24 @@ hipError_t launch_vec_axpy_rows(float* x, const float* d, const double* coef, in
   return hipGetLastError();
 }
 
-hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int count,
-                             const int* rows, int nblk, int mode, double* part, double* out,
-                             hipStream_t s) {
+hipError_t launch_vec_reduce(const float* a, const float* b, int64_t npix, int count,
+                             const int* rows, int nblk, int ops, int nops, int T, double* part,
+                             double* out, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vec_reduce, dim3(nblk, count), dim3(PW_THREADS), 0, s, x, y, npix, nblk,
-                     mode, rows, part);
-  hipLaunchKernelGGL(k_vec_finish, dim3((count + 63) / 64), dim3(64), 0, s, part, count, nblk,
-                     mode, rows, out);
+  if (nops < 1 || nops > 3 || count > T) return hipErrorInvalidValue;
+  for (int r = 0; r < nops; ++r) {
+    const int op = (ops >> (4 * r)) & 15;
+    if (op > 3 || ((op == 1 || op == 2) && !b)) return hipErrorInvalidValue;
+  }
+  const dim3 grid(nblk, count);
+  if (nops == 1)
+    hipLaunchKernelGGL(k_vec_reduce<1>, grid, dim3(PW_THREADS), 0, s, a, b, npix, nblk, ops, T, rows, part);
+  else if (nops == 2)
+    hipLaunchKernelGGL(k_vec_reduce<2>, grid, dim3(PW_THREADS), 0, s, a, b, npix, nblk, ops, T, rows, part);
+  else
+    hipLaunchKernelGGL(k_vec_reduce<3>, grid, dim3(PW_THREADS), 0, s, a, b, npix, nblk, ops, T, rows, part);
+  hipLaunchKernelGGL(k_vec_finish, dim3((count + 63) / 64, nops), dim3(64), 0, s, part, count,
+                     nblk, ops, nops, T, rows, out);
   return hipGetLastError();
 }
 

@@ -53,11 +53,13 @@ hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m,
                             hipStream_t s);
 // Per-target vector operations on `count` target rows listed in `rows`
 // (device; null = rows 0..count-1); other rows are left untouched.
-// mode 0: per-target sum x.y ; mode 1: per-target max |x|.  part: ntargets x nblk,
-// out[t] for the listed t.
-hipError_t launch_vec_reduce(const float* x, const float* y, int64_t npix, int count,
-                             const int* rows, int nblk, int mode, double* part, double* out,
-                             hipStream_t s);
+// Up to nops <= 3 per-target reductions of a and b in one pass, op r in bits
+// 4r..4r+3 of ops: 0 a.a, 1 a.b, 2 b.b, 3 max |a| (b may be null for 0 and
+// 3).  part: nops x T x nblk; out[r * T + t] for the listed t (T: the stride,
+// at least count).
+hipError_t launch_vec_reduce(const float* a, const float* b, int64_t npix, int count,
+                             const int* rows, int nblk, int ops, int nops, int T, double* part,
+                             double* out, hipStream_t s);
 // d = a[t] g + b[t] d
 hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int64_t npix,
                           int count, const int* rows, float* d, hipStream_t s);

@@ -234,39 +234,55 @@ __global__ void k_conv_weight_rot(const bf16_t* __restrict__ Wt, int Cout, int C
   }
 }
 
+// Eight BF16 values as one 16-byte word.
+union B8 {
+  uint4 u;
+  bf16_t h[8];
+};
+
 // ---------------------------------------------------------------------------
-// NHWC <-> NCHW through a 32 x 32 LDS tile (pixels x channels).
+// NHWC -> NCHW through a 64-pixel x 32-channel LDS tile (HW % 64 == 0, C % 8
+// == 0): each thread reads one 16-byte run of 8 channels of a pixel and
+// writes one 16-byte run of 8 pixels of a channel.  MASK: the ReLU mask of
+// the backward pass, dY zeroed (in place, and in the copy) where Y <= 0.
 // ---------------------------------------------------------------------------
 template <bool MASK>
 __global__ __launch_bounds__(256) void k_transpose_t(bf16_t* __restrict__ dY,
                                                      const bf16_t* __restrict__ Y, int N, int HW,
                                                      int C, bf16_t* __restrict__ out) {
-  __shared__ bf16_t tile[32][33];
-  const int pt = blockIdx.x, ct = blockIdx.y;
-  const int64_t p0 = (int64_t)pt * 32;  // flat pixel over N * HW (HW % 32 == 0)
-  const int c0 = ct * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32
-  for (int r = ty; r < 32; r += 8) {
-    const int64_t p = p0 + r;
-    const int c = c0 + tx;
-    bf16_t v = 0;
-    if (c < C) {
-      const int64_t idx = p * C + c;
-      v = dY[idx];
-      if (MASK && Y && bf2f(Y[idx]) <= 0.f) {
-        v = 0;
-        dY[idx] = 0;
+  constexpr int TP = 64, TC = 32, LP = TP + 8;  // padded pixel stride (16-byte aligned rows)
+  __shared__ __align__(16) bf16_t tile[TC][LP];
+  const int64_t p0 = (int64_t)blockIdx.x * TP;  // flat pixel over N * HW
+  const int c0 = blockIdx.y * TC;
+  {
+    const int px = threadIdx.x >> 2, ch = 8 * (threadIdx.x & 3);
+    if (c0 + ch < C) {
+      const int64_t idx = (p0 + px) * C + c0 + ch;
+      B8 v;
+      v.u = *reinterpret_cast<const uint4*>(dY + idx);
+      if (MASK && Y) {
+        B8 y;
+        y.u = *reinterpret_cast<const uint4*>(Y + idx);
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (bf2f(y.h[k]) <= 0.f) {
+            v.h[k] = 0;
+            any = true;
+          }
+        if (any) *reinterpret_cast<uint4*>(dY + idx) = v.u;
       }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) tile[ch + k][px] = v.h[k];
     }
-    tile[r][tx] = v;
   }
   __syncthreads();
   const int n = (int)(p0 / HW);
   const int hw0 = (int)(p0 - (int64_t)n * HW);
-  for (int r = ty; r < 32; r += 8) {
-    const int c = c0 + r;
-    if (c < C) out[((int64_t)n * C + c) * HW + hw0 + tx] = tile[tx][r];
-  }
+  const int c = threadIdx.x >> 3, pc = 8 * (threadIdx.x & 7);
+  if (c0 + c < C)
+    *reinterpret_cast<uint4*>(out + ((int64_t)n * C + c0 + c) * HW + hw0 + pc) =
+        *reinterpret_cast<const uint4*>(&tile[c][pc]);
 }
 
 // ---------------------------------------------------------------------------
@@ -421,11 +437,6 @@ __global__ __launch_bounds__(256) void k_channel_partial(const bf16_t* __restric
 // ---------------------------------------------------------------------------
 // Element-wise kernels on NHWC tensors with channel counts that are multiples
 // of 8: one thread per (pixel, 8-channel group), 16-byte loads and stores.
-union B8 {
-  uint4 u;
-  bf16_t h[8];
-};
-
 __device__ __forceinline__ B8 ld_b8(const bf16_t* p) {
   B8 r;
   r.u = *reinterpret_cast<const uint4*>(p);
@@ -708,8 +719,8 @@ hipError_t launch_conv_weight_rot(const bf16_t* Wt, int Cout, int Cin, bf16_t* W
 
 hipError_t launch_nhwc_to_nchw(const bf16_t* X, int N, int H, int W, int C, bf16_t* XT,
                                hipStream_t s) {
-  if ((H * W) % 32 != 0) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((int64_t)N * H * W / 32), (unsigned)((C + 31) / 32));
+  if ((H * W) % 64 != 0 || C % 8 != 0) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((int64_t)N * H * W / 64), (unsigned)((C + 31) / 32));
   hipLaunchKernelGGL((k_transpose_t<false>), grid, dim3(256), 0, s, const_cast<bf16_t*>(X),
                      nullptr, N, H * W, C, XT);
   return hipGetLastError();
@@ -717,8 +728,8 @@ hipError_t launch_nhwc_to_nchw(const bf16_t* X, int N, int H, int W, int C, bf16
 
 hipError_t launch_relu_mask_t(bf16_t* dY, const bf16_t* Y, int N, int H, int W, int C,
                               bf16_t* dYT, hipStream_t s) {
-  if ((H * W) % 32 != 0) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((int64_t)N * H * W / 32), (unsigned)((C + 31) / 32));
+  if ((H * W) % 64 != 0 || C % 8 != 0) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((int64_t)N * H * W / 64), (unsigned)((C + 31) / 32));
   hipLaunchKernelGGL((k_transpose_t<true>), grid, dim3(256), 0, s, dY, Y, N, H * W, C, dYT);
   return hipGetLastError();
 }

@@ -17,7 +17,7 @@ hipError_t launch_conv3x3_fwd(const bf16_t* X, int N, int H, int W, int Cin, con
                               const float* bias, int Cout, int relu, bf16_t* Y, hipStream_t s);
 // Wr[ci][t][co] = W[co][8 - t][ci] (the kernel of the data gradient).
 hipError_t launch_conv_weight_rot(const bf16_t* Wt, int Cout, int Cin, bf16_t* Wr, hipStream_t s);
-// XT[n][c][y][x] = X[n][y][x][c].
+// XT[n][c][y][x] = X[n][y][x][c] (H W a multiple of 64, C of 8; likewise below).
 hipError_t launch_nhwc_to_nchw(const bf16_t* X, int N, int H, int W, int C, bf16_t* XT,
                                hipStream_t s);
 // dY *= (Y > 0) in place (NHWC) and dYT = the masked dY as NCHW.  Y may be
