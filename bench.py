@@ -172,7 +172,11 @@ def bench_fixed(args, H, W, np, torch):
     value = nimg * n * n / 1e6 / dt
     algo_bytes = 4.0 * n * n * nclips + 4.0 * n * n * nimg + sum(8.0 * k * s for k, s in zip(Ks, S))
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in stages.items() if v[1] > 0}
-    f_ms = stage_ms["socs_fields"]
+    # the call is issued as clip ranges: totals per step, launches per step
+    step_ms = {k: round(v[0] / args.steps, 4) for k, v in stages.items() if v[1] > 0}
+    launches = {k: v[1] // args.steps for k, v in stages.items() if v[1] > 0}
+    nl = launches["socs_fields"]
+    f_ms = step_ms["socs_fields"]
     exec_flops = fields_flops(nimg, max(Ks), m)
     algo_flops = fields_algorithmic_flops(Ks, B, m, nclips)
     achieved_tf = algo_flops / (f_ms * 1e-3) / 1e12
@@ -207,28 +211,32 @@ def bench_fixed(args, H, W, np, torch):
                         "(huv_kernels_build_device) + uploads",
         "host_kernel_build_s_per_plane": round(host_plane_s, 3),
         "stage_ms_per_launch": stage_ms,
+        "stage_ms_per_step": step_ms,
+        "clip_ranges": nl,
         "roofline": {"bound": "valu", "achieved": round(achieved_tf, 2), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4),
                      "traffic": traffic, "kernel": "socs_fields",
                      "compute": "FP32 on the vector ALUs (no matrix instructions)",
-                     "algorithmic_flops_per_launch": algo_flops,
+                     "algorithmic_flops_per_launch": algo_flops / nl,
+                     "launches_per_step": nl,
                      "algorithmic_formula": "per image: K (R + m) 5 m log2 m + 3 K m^2, "
                                             f"R = 2B+1 = {2 * B + 1}, m = {m}",
-                     "executed_flops": exec_flops,
+                     "executed_flops_per_launch": exec_flops / nl,
                      "frac_executed": round(exec_tf / FP32_PEAK_TFLOPS, 4),
                      "attainable": {
                          "rate_tflops": FP32_NONPACKED_TFLOPS,
                          "why": "packed FP32 instructions are disabled in every device build, "
                                 "halving the vector rate",
-                         "min_ms_executed_flops": round(exec_flops / FP32_NONPACKED_TFLOPS / 1e9, 4),
+                         "min_ms_per_step_executed_flops": round(
+                             exec_flops / FP32_NONPACKED_TFLOPS / 1e9, 4),
                          "frac_of_attainable": round(exec_tf / FP32_NONPACKED_TFLOPS, 4)},
-                     "ms_per_launch": f_ms},
+                     "ms_per_launch": stage_ms["socs_fields"], "ms_per_step": f_ms},
         "roofline_path_hbm": {"bound": "hbm", "achieved": round(algo_bytes / dt / 1e9, 1),
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(algo_bytes / dt / 1e9 / HBM_PEAK_GBS, 4),
                               "algorithmic_bytes_per_step": algo_bytes,
                               "band_image_write_GBps": round(
-                                  4.0 * n * n * nimg / (stage_ms["band_image"] * 1e-3) / 1e9, 1)},
+                                  4.0 * n * n * nimg / (step_ms["band_image"] * 1e-3) / 1e9, 1)},
     }
     if not args.no_cpu:
         import oracle
@@ -463,7 +471,7 @@ def bench_uneven(args, H, W, np, torch):
     for masks, planes, _, eng in groups:
         e = masks.shape[-1]
         tuned[str(e)] = {"band": planes[0].band, "count": int(masks.shape[0]),
-                         "m_n1_xnb_xlanes": ctx.tuning(e, planes[0].band, 1, planes[0].host.num_kernels)}
+                         "m_n1_xnb_xlanes_chunks": ctx.tuning(e, planes[0].band, 1, planes[0].host.num_kernels)}
     return {"workload": f"{len(edges)} clips, edges 256/512/1024/2048 px at 2 nm, ArF 193i "
                         "annular 0.7/0.9, K=32, one graph launch",
             "mpix_per_batch": round(pix / 1e6, 1), "ms_per_batch": round(dt * 1e3, 3),

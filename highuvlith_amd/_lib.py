@@ -78,7 +78,8 @@ class huv_polygons(C.Structure):
 
 
 class huv_socs_tuning(C.Structure):
-    _fields_ = [("m", C.c_int), ("n1", C.c_int), ("xnb", C.c_int), ("xlanes", C.c_int)]
+    _fields_ = [("m", C.c_int), ("n1", C.c_int), ("xnb", C.c_int), ("xlanes", C.c_int),
+                ("chunks", C.c_int)]
 
 
 class huv_clip_group(C.Structure):

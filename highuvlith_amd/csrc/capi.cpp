@@ -658,7 +658,7 @@ static huv::Illumination to_ill(const huv_illumination* i) {
 extern "C" {
 
 const char* huv_last_error(void) { return g_err.c_str(); }
-int huv_abi_version(void) { return 101; }
+int huv_abi_version(void) { return 102; }
 
 double huv_illumination_at(const huv_illumination* ill, double sx, double sy) {
   if (!ill) return 0.0;
@@ -1063,8 +1063,31 @@ huv_status plane_pack(Ctx* c, int nplanes, huv_kset* const* planes, int m, bool 
   return HUV_OK;
 }
 
-// Forward images of real masks: band spectrum -> fused fields -> epilogue.
-// Leaves the twisted band spectrum in Workspace::BAND (returned in *Mq_out).
+// Bytes one image keeps between the stages of socs_forward: S coarse partials
+// (FP32 m x m) and the intensity band ((2B + 1) x (4B + 1) complex).
+size_t image_interm_bytes(int m, int B, int S) {
+  return (size_t)S * m * m * sizeof(float) + (size_t)(2 * B + 1) * (4 * B + 1) * sizeof(float2);
+}
+
+// Default number of clip ranges: enough that one range's intermediates (the
+// coarse images and intensity bands written by one stage and read by the
+// next) stay within about 160 MB, so they are served from the device's
+// last-level cache rather than HBM, but no more than keeps at least 768
+// field work-groups per range (1.5 rounds of two per CU).
+int clip_ranges_default(int nclips, int nplanes, int m, int B, int S) {
+  const double interm = (double)nclips * nplanes * (double)image_interm_bytes(m, B, S);
+  int r = (int)std::ceil(interm / 160e6);
+  const int wgs = nclips * huvk::coarse_classes(m) * S;
+  r = std::min(r, std::max(1, wgs / 768));
+  return std::max(1, std::min(r, 16));
+}
+
+// Forward images: band spectrum -> fields -> coarse transforms -> images,
+// issued per clip range (all on the context stream; huv_socs_tuning.chunks
+// ranges or clip_ranges_default).  Results do not depend on the ranges: the
+// kernel splits S follow the whole call (or split_batch), and every clip's
+// arithmetic is the same.  Leaves the twisted band spectrum of every clip in
+// Workspace::BAND (returned in *Mq_out).
 huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                         huv_kset* const* planes, float* out, bool pt, float2** Mq_out,
                         const huv_socs_tuning* tune, int max_splits, int split_batch,
@@ -1080,7 +1103,6 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int np
   const double flare = planes[0]->flare;
   hipError_t e;
   const int W = 2 * BB + 1;
-  const int nimg = nclips * nplanes;
   float2* Mq = (float2*)c->buf(Workspace::BAND, (size_t)nclips * W * W * sizeof(float2), e);
   if (!Mq) return hip_fail(e, "workspace");
   if (Mq_out) *Mq_out = Mq;
@@ -1090,26 +1112,10 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int np
     Mp = (float2*)c->buf(Workspace::BAND_PAIRED, (size_t)nclips * (W + 1) * W * sizeof(float2), e);
     if (!Mp) return hip_fail(e, "workspace");
   }
-  huv_status st = HUV_OK;
-  switch (src.kind) {
-    case MaskSource::REAL:
-      st = band_stage(c, nclips, n, B, BB, src.raster, Mq, pt, n1, Mp);
-      break;
-    case MaskSource::COMPLEX:
-      st = band_stage_complex(c, nclips, n, B, BB, (const float2*)src.raster, Mq, pt, n1, Mp);
-      break;
-    case MaskSource::FEATURES:
-      st = rect_band_stage(c, nclips, n, B, BB, *src.rects, Mq, Mp);
-      break;
-    case MaskSource::SPECTRUM:
-      st = spectrum_band_stage(c, nclips, n, B, BB, src.spectra, Mq, Mp);
-      break;
-  }
-  if (st != HUV_OK) return st;
   const float2* kern;
   const float* lam;
   int Kmax;
-  st = plane_pack(c, nplanes, planes, m, xlanes == 2, &kern, &lam, &Kmax);
+  huv_status st = plane_pack(c, nplanes, planes, m, xlanes == 2, &kern, &lam, &Kmax);
   if (st != HUV_OK) return st;
   int S = huvk::fields_splits(m, split_batch > 0 ? split_batch : nclips, Kmax);
   if (max_splits > 0 && S > max_splits) {
@@ -1117,47 +1123,79 @@ huv_status socs_forward(Ctx* c, int nclips, int n, const MaskSource& src, int np
     const int ks = (Kmax + max_splits - 1) / max_splits;
     S = (Kmax + ks - 1) / ks;
   }
-  float* coarse = (float*)c->buf(Workspace::COARSE, (size_t)S * nimg * m * m * sizeof(float), e);
+  // Clip ranges (one for the fidelity epilogue, whose partials are indexed by
+  // image, and for the no-allocation path, whose scratch is sized for one).
+  int nr = tune && tune->chunks ? tune->chunks : clip_ranges_default(nclips, nplanes, m, B, S);
+  if (fid || c->fixed_mode) nr = 1;
+  nr = std::max(1, std::min(nr, nclips));
+  const int cc = (nclips + nr - 1) / nr;  // clips per range
+  nr = (nclips + cc - 1) / cc;            // ranges actually used
+  const int cimg = cc * nplanes;          // images per range
+  const bool raster = src.kind == MaskSource::REAL || src.kind == MaskSource::COMPLEX;
+  // Feature lists and host spectra: the band of every clip at once (their
+  // stages keep no per-clip rows); rasters: per range, below.
+  if (!raster) {
+    st = src.kind == MaskSource::FEATURES
+             ? rect_band_stage(c, nclips, n, B, BB, *src.rects, Mq, Mp)
+             : spectrum_band_stage(c, nclips, n, B, BB, src.spectra, Mq, Mp);
+    if (st != HUV_OK) return st;
+  }
+  float* coarse = (float*)c->buf(Workspace::COARSE, (size_t)S * cimg * m * m * sizeof(float), e);
   if (!coarse) return hip_fail(e, "workspace");
   const float2* tw_m = c->twiddles(m, e);
   if (!tw_m) return hip_fail(e, "twiddle table");
-  hipEvent_t ev;
-  c->stage_begin(ST_FIELDS, &ev);
-  e = huvk::launch_socs_fields(m, xnb, xlanes, nclips, nplanes, Kmax, S, kern, lam,
-                               xlanes == 2 ? Mp : Mq, tw_m, coarse, c->stream());
-  if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
-  c->stage_end(ST_FIELDS, ev);
   const float2* tw_n = c->twiddles(n, e);
   if (!tw_n) return hip_fail(e, "twiddle table");
   const int G = 2 * B;
   float2* tmp1 = nullptr;
   const bool fused = huvk::coarse_fused_supported(m);
   if (!fused) {
-    tmp1 = (float2*)c->buf(Workspace::TMP_COARSE, (size_t)nimg * (2 * G + 1) * (m / 2) * sizeof(float2), e);
+    tmp1 = (float2*)c->buf(Workspace::TMP_COARSE, (size_t)cimg * (2 * G + 1) * (m / 2) * sizeof(float2), e);
     if (!tmp1) return hip_fail(e, "workspace");
   }
-  float2* Ib = (float2*)c->buf(Workspace::IBAND, (size_t)nimg * (G + 1) * (2 * G + 1) * sizeof(float2), e);
+  float2* Ib = (float2*)c->buf(Workspace::IBAND, (size_t)cimg * (G + 1) * (2 * G + 1) * sizeof(float2), e);
   if (!Ib) return hip_fail(e, "workspace");
-  if (fused) {
-    c->stage_begin(ST_COARSE, &ev);
-    e = huvk::launch_coarse_fused(m, B, nimg, S, coarse, (float)flare, tw_m, Ib, c->stream());
-    if (e != hipSuccess) return hip_fail(e, "coarse launch");
-    c->stage_end(ST_COARSE, ev);
-  } else {
-    c->stage_begin(ST_COARSE_ROWS, &ev);
-    e = huvk::launch_coarse_rows(m, B, nimg, S, coarse, tw_m, tmp1, c->stream());
-    if (e != hipSuccess) return hip_fail(e, "coarse_rows launch");
-    c->stage_end(ST_COARSE_ROWS, ev);
-    c->stage_begin(ST_COARSE_COLS, &ev);
-    e = huvk::launch_coarse_cols(m, B, nimg, (float)flare, tmp1, tw_m, Ib, c->stream());
-    if (e != hipSuccess) return hip_fail(e, "coarse_cols launch");
-    c->stage_end(ST_COARSE_COLS, ev);
+  const float2* box = xlanes == 2 ? Mp : Mq;
+  const size_t box_c = (size_t)(xlanes == 2 ? W + 1 : W) * W;  // band box per clip
+  hipEvent_t ev;
+  for (int i = 0; i < nr; ++i) {
+    const int c0 = i * cc, cn = std::min(cc, nclips - c0), ni = cn * nplanes;
+    if (src.kind == MaskSource::REAL) {
+      st = band_stage(c, cn, n, B, BB, src.raster + (size_t)c0 * n * n, Mq + (size_t)c0 * W * W,
+                      pt, n1, Mp ? Mp + (size_t)c0 * (W + 1) * W : nullptr);
+    } else if (src.kind == MaskSource::COMPLEX) {
+      st = band_stage_complex(c, cn, n, B, BB, (const float2*)src.raster + (size_t)c0 * n * n,
+                              Mq + (size_t)c0 * W * W, pt, n1,
+                              Mp ? Mp + (size_t)c0 * (W + 1) * W : nullptr);
+    }
+    if (st != HUV_OK) return st;
+    c->stage_begin(ST_FIELDS, &ev);
+    e = huvk::launch_socs_fields(m, xnb, xlanes, cn, nplanes, Kmax, S, kern, lam,
+                                 box + (size_t)c0 * box_c, tw_m, coarse, c->stream());
+    if (e != hipSuccess) return hip_fail(e, "socs_fields launch");
+    c->stage_end(ST_FIELDS, ev);
+    if (fused) {
+      c->stage_begin(ST_COARSE, &ev);
+      e = huvk::launch_coarse_fused(m, B, ni, S, coarse, (float)flare, tw_m, Ib, c->stream());
+      if (e != hipSuccess) return hip_fail(e, "coarse launch");
+      c->stage_end(ST_COARSE, ev);
+    } else {
+      c->stage_begin(ST_COARSE_ROWS, &ev);
+      e = huvk::launch_coarse_rows(m, B, ni, S, coarse, tw_m, tmp1, c->stream());
+      if (e != hipSuccess) return hip_fail(e, "coarse_rows launch");
+      c->stage_end(ST_COARSE_ROWS, ev);
+      c->stage_begin(ST_COARSE_COLS, &ev);
+      e = huvk::launch_coarse_cols(m, B, ni, (float)flare, tmp1, tw_m, Ib, c->stream());
+      if (e != hipSuccess) return hip_fail(e, "coarse_cols launch");
+      c->stage_end(ST_COARSE_COLS, ev);
+    }
+    c->stage_begin(ST_BAND_IMAGE, &ev);
+    e = fid ? huvk::launch_band_image_fidelity(n, G, ni, Ib, tw_n, *fid, c->stream())
+            : huvk::launch_band_image(n, G, ni, Ib, tw_n, out + (size_t)c0 * nplanes * n * n,
+                                      c->stream());
+    if (e != hipSuccess) return hip_fail(e, "band_image launch");
+    c->stage_end(ST_BAND_IMAGE, ev);
   }
-  c->stage_begin(ST_BAND_IMAGE, &ev);
-  e = fid ? huvk::launch_band_image_fidelity(n, G, nimg, Ib, tw_n, *fid, c->stream())
-          : huvk::launch_band_image(n, G, nimg, Ib, tw_n, out, c->stream());
-  if (e != hipSuccess) return hip_fail(e, "band_image launch");
-  c->stage_end(ST_BAND_IMAGE, ev);
   return HUV_OK;
 }
 
@@ -1176,7 +1214,8 @@ std::vector<int> tuning_key(int n, const huv_kset* const* planes, int nplanes) {
 }
 
 // Times every launch shape on this call's inputs (coordinate descent: coarse
-// grid, then band-FFT factor, then x-pass block) and records the fastest.
+// grid, then band-FFT factor, then x-pass block, then clip ranges) and
+// records the fastest.
 // The last run leaves a valid result in `out`.
 huv_status autotune_socs(Ctx* c, int nclips, int n, const MaskSource& src, int nplanes,
                          huv_kset* const* planes, float* out, bool pt, huv_socs_tuning* best) {
@@ -1204,12 +1243,12 @@ huv_status autotune_socs(Ctx* c, int nclips, int n, const MaskSource& src, int n
     return HUV_OK;
   };
   // explicit values throughout (0 would mean "default" in the table)
-  huv_socs_tuning cur{planes[0]->m, huvk::band_fft_default(n), huvk::kFieldsXnbDefault, 1};
+  huv_socs_tuning cur{planes[0]->m, huvk::band_fft_default(n), huvk::kFieldsXnbDefault, 1, 1};
   double cur_ms = 1e30;
   const int B = planes[0]->B;
   for (int m : {32, 64, 128, 192, 256}) {
     if (m < 4 * B + 1 || m > n) continue;
-    huv_socs_tuning t{m, cur.n1, cur.xnb, cur.xlanes};
+    huv_socs_tuning t{m, cur.n1, cur.xnb, cur.xlanes, cur.chunks};
     double ms;
     st = time_one(t, ms);
     if (st != HUV_OK) break;
@@ -1236,6 +1275,14 @@ huv_status autotune_socs(Ctx* c, int nclips, int n, const MaskSource& src, int n
       if (st == HUV_OK && ms < cur_ms) { cur_ms = ms; cur = t; }
     }
   }
+  for (int chunks : {2, 3, 4, 6, 8}) {
+    if (st != HUV_OK || chunks > nclips) break;
+    huv_socs_tuning t = cur;
+    t.chunks = chunks;
+    double ms;
+    st = time_one(t, ms);
+    if (st == HUV_OK && ms < cur_ms) { cur_ms = ms; cur = t; }
+  }
   // leave the chosen shape's result in `out`
   if (st == HUV_OK) st = socs_forward(c, nclips, n, src, nplanes, planes, out, pt, nullptr, &cur);
   c->profiling = prof;
@@ -1256,7 +1303,7 @@ huv_status socs_shape(Ctx* c, int nclips, int n, const MaskSource& src, int npla
     *t = it->second;
     return HUV_OK;
   }
-  *t = huv_socs_tuning{0, 0, 0, 0};
+  *t = huv_socs_tuning{0, 0, 0, 0, 0};
   if (!c->autotune) return HUV_OK;
   huv_status st = autotune_socs(c, nclips, n, src, nplanes, planes, out, pt, t);
   if (st != HUV_OK) return st;
@@ -1359,7 +1406,8 @@ huv_status socs_groups(Ctx* c, int ngroups, const huv_clip_group* groups, const 
     plan.push_back(gp);
     key.insert(key.end(), {(uint64_t)g.n, (uint64_t)g.nclips, (uint64_t)(uintptr_t)g.masks,
                            (uint64_t)(uintptr_t)g.out, (uint64_t)g.nplanes, (uint64_t)gp.t.m,
-                           (uint64_t)gp.t.n1, (uint64_t)gp.t.xnb, (uint64_t)gp.t.xlanes});
+                           (uint64_t)gp.t.n1, (uint64_t)gp.t.xnb, (uint64_t)gp.t.xlanes,
+                           (uint64_t)gp.t.chunks});
     for (int p = 0; p < g.nplanes; ++p) {
       key.push_back(g.planes[p]->id);
       ksets.push_back(g.planes[p]->id);
@@ -1540,7 +1588,7 @@ huv_status huv_ctx_tuning_get(huv_ctx* c, int n, int band, int nplanes, int kmax
   std::lock_guard<std::mutex> lk(c->mu);
   auto it = c->tuned.find({n, band, nplanes, kmax});
   *found = it != c->tuned.end();
-  *out = *found ? it->second : huv_socs_tuning{0, 0, 0, 0};
+  *out = *found ? it->second : huv_socs_tuning{0, 0, 0, 0, 0};
   return HUV_OK;
 }
 
@@ -1562,6 +1610,8 @@ huv_status huv_ctx_tuning_set(huv_ctx* c, int n, int band, int nplanes, int kmax
   if (t->xlanes < 0 || t->xlanes > 2 ||
       (t->xlanes == 2 && t->m != 0 && !huvk::fields_lanes_supported(t->m, 2)))
     return fail(HUV_INVALID_PARAMETER, "tuning: xlanes must be 1, or 2 with a coarse grid of 64 or more");
+  if (t->chunks < 0 || t->chunks > 16)
+    return fail(HUV_INVALID_PARAMETER, "tuning: chunks must be 0 (default) or 1 to 16");
   std::lock_guard<std::mutex> lk(c->mu);
   c->tuned[{n, band, nplanes, kmax}] = *t;
   return HUV_OK;

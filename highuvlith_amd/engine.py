@@ -356,15 +356,15 @@ class DeviceContext:
         check(self.lib.huv_ctx_set_autotune(self.handle, int(bool(on))))
 
     def tuning(self, n, band, nplanes, kmax):
-        """(m, n1, xnb, xlanes) chosen for this shape class, or None."""
+        """(m, n1, xnb, xlanes, chunks) chosen for this shape class, or None."""
         t = _lib.huv_socs_tuning()
         found = C.c_int()
         check(self.lib.huv_ctx_tuning_get(self.handle, n, band, nplanes, kmax, C.byref(t),
                                           C.byref(found)))
-        return (t.m, t.n1, t.xnb, t.xlanes) if found.value else None
+        return (t.m, t.n1, t.xnb, t.xlanes, t.chunks) if found.value else None
 
-    def set_tuning(self, n, band, nplanes, kmax, m=0, n1=0, xnb=0, xlanes=0):
-        t = _lib.huv_socs_tuning(int(m), int(n1), int(xnb), int(xlanes))
+    def set_tuning(self, n, band, nplanes, kmax, m=0, n1=0, xnb=0, xlanes=0, chunks=0):
+        t = _lib.huv_socs_tuning(int(m), int(n1), int(xnb), int(xlanes), int(chunks))
         check(self.lib.huv_ctx_tuning_set(self.handle, n, band, nplanes, kmax, C.byref(t)))
 
     def upload(self, ks, flare):

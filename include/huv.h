@@ -201,13 +201,17 @@ typedef struct {
  * coarse grid m (32 Q; sets the number of 32-point transform classes Q),
  * band-FFT factor n1 (four-step n = n1 x n / n1), and the field kernel's
  * x pass: xlanes = 1 or 2 lanes per band row (2 needs m >= 64), xnb = 4, 6
- * or 8 band columns per load block.  Zero fields select the built-in
+ * or 8 band columns per load block; chunks = 1..16 clip ranges issued one
+ * after the other (each range's intermediates are then small enough to stay
+ * in the device's last-level cache between stages; the default is sized for
+ * that; results do not depend on it).  Zero fields select the built-in
  * defaults. */
 typedef struct {
   int m;
   int n1;
   int xnb;
   int xlanes;
+  int chunks;
 } huv_socs_tuning;
 
 /* Autotuner: when on, the first huv_socs_images / huv_socs_images_batch call

@@ -37,7 +37,7 @@ def test_every_declared_symbol_has_a_python_signature():
 
 def test_abi_version_and_errors_are_reported():
     lib = _lib.load()
-    assert lib.huv_abi_version() == 101
+    assert lib.huv_abi_version() == 102
     o = H.ProjectionOptics(0.75).to_c()
     pts = np.array([[0.0, 0.0, 1.0]])
     h = C.c_void_p()

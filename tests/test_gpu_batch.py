@@ -57,6 +57,67 @@ def test_graph_replays_match_eager_calls():
     assert np.abs(out_g[1, 1].cpu().numpy() - ref).max() <= TOL
 
 
+def test_clip_ranges_equal_one_range():
+    """A call issued as several clip ranges (huv_socs_tuning.chunks: band,
+    fields, coarse transforms and images per range, one range after the other)
+    gives the same images bit for bit as one range: ragged last ranges, more
+    ranges than clips, the fused (m <= 192) and two-pass (m = 256) coarse
+    stage, real and complex rasters and rectangle lists, eager calls and graph
+    replays with torch work on the same buffers between them."""
+    import torch
+    eng = _arf_engine(256, 2.0, 10)
+    ctx = eng.context()
+    planes = [eng.device_kernel_set(0.0), eng.device_kernel_set(40.0)]
+    B, K = planes[0].band, max(p.host.num_kernels for p in planes)
+    N = 7
+    masks = torch.from_numpy(_masks(256, N, 21)).cuda()
+    cmasks = (masks * torch.exp(1j * 0.3 * masks)).to(torch.complex64).contiguous()
+    rng = np.random.default_rng(5)
+    rect_clips = [[(float(rng.uniform(-200, 200)), float(rng.uniform(-200, 200)),
+                    float(rng.uniform(20, 150)), float(rng.uniform(20, 150)), 1.0)
+                   for _ in range(6)] for _ in range(N)]
+    arrays = H.engine.rect_arrays(rect_clips)
+    grid = H.GridConfig(256, 2.0)
+    calls = {
+        "real": lambda o, g: H.engine.socs_images_device(ctx, masks, planes, o, graph=g),
+        "complex": lambda o, g: H.engine.socs_images_complex_device(ctx, cmasks, planes, o,
+                                                                     graph=g),
+        "rects": lambda o, g: H.engine.socs_images_rects_device(ctx, grid, arrays, planes, o),
+    }
+    ref = torch.empty((N, 2, 256, 256), device="cuda")
+    out = torch.empty_like(ref)
+    sizes = [m for m in (64, 128, 256) if m >= 4 * B + 1]
+    assert 256 in sizes and len(sizes) >= 2
+    for m in sizes:
+        for kind, call in calls.items():
+            if kind != "real" and m != sizes[0]:
+                continue
+            ctx.set_tuning(256, B, 2, K, m=m, chunks=1)
+            call(ref, False)
+            torch.cuda.synchronize()
+            if kind == "real":
+                err = np.abs(ref[6, 1].cpu().numpy()
+                             - _oracle(eng, masks[6].cpu().numpy(), 40.0)).max()
+                assert err <= TOL
+            for chunks in (2, 3, 4, 16):
+                ctx.set_tuning(256, B, 2, K, m=m, chunks=chunks)
+                out.fill_(-1.0)
+                call(out, False)
+                torch.cuda.synchronize()
+                assert torch.equal(out, ref), (kind, m, chunks)
+                if kind == "rects":
+                    continue
+                for rep in range(4):  # the first call records, the others replay
+                    out.fill_(-1.0)
+                    call(out, True)
+                    scratch = torch.randn(1 << 18, device="cuda").sum()
+                    torch.cuda.synchronize()
+                    print(kind, "m", m, "chunks", chunks, "replay", rep, float(scratch))
+                    assert torch.equal(out, ref), (kind, m, chunks, rep)
+    with pytest.raises(H.HuvError):
+        ctx.set_tuning(256, B, 2, K, chunks=17)
+
+
 def test_uneven_batch_matches_per_group_calls_and_oracle():
     import torch
     ctx = H.DeviceContext(0)
@@ -99,7 +160,7 @@ def test_autotuner_records_a_valid_shape_within_the_gate():
     H.engine.socs_images_device(ctx, masks, planes, out)
     torch.cuda.synchronize()
     t = ctx.tuning(1024, B, 1, planes[0].host.num_kernels)
-    print("band", B, "tuned (m, n1, xnb, xlanes)", t)
+    print("band", B, "tuned (m, n1, xnb, xlanes, chunks)", t)
     assert t is not None and t[0] >= 4 * B + 1 and t[0] in (32, 64, 128, 192, 256)
     assert t[1] in (16, 32) and t[2] in (4, 6, 8) and t[3] in (1, 2)
     for c in range(2):
