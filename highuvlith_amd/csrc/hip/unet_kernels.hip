@@ -130,15 +130,17 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd(const bf16_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Forward 3x3 convolution with the input staged in LDS.  Work-group = 4 waves
-// over an output tile of TH = 2 BM rows x 64 columns of one image; wave w takes
-// columns x0 + 32 (w % 2) .. + 31 of rows y0 + w / 2 + 2 b (b < BM), and BN
-// blocks of 32 output channels, so one weight fragment feeds BM products.
-// Input channels run in chunks of CC: the (TH + 2) x (64 + 2) x CC halo is
-// loaded once per chunk (zero outside the image) and read by all nine taps
-// from LDS; A = 32 pixels x 16 k per MFMA step (two 8-channel blocks of
-// (tap, channel) index 2s and 2s + 1), B = weights from global memory.
-// Needs W % 64 == 0 and H % TH == 0.
+// Forward 3x3 convolution with the input and the weights staged in LDS.
+// Work-group = 4 waves over an output tile of TH = 2 BM rows x 64 columns of
+// one image; wave w takes columns x0 + 32 (w % 2) .. + 31 of rows
+// y0 + w / 2 + 2 b (b < BM), and BN blocks of 32 output channels, so one
+// weight fragment feeds BM products.  Input channels run in chunks of CC:
+// the (TH + 2) x (64 + 2) x CC halo and the chunk's 9 x CC weights of the
+// 32 BN output channels are loaded once per chunk (zero outside the image
+// and beyond Cout) and read by all four waves from LDS; A = 32 pixels x 16 k
+// per MFMA step (two 8-channel blocks of (tap, channel) index 2s and
+// 2s + 1), B = 16 k x 32 output channels.  Needs W % 64 == 0 and
+// H % TH == 0.
 // ---------------------------------------------------------------------------
 template <int BM, int BN, int CC>
 __global__ __launch_bounds__(256) void k_conv3x3_fwd_lds(const bf16_t* __restrict__ X, int N,
@@ -149,7 +151,9 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd_lds(const bf16_t* __restric
   constexpr int TH = 2 * BM, TW = 64, HR = TH + 2, HC = TW + 2;
   constexpr int PS = CC + 8;  // pixel stride in the tile (bf16), spreads the LDS banks
   constexpr int CB = CC / 8, KBC = 9 * CB, STEPS = (KBC + 1) / 2;
+  constexpr int WS = 9 * CC + 8;  // output-channel stride of the staged weights (bf16)
   __shared__ __attribute__((aligned(16))) bf16_t tile[HR * HC * PS];
+  __shared__ __attribute__((aligned(16))) bf16_t wts[32 * BN * WS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int half = lane >> 5, li = lane & 31;
   const int tilesX = W / TW, tilesY = H / TH;
@@ -172,14 +176,40 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd_lds(const bf16_t* __restric
       for (int r = 0; r < 16; ++r) acc[bm][bn][r] = 0.f;
 
   for (int c0 = 0; c0 < Cin; c0 += CC) {
-    __syncthreads();  // the previous chunk has been consumed
-    for (int i = threadIdx.x; i < HR * HC * CB; i += 256) {
+    // Every load of the chunk is issued before the first LDS store, so their
+    // latencies overlap.
+    constexpr int HALO = HR * HC * CB, HIT = (HALO + 255) / 256;
+    constexpr int WGT = 32 * BN * KBC, WIT = (WGT + 255) / 256;
+    uint4 hv[HIT], wv[WIT];
+#pragma unroll
+    for (int k = 0; k < HIT; ++k) {
+      const int i = threadIdx.x + 256 * k;
       const int cb = i % CB, pix = i / CB;
       const int yy = y0 - 1 + pix / HC, xx = x0 - 1 + pix % HC;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-        v = *reinterpret_cast<const uint4*>(X + (((size_t)n * H + yy) * W + xx) * Cin + c0 + 8 * cb);
-      *reinterpret_cast<uint4*>(tile + pix * PS + 8 * cb) = v;
+      hv[k] = make_uint4(0, 0, 0, 0);
+      if (i < HALO && yy >= 0 && yy < H && xx >= 0 && xx < W)
+        hv[k] = *reinterpret_cast<const uint4*>(X + (((size_t)n * H + yy) * W + xx) * Cin + c0 +
+                                                8 * cb);
+    }
+#pragma unroll
+    for (int k = 0; k < WIT; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      const int kb = i % KBC, col = i / KBC;  // kb = tap * CB + cb
+      const int tap = kb / CB, cb = kb % CB, c = cobase + col;
+      wv[k] = make_uint4(0, 0, 0, 0);
+      if (i < WGT && c < Cout)
+        wv[k] = *reinterpret_cast<const uint4*>(Wt + ((size_t)c * 9 + tap) * Cin + c0 + 8 * cb);
+    }
+    __syncthreads();  // the previous chunk has been consumed
+#pragma unroll
+    for (int k = 0; k < HIT; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      if (i < HALO) *reinterpret_cast<uint4*>(tile + (i / CB) * PS + 8 * (i % CB)) = hv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < WIT; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      if (i < WGT) *reinterpret_cast<uint4*>(wts + (i / KBC) * WS + 8 * (i % KBC)) = wv[k];
     }
     __syncthreads();
 #pragma unroll
@@ -190,13 +220,15 @@ __global__ __launch_bounds__(256) void k_conv3x3_fwd_lds(const bf16_t* __restric
       const int dy = tap / 3, dx = tap % 3;
       bf16x8 bw[BN];
 #pragma unroll
-      for (int bn = 0; bn < BN; ++bn)
-        bw[bn] = (kv && co[bn] < Cout) ? ld8(Wt + ((size_t)co[bn] * 9 + tap) * Cin + c0 + 8 * cb)
-                                       : zero8();
+      for (int bn = 0; bn < BN; ++bn) {
+        const bf16x8 w = ld8(wts + (32 * bn + li) * WS + 8 * (kv ? kb : 0));
+        bw[bn] = kv ? w : zero8();
+      }
 #pragma unroll
       for (int bm = 0; bm < BM; ++bm) {
         const int py = (wave >> 1) + 2 * bm;
-        const bf16x8 a = kv ? ld8(tile + ((py + dy) * HC + px + dx) * PS + 8 * cb) : zero8();
+        const bf16x8 a0 = ld8(tile + ((py + dy) * HC + px + dx) * PS + 8 * cb);
+        const bf16x8 a = kv ? a0 : zero8();
 #pragma unroll
         for (int bn = 0; bn < BN; ++bn)
           acc[bm][bn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[bn], acc[bm][bn], 0, 0, 0);
@@ -337,21 +369,31 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16_t* __restrict_
     for (int bm = 0; bm < BM; ++bm)
       arow[bm] = dYT + (((int64_t)n * Cout + (av[bm] ? coA[bm] : 0)) * H + y) * W;
     const bf16_t* brow = XT + (((int64_t)n * Cin + (bv ? ciB : 0)) * H + (rv ? yy : 0)) * W;
-    for (int x0 = 0; x0 < W; x0 += 16) {
-      const int xs = x0 + 8 * half;
+    // One 16-pixel segment: the dY rows (A), the input row (B) and its two
+    // neighbouring words for the -1 / +1 column shifts.
+    struct Seg {
       bf16x8 a[BM];
+      uint4 w;
+      uint32_t prev, next;
+    };
+    auto load = [&](int x0, Seg& g) {
+      const int xs = x0 + 8 * half;
 #pragma unroll
-      for (int bm = 0; bm < BM; ++bm) a[bm] = av[bm] ? ld8(arow[bm] + xs) : zero8();
-      uint4 w = make_uint4(0, 0, 0, 0);
-      uint32_t prev = 0u, next = 0u;
+      for (int bm = 0; bm < BM; ++bm) g.a[bm] = av[bm] ? ld8(arow[bm] + xs) : zero8();
+      g.w = make_uint4(0, 0, 0, 0);
+      g.prev = 0u;
+      g.next = 0u;
       if (rv) {
-        w = *reinterpret_cast<const uint4*>(brow + xs);
-        if (xs > 0) prev = *reinterpret_cast<const uint32_t*>(brow + xs - 2);
-        if (xs + 8 < W) next = *reinterpret_cast<const uint32_t*>(brow + xs + 8);
+        g.w = *reinterpret_cast<const uint4*>(brow + xs);
+        if (xs > 0) g.prev = *reinterpret_cast<const uint32_t*>(brow + xs - 2);
+        if (xs + 8 < W) g.next = *reinterpret_cast<const uint32_t*>(brow + xs + 8);
       }
+    };
+    auto mma = [&](const Seg& g) {
+      const uint4 w = g.w;
       // kw = 0: input column x - 1; kw = 1: x; kw = 2: x + 1
       const bf16x8 b0 = __builtin_bit_cast(
-          bf16x8, make_uint4(__builtin_amdgcn_alignbit(w.x, prev, 16),
+          bf16x8, make_uint4(__builtin_amdgcn_alignbit(w.x, g.prev, 16),
                              __builtin_amdgcn_alignbit(w.y, w.x, 16),
                              __builtin_amdgcn_alignbit(w.z, w.y, 16),
                              __builtin_amdgcn_alignbit(w.w, w.z, 16)));
@@ -360,13 +402,29 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16_t* __restrict_
           bf16x8, make_uint4(__builtin_amdgcn_alignbit(w.y, w.x, 16),
                              __builtin_amdgcn_alignbit(w.z, w.y, 16),
                              __builtin_amdgcn_alignbit(w.w, w.z, 16),
-                             __builtin_amdgcn_alignbit(next, w.w, 16)));
+                             __builtin_amdgcn_alignbit(g.next, w.w, 16)));
 #pragma unroll
       for (int bm = 0; bm < BM; ++bm) {
-        acc[bm][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bm], b0, acc[bm][0], 0, 0, 0);
-        acc[bm][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bm], b1, acc[bm][1], 0, 0, 0);
-        acc[bm][2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bm], b2, acc[bm][2], 0, 0, 0);
+        acc[bm][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g.a[bm], b0, acc[bm][0], 0, 0, 0);
+        acc[bm][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g.a[bm], b1, acc[bm][1], 0, 0, 0);
+        acc[bm][2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g.a[bm], b2, acc[bm][2], 0, 0, 0);
       }
+    };
+    // Segments in groups of U, every load of a group issued before its
+    // products, so the loads' latencies overlap.
+    constexpr int U = 4;
+    int x0 = 0;
+    for (; x0 + 16 * U <= W; x0 += 16 * U) {
+      Seg g[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) load(x0 + 16 * u, g[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) mma(g[u]);
+    }
+    for (; x0 < W; x0 += 16) {
+      Seg g;
+      load(x0, g);
+      mma(g);
     }
   }
 #pragma unroll
