@@ -36,7 +36,7 @@ struct Acts {
   std::vector<bf16_t*> e1, e2, pool, cat, d1, d2;        // forward
   std::vector<bf16_t*> ge1, ge2, gpool, gcat, gd1, gd2;  // gradients
   bf16_t *xt = nullptr, *dyt = nullptr, *tmp = nullptr;
-  float *out = nullptr, *dout = nullptr, *part = nullptr, *hpart = nullptr;
+  float *out = nullptr, *dout = nullptr, *part = nullptr, *hpart = nullptr, *bpart = nullptr;
   double* lpart = nullptr;
   size_t part_floats = 0;
   void release() {
@@ -89,10 +89,19 @@ huv_status alloc_t(Acts& a, T** p, size_t count) {
   return alloc(a, (void**)p, count * sizeof(T));
 }
 
-int wgrad_splits(const Conv& c, int N, int H) {
+// The weight gradient reads the NHWC tensors directly where the row width
+// allows 64-pixel segments; otherwise it reads NCHW copies.
+bool wgrad_nhwc(const Conv& c, int W) { return W % 64 == 0 && c.cin % 8 == 0 && c.cout % 8 == 0; }
+
+int wgrad_splits(const Conv& c, int N, int H, int W) {
+  const int rows = N * H;
+  if (wgrad_nhwc(c, W)) {
+    // about 1024 work-groups (four 3-wave groups per CU)
+    const int tiles = ((c.cout + 31) / 32) * ((c.cin + 31) / 32);
+    return std::max(1, std::min((1024 + tiles - 1) / tiles, rows));
+  }
   const int cob = (c.cout + 31) / 32;
   const int tiles = (cob >= 2 ? (cob + 1) / 2 : cob) * ((c.cin + 31) / 32) * 3;
-  const int rows = N * H;
   int s = (1024 + tiles - 1) / tiles;
   s = std::min(s, std::max(1, rows / 4));
   return std::max(1, s);
@@ -125,11 +134,13 @@ huv_status ensure_acts(huv_unet* u, int N, int H, int W) {
   a.pool.resize(L); a.gpool.resize(L); a.cat.resize(L); a.gcat.resize(L);
   a.d1.resize(L); a.d2.resize(L); a.gd1.resize(L); a.gd2.resize(L);
   size_t maxt = (size_t)P * 8;
-  size_t part = 0;
+  size_t part = 0, bpart = 0;
   auto note_conv = [&](const Conv& c, int l) {
-    const int Hl = H >> l;
-    part = std::max(part, (size_t)wgrad_splits(c, N, Hl) * c.cout * 9 * c.cin);
-    part = std::max(part, (size_t)bias_splits(c, N, Hl, W >> l) * c.cout);
+    const int Hl = H >> l, Wl = W >> l;
+    const size_t ws = (size_t)wgrad_splits(c, N, Hl, Wl);
+    part = std::max(part, ws * c.cout * 9 * c.cin);
+    part = std::max(part, (size_t)bias_splits(c, N, Hl, Wl) * c.cout);
+    bpart = std::max(bpart, ws * c.cout);
   };
   for (int l = 0; l < L; ++l) {
     const int64_t Pl = P >> (2 * l);
@@ -152,6 +163,7 @@ huv_status ensure_acts(huv_unet* u, int N, int H, int W) {
   A(a.out, P); A(a.dout, P);
   a.part_floats = std::max(part, (size_t)1);
   A(a.part, a.part_floats);
+  A(a.bpart, std::max(bpart, (size_t)1));
   A(a.hpart, (size_t)kHeadBlocks * (u->base + 1));
   A(a.lpart, kLossBlocks);
 #undef A
@@ -213,9 +225,25 @@ huv_status conv_bwd(huv_unet* u, const Conv& c, const bf16_t* x, const bf16_t* y
                     int l, bf16_t* dx) {
   Acts& a = u->a;
   const int H = a.H >> l, W = a.W >> l;
+  const int splits = wgrad_splits(c, a.N, H, W);
+  if (wgrad_nhwc(c, W)) {
+    if (y) HCHK(huvk::launch_relu_mask(dy, y, a.N, H, W, c.cout, S(u)), "U-Net relu mask");
+    HCHK(huvk::launch_conv3x3_wgrad_nhwc(dy, x, a.N, H, W, c.cin, c.cout, splits, a.part, a.bpart,
+                                         S(u)),
+         "U-Net weight gradient");
+    HCHK(huvk::launch_sum_splits(a.part, splits, (int64_t)c.cout * 9 * c.cin, u->g + c.w_off,
+                                 S(u)),
+         "U-Net split sum");
+    HCHK(huvk::launch_sum_splits(a.bpart, splits, c.cout, u->g + c.b_off, S(u)),
+         "U-Net bias split sum");
+    if (dx)
+      HCHK(huvk::launch_conv3x3_fwd(dy, a.N, H, W, c.cout, wrot(u, c), nullptr, c.cin, 0, dx,
+                                    S(u)),
+           "U-Net data gradient");
+    return HUV_OK;
+  }
   HCHK(huvk::launch_relu_mask_t(dy, y, a.N, H, W, c.cout, a.dyt, S(u)), "U-Net relu mask");
   HCHK(huvk::launch_nhwc_to_nchw(x, a.N, H, W, c.cin, a.xt, S(u)), "U-Net transpose");
-  const int splits = wgrad_splits(c, a.N, H);
   HCHK(huvk::launch_conv3x3_wgrad(a.dyt, a.xt, a.N, H, W, c.cin, c.cout, splits, a.part, S(u)),
        "U-Net weight gradient");
   HCHK(huvk::launch_sum_splits(a.part, splits, (int64_t)c.cout * 9 * c.cin, u->g + c.w_off, S(u)),

@@ -445,6 +445,176 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16_t* __restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Weight and bias gradients read straight from the NHWC tensors.  Work-group
+// = 3 waves on one block of 32 output channels x one block of 32 input
+// channels x a contiguous range of image rows (split K).  The 64-pixel
+// column segments are the outer loop and the rows the inner one: for each
+// row y, dY[y][seg][co] and the input row y + 1 (columns seg - 1 .. seg + 64)
+// are staged in LDS channel-major (one 16-byte word holds 8 consecutive
+// pixels of a channel); the input rows y - 1 and y stay from the previous
+// rows in a ring of three (all three are loaded at a segment's first row and
+// where an image ends).  Wave ky takes the kernel row ky: for every 16
+// pixels, A = dY (32 co x 16 px) and B(kx) = input row y + ky - 1 shifted by
+// kx - 1 pixels (the one- and two-pixel shifts are funnel shifts of the
+// aligned word and the next), acc[kx] += A B, so each wave owns its three
+// taps outright.  Wave 0 of the work-groups of input block 0 also sums dY
+// over the pixels (the bias gradient).  part[split][co][tap][ci],
+// bpart[split][co].  Needs W % 64 == 0, Cin % 8 == 0, Cout % 8 == 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(192) void k_conv3x3_wgrad_nhwc(
+    const bf16_t* __restrict__ dY, const bf16_t* __restrict__ X, int N, int H, int W, int Cin,
+    int Cout, int rows_per_split, float* __restrict__ part, float* __restrict__ bpart) {
+  constexpr int SEG = 64, LP = SEG + 8;  // staged rows: 72 pixels (66 used), 16-byte aligned
+  constexpr int NT = 192;
+  constexpr int DT = SEG * 4, DI = (DT + NT - 1) / NT;            // dY: 64 px x 4 groups
+  constexpr int RT = (SEG + 2) * 4;                               // one input row: 66 px x 4
+  constexpr int XT = 3 * RT;                                      // three input rows
+  __shared__ __align__(16) bf16_t dyT[32][LP];
+  __shared__ __align__(16) bf16_t xT[3][32][LP];  // ring: input row yy in slot yy % 3
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, ky = tid >> 6;
+  const int half = lane >> 5, li = lane & 31;
+  const int ciblocks = (Cin + 31) / 32;
+  const int cib = blockIdx.x % ciblocks, cob = blockIdx.x / ciblocks;
+  const int co0 = 32 * cob, ci0 = 32 * cib;
+  const int split = blockIdx.y;
+  const int rows = N * H;
+  const int r0 = split * rows_per_split, r1 = min(rows, r0 + rows_per_split);
+  const bool bias = bpart != nullptr && cib == 0 && ky == 0;
+  f32x16 acc[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
+  float bsum = 0.f;
+  for (int x0 = 0; x0 < W; x0 += SEG) {
+    for (int row = r0; row < r1; ++row) {
+      const int n = row / H, y = row - n * H;
+      // rows y - 1 .. y + 1 at a segment's first row or an image's first row
+      // (a plain loop), else row y + 1 only (every load of the row issued
+      // before the barrier)
+      const bool all3 = row == r0 || y == 0;
+      const int yfirst = all3 ? y - 1 : y + 1, nrow = all3 ? 3 : 1;
+      auto x_load = [&](int t) -> uint4 {
+        const int r = t / RT, rem = t % RT;
+        const int jj = rem >> 2, cg = rem & 3;
+        const int yy = yfirst + r, xx = x0 + jj - 1;
+        if (r < nrow && yy >= 0 && yy < H && xx >= 0 && xx < W && ci0 + 8 * cg < Cin)
+          return *reinterpret_cast<const uint4*>(X + (((size_t)n * H + yy) * W + xx) * Cin + ci0 +
+                                                 8 * cg);
+        return make_uint4(0, 0, 0, 0);
+      };
+      auto x_store = [&](int t, const B8& v) {
+        const int r = t / RT, rem = t % RT;
+        const int slot = (yfirst + r + 3) % 3;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xT[slot][8 * (rem & 3) + e][rem >> 2] = v.h[e];
+      };
+      B8 dv[DI];
+#pragma unroll
+      for (int k = 0; k < DI; ++k) {
+        const int t = tid + NT * k, px = t >> 2, cg = t & 3;
+        dv[k].u = make_uint4(0, 0, 0, 0);
+        if (t < DT && co0 + 8 * cg < Cout)
+          dv[k].u = *reinterpret_cast<const uint4*>(
+              dY + (((size_t)n * H + y) * W + x0 + px) * Cout + co0 + 8 * cg);
+      }
+      constexpr int X1 = (RT + NT - 1) / NT;
+      B8 xv[X1];
+      if (!all3) {
+#pragma unroll
+        for (int k = 0; k < X1; ++k) {
+          const int t = tid + NT * k;
+          xv[k].u = t < RT ? x_load(t) : make_uint4(0, 0, 0, 0);
+        }
+      }
+      __syncthreads();  // the previous row has been read
+#pragma unroll
+      for (int k = 0; k < DI; ++k) {
+        const int t = tid + NT * k;
+        if (t < DT) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) dyT[8 * (t & 3) + e][t >> 2] = dv[k].h[e];
+        }
+      }
+      if (all3) {
+        for (int t = tid; t < XT; t += NT) {
+          B8 v;
+          v.u = x_load(t);
+          x_store(t, v);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < X1; ++k) {
+          const int t = tid + NT * k;
+          if (t < RT) x_store(t, xv[k]);
+        }
+      }
+      __syncthreads();
+      const int slot = (y + ky + 2) % 3;  // input row y + ky - 1
+#pragma unroll
+      for (int ks = 0; ks < SEG / 16; ++ks) {
+        const int j0 = 16 * ks + 8 * half;  // this lane's 8 pixels
+        const uint4 au = *reinterpret_cast<const uint4*>(&dyT[li][j0]);
+        const bf16x8 a = __builtin_bit_cast(bf16x8, au);
+        if (bias) {
+          B8 ab;
+          ab.u = au;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) bsum += bf2f(ab.h[e]);
+        }
+        const uint4 w = *reinterpret_cast<const uint4*>(&xT[slot][li][j0]);
+        const uint32_t nx = *reinterpret_cast<const uint32_t*>(&xT[slot][li][j0 + 8]);
+        const bf16x8 b0 = __builtin_bit_cast(bf16x8, w);
+        const bf16x8 b1 = __builtin_bit_cast(
+            bf16x8, make_uint4(__builtin_amdgcn_alignbit(w.y, w.x, 16),
+                               __builtin_amdgcn_alignbit(w.z, w.y, 16),
+                               __builtin_amdgcn_alignbit(w.w, w.z, 16),
+                               __builtin_amdgcn_alignbit(nx, w.w, 16)));
+        const bf16x8 b2 = __builtin_bit_cast(bf16x8, make_uint4(w.y, w.z, w.w, nx));
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b2, acc[2], 0, 0, 0);
+      }
+    }
+  }
+  // lane (li, half) holds rows 8 (r / 4) + 4 half + r % 4 (output channels) of
+  // column li (input channel) of its wave's three taps
+  const int ci = ci0 + li;
+#pragma unroll
+  for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int co = co0 + 8 * (r / 4) + 4 * half + (r % 4);
+      if (co < Cout && ci < Cin)
+        part[(((size_t)split * Cout + co) * 9 + 3 * ky + kx) * Cin + ci] = acc[kx][r];
+    }
+  if (bias) {
+    bsum += __shfl_xor(bsum, 32);  // the two pixel halves of channel li
+    if (half == 0 && co0 + li < Cout) bpart[(size_t)split * Cout + co0 + li] = bsum;
+  }
+}
+
+// dY *= (Y > 0), in place (NHWC, 8 channels per thread).
+__global__ void k_relu_mask(bf16_t* __restrict__ dY, const bf16_t* __restrict__ Y,
+                            int64_t count8) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count8;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    B8 v, yv;
+    v.u = reinterpret_cast<const uint4*>(dY)[i];
+    yv.u = reinterpret_cast<const uint4*>(Y)[i];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (bf2f(yv.h[k]) <= 0.f) {
+        v.h[k] = 0;
+        any = true;
+      }
+    if (any) reinterpret_cast<uint4*>(dY)[i] = v.u;
+  }
+}
+
 // out[i] = sum over k of part[k][i]: 32 outputs per work-group, the splits
 // dealt round-robin over 32 groups of 32 lanes (each summed in increasing k),
 // then the 32 group sums added in group order, so the result does not depend
@@ -807,6 +977,26 @@ hipError_t launch_conv3x3_wgrad(const bf16_t* dYT, const bf16_t* XT, int N, int 
     hipLaunchKernelGGL(k_conv3x3_wgrad<1>, dim3(tiles, splits), dim3(256), 0, s, dYT, XT, N, H,
                        W, Cin, Cout, rps, part);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_conv3x3_wgrad_nhwc(const bf16_t* dY, const bf16_t* X, int N, int H, int W,
+                                     int Cin, int Cout, int splits, float* part, float* bpart,
+                                     hipStream_t s) {
+  if (W % 64 != 0 || Cin % 8 != 0 || Cout % 8 != 0 || splits < 1 || splits > N * H)
+    return hipErrorInvalidValue;
+  const int rps = (N * H + splits - 1) / splits;
+  const int blocks = ((Cout + 31) / 32) * ((Cin + 31) / 32);
+  hipLaunchKernelGGL(k_conv3x3_wgrad_nhwc, dim3(blocks, splits), dim3(192), 0, s, dY, X, N, H, W,
+                     Cin, Cout, rps, part, bpart);
+  return hipGetLastError();
+}
+
+hipError_t launch_relu_mask(bf16_t* dY, const bf16_t* Y, int N, int H, int W, int C,
+                            hipStream_t s) {
+  if (C % 8 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_relu_mask, grid1d((int64_t)N * H * W * C / 8), dim3(256), 0, s, dY, Y,
+                     (int64_t)N * H * W * C / 8);
   return hipGetLastError();
 }
 

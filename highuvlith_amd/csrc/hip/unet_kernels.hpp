@@ -28,6 +28,16 @@ hipError_t launch_relu_mask_t(bf16_t* dY, const bf16_t* Y, int N, int H, int W, 
 // Cin % 8 == 0; rows (n, y) are split into `splits` contiguous ranges.
 hipError_t launch_conv3x3_wgrad(const bf16_t* dYT, const bf16_t* XT, int N, int H, int W,
                                 int Cin, int Cout, int splits, float* part, hipStream_t s);
+// Weight gradient partials part[split][co][t][ci] and, when bpart is set, bias
+// gradient partials bpart[split][co] straight from the NHWC tensors dY (ReLU
+// already applied) and X; W % 64 == 0, Cin and Cout multiples of 8; rows
+// (n, y) are split into `splits` contiguous ranges.
+hipError_t launch_conv3x3_wgrad_nhwc(const bf16_t* dY, const bf16_t* X, int N, int H, int W,
+                                     int Cin, int Cout, int splits, float* part, float* bpart,
+                                     hipStream_t s);
+// dY *= (Y > 0) in place (NHWC, C % 8 == 0).
+hipError_t launch_relu_mask(bf16_t* dY, const bf16_t* Y, int N, int H, int W, int C,
+                            hipStream_t s);
 // out[i] = sum_s part[s][i] (fixed order), i < count.
 hipError_t launch_sum_splits(const float* part, int splits, int64_t count, float* out,
                              hipStream_t s);
