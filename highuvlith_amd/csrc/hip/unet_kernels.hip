@@ -11,9 +11,11 @@
 //   data gradient: the forward kernel on the ReLU-masked output gradient
 //             with the rotated, transposed weights.
 //   weight gradient: rows = output channels, columns = input channels of
-//             one tap, k = pixels (16 consecutive x of one row), from NCHW
-//             copies; the +-1 column shift of a tap is a 2-byte funnel shift
-//             of aligned loads.  Split-K partials are summed in a fixed order.
+//             one tap, k = pixels (16 consecutive x of one row), staged
+//             channel-major in LDS from the NHWC tensors (rows a multiple of
+//             64 wide) or read from NCHW copies (narrower rows); the +-1
+//             column shift of a tap is a 2-byte funnel shift of aligned
+//             words.  Split-K partials are summed in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include "unet_kernels.hpp"
