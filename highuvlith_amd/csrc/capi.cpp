@@ -228,6 +228,7 @@ Ctx::~Ctx() {
   for (auto& kv : graphs) drop_graph(kv.second);
   if (pin_h) hipHostFree(pin_h);
   if (pin_d) hipFree(pin_d);
+  if (dl_h) hipHostFree(dl_h);
   if (own_stream) hipStreamDestroy(own_stream);
 }
 
@@ -249,6 +250,23 @@ bool Ctx::staging(size_t bytes, hipError_t& e) {
     return false;
   }
   pin_cap = bytes;
+  return true;
+}
+
+bool Ctx::download_area(size_t bytes, hipError_t& e) {
+  e = hipSuccess;
+  if (bytes <= dl_cap) return true;
+  e = hipStreamSynchronize(stream());
+  if (e != hipSuccess) return false;
+  if (dl_h) hipHostFree(dl_h);
+  dl_h = nullptr;
+  dl_cap = 0;
+  e = hipHostMalloc((void**)&dl_h, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    dl_h = nullptr;
+    return false;
+  }
+  dl_cap = bytes;
   return true;
 }
 

@@ -433,11 +433,9 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   float* dir = (float*)(vec + 2 * vb);
   const int nblk = 64;
   constexpr int kRed = 3;  // reductions fetched together
-  char* sc = (char*)c->ws.get(
-      Workspace::ILT7, ((size_t)kRed * T * nblk + (size_t)kRed * T) * sizeof(double), e);
+  char* sc = (char*)c->ws.get(Workspace::ILT7, (size_t)kRed * T * nblk * sizeof(double), e);
   if (!sc) return hip_fail(e, "ILT optimizer workspace");
   double* part = (double*)sc;
-  double* red = part + (size_t)kRed * T * nblk;
 
   auto fail_e = [&](const char* what) { return hip_fail(e, what); };
   std::vector<double> h_red((size_t)kRed * T), h_a(T), h_b(T);
@@ -446,6 +444,10 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   // them; the area is reused after every stream synchronisation.
   const size_t stage_cap = 96 * (size_t)T + 4096;
   if (!c->staging(stage_cap, e)) return fail_e("staging buffer");
+  // Downloads (reduction partials, costs) land in pinned host memory.
+  if (!c->download_area((size_t)kRed * T * nblk * sizeof(double), e))
+    return fail_e("download buffer");
+  double* dl = (double*)c->dl_h;
   size_t so = 0, sf = 0;
   auto flush = [&]() -> bool {
     if (so > sf) {
@@ -456,6 +458,15 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     return true;
   };
   auto synced = [&]() { so = sf = 0; };  // after a stream synchronisation
+  // costs of `count` evaluation rows, after one synchronisation
+  auto get_costs = [&](int count, double* dst) -> bool {
+    e = hipMemcpyAsync(dl, b.cost_d, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    synced();
+    if (e != hipSuccess) return false;
+    std::memcpy(dst, dl, (size_t)count * sizeof(double));
+    return true;
+  };
   // One synchronisation epoch stages at most ~32 T bytes plus alignment.
   auto put = [&](const void* src, size_t bytes) -> void* {
     const size_t a = (so + 15) & ~(size_t)15;
@@ -474,7 +485,9 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   };
   // Up to kRed reductions of two vectors a, b over `rows` in one pass (op codes of
   // launch_vec_reduce: 0 a.a, 1 a.b, 2 b.b, 3 max |a|), one synchronisation;
-  // h_red[r * T + t] holds reduction r for each listed t.
+  // h_red[r * T + t] holds reduction r for each listed t: the device writes
+  // per-block partials and the host adds them in block order (the order the
+  // device's own finishing kernel uses), which saves a launch per reduction.
   enum { AA = 0, AB = 1, BB = 2, MAXA = 3 };
   auto reduce = [&](const float* a, const float* bv, std::initializer_list<int> ops,
                     const std::vector<int>& rows) -> bool {
@@ -483,13 +496,25 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     if (e != hipSuccess || !flush()) return false;
     int code = 0, r = 0;
     for (int op : ops) code |= op << (4 * r++);
-    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, part, red, s);
+    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, part, nullptr,
+                                s);
     if (e != hipSuccess) return false;
-    e = hipMemcpyAsync(h_red.data(), red, (size_t)r * T * sizeof(double), hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(dl, part, (size_t)r * T * nblk * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return false;
     e = hipStreamSynchronize(s);
     synced();
-    return e == hipSuccess;
+    if (e != hipSuccess) return false;
+    int q = 0;
+    for (int op : ops) {
+      for (int t : rows) {
+        const double* pk = dl + ((size_t)q * T + t) * nblk;
+        double acc = 0.0;
+        for (int k = 0; k < nblk; ++k) acc = op == MAXA ? std::fmax(acc, pk[k]) : acc + pk[k];
+        h_red[(size_t)q * T + t] = acc;
+      }
+      ++q;
+    }
+    return true;
   };
   // d = a g + b d on the listed rows (a, b from h_a, h_b)
   auto dir_rows = [&](const float* g, const std::vector<int>& rows) -> bool {
@@ -520,10 +545,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       src.theta = th;
       huv_status r = ilt_eval(p, T, targets, nullptr, src, g, nullptr, b);
       if (r != HUV_OK) return r;
-      e = hipMemcpyAsync(out.data(), b.cost_d, T * sizeof(double), hipMemcpyDeviceToHost, s);
-      if (e == hipSuccess) e = hipStreamSynchronize(s);
-      synced();
-      return e == hipSuccess ? HUV_OK : hip_fail(e, "cost copy");
+      return get_costs(T, out.data()) ? HUV_OK : hip_fail(e, "cost copy");
     }
     const int* idx = stage_rows(which);
     if (!idx || !flush()) return hip_fail(e, "row staging");
@@ -533,10 +555,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     src.stage = th_c;
     huv_status r = ilt_eval(p, Tc, targets, idx, src, g, idx, b);
     if (r != HUV_OK) return r;
-    e = hipMemcpyAsync(cost_c.data(), b.cost_d, Tc * sizeof(double), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    synced();
-    if (e != hipSuccess) return hip_fail(e, "cost copy");
+    if (!get_costs(Tc, cost_c.data())) return hip_fail(e, "cost copy");
     for (int j = 0; j < Tc; ++j) out[which[j]] = cost_c[j];
     return HUV_OK;
   };
@@ -660,10 +679,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         trial.stage = th_c;
         st = ilt_eval(p, np, targets, prow_d, trial, nullptr, nullptr, b);
         if (st != HUV_OK) return st;
-        e = hipMemcpyAsync(cost_c.data(), b.cost_d, np * sizeof(double), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        synced();
-        if (e != hipSuccess) return fail_e("cost copy");
+        if (!get_costs(np, cost_c.data())) return fail_e("cost copy");
         acc_src.clear();
         acc_dst.clear();
         for (int t : which) {

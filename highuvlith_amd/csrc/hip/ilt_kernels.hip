@@ -901,8 +901,9 @@ hipError_t launch_vec_reduce(const float* a, const float* b, int64_t npix, int c
     hipLaunchKernelGGL(k_vec_reduce<2>, grid, dim3(PW_THREADS), 0, s, a, b, npix, nblk, ops, T, rows, part);
   else
     hipLaunchKernelGGL(k_vec_reduce<3>, grid, dim3(PW_THREADS), 0, s, a, b, npix, nblk, ops, T, rows, part);
-  hipLaunchKernelGGL(k_vec_finish, dim3((count + 63) / 64, nops), dim3(64), 0, s, part, count,
-                     nblk, ops, nops, T, rows, out);
+  if (out)
+    hipLaunchKernelGGL(k_vec_finish, dim3((count + 63) / 64, nops), dim3(64), 0, s, part, count,
+                       nblk, ops, nops, T, rows, out);
   return hipGetLastError();
 }
 

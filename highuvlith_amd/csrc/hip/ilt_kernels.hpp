@@ -56,7 +56,8 @@ hipError_t launch_ilt_chain(const float* dcdm, const float* reg, const float* m,
 // Up to nops <= 3 per-target reductions of a and b in one pass, op r in bits
 // 4r..4r+3 of ops: 0 a.a, 1 a.b, 2 b.b, 3 max |a| (b may be null for 0 and
 // 3).  part: nops x T x nblk; out[r * T + t] for the listed t (T: the stride,
-// at least count).
+// at least count); out null: the partials only (summed by the caller, over
+// the blocks in order).
 hipError_t launch_vec_reduce(const float* a, const float* b, int64_t npix, int count,
                              const int* rows, int nblk, int ops, int nops, int T, double* part,
                              double* out, hipStream_t s);
