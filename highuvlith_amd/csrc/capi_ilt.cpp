@@ -30,6 +30,7 @@ struct huv_ilt {
   float filter_c = 0.0f;     // filter exponent per squared bin index, 2 pi^2 s^2 df^2
   float* dose_d = nullptr;
   float* weight_d = nullptr;
+  std::vector<float> weight_h;  // host copy of the condition weights
   ~huv_ilt() {
     if (dose_d) hipFree(dose_d);
     if (weight_d) hipFree(weight_d);
@@ -181,11 +182,13 @@ huv_status ilt_mask_t(huv_ilt* p, int count, const ThetaRows& src, float* m, voi
   return e == hipSuccess ? HUV_OK : hip_fail(e, "ilt mask launch");
 }
 
-// Cost of T evaluation rows (into b.cost_d, device) and optionally the
-// theta-gradient.  Row j compares with target row trows[j] (null: j) and its
-// gradient goes to grad row grows[j] (null: j).
+// Cost of T evaluation rows (into b.cost_d, device; device_cost false: only
+// the per-row partial sums b.fid_part and b.reg_part, for ilt_host_costs) and
+// optionally the theta-gradient.  Row j compares with target row trows[j]
+// (null: j) and its gradient goes to grad row grows[j] (null: j).
 huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const int* trows,
-                    const ThetaRows& src, float* grad, const int* grows, IltBuffers& b) {
+                    const ThetaRows& src, float* grad, const int* grows, IltBuffers& b,
+                    bool device_cost = true) {
   Ctx* c = p->ctx;
   hipStream_t s = c->stream();
   hipError_t e;
@@ -225,9 +228,11 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const int* trows,
                              (float)p->cfg.bin_weight, b.reg_grad, b.reg_part, s);
     if (e != hipSuccess) return hip_fail(e, "ilt reg launch");
   }
-  e = huvk::launch_ilt_cost_reduce(b.fid_part, reg ? b.reg_part : nullptr, T, C, n / 32, kBpi,
-                                   p->weight_d, 1.0 / (double)N, b.cost_d, s);
-  if (e != hipSuccess) return hip_fail(e, "ilt cost reduce launch");
+  if (device_cost) {
+    e = huvk::launch_ilt_cost_reduce(b.fid_part, reg ? b.reg_part : nullptr, T, C, n / 32, kBpi,
+                                     p->weight_d, 1.0 / (double)N, b.cost_d, s);
+    if (e != hipSuccess) return hip_fail(e, "ilt cost reduce launch");
+  }
   c->stage_end(ST_ILT_POINTWISE, ev);
   if (!grad) return HUV_OK;
   if (p->proxy) {
@@ -293,6 +298,28 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const int* trows,
   }
   c->stage_end(ST_ILT_OTHER, ev);
   return HUV_OK;
+}
+
+// Costs of `count` evaluation rows from their partial sums (fid: count x ncond
+// x bpi, reg: count x kBpi or null), with the arithmetic of
+// k_ilt_cost_reduce: per condition the partials in order, weighted and scaled
+// with one fused multiply-add, then the regularisation partials in order.
+void ilt_host_costs(const huv_ilt* p, int count, const double* fid, const double* reg,
+                    double* cost) {
+  const int C = p->ncond, bpi = p->n / 32;
+  const double inv_n = 1.0 / ((double)p->n * p->n);
+  for (int t = 0; t < count; ++t) {
+    double total = 0.0;
+    for (int c = 0; c < C; ++c) {
+      const double* q = fid + ((size_t)t * C + c) * bpi;
+      double s = 0.0;
+      for (int b = 0; b < bpi; ++b) s += q[b];
+      total = std::fma((double)p->weight_h[c] * s, inv_n, total);
+    }
+    if (reg)
+      for (int b = 0; b < kBpi; ++b) total += reg[(size_t)t * kBpi + b];
+    cost[t] = total;
+  }
 }
 
 }  // namespace
@@ -361,6 +388,7 @@ huv_status huv_ilt_create(huv_ctx* c, int n, int ncond, huv_kset* const* conds, 
   p->Bf = Bf;
   p->filter_c = filter_c;
   std::vector<float> d(dose, dose + ncond), w(weight, weight + ncond);
+  p->weight_h = w;
   hipError_t e = hipMalloc(&p->dose_d, ncond * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&p->weight_d, ncond * sizeof(float));
   if (e == hipSuccess) e = hipMemcpy(p->dose_d, d.data(), ncond * sizeof(float), hipMemcpyHostToDevice);
@@ -445,8 +473,9 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   const size_t stage_cap = 96 * (size_t)T + 4096;
   if (!c->staging(stage_cap, e)) return fail_e("staging buffer");
   // Downloads (reduction partials, costs) land in pinned host memory.
-  if (!c->download_area((size_t)kRed * T * nblk * sizeof(double), e))
-    return fail_e("download buffer");
+  const size_t dl_count = std::max((size_t)kRed * T * nblk,
+                                   (size_t)T * (p->ncond * (p->n / 32) + kBpi));
+  if (!c->download_area(dl_count * sizeof(double), e)) return fail_e("download buffer");
   double* dl = (double*)c->dl_h;
   size_t so = 0, sf = 0;
   auto flush = [&]() -> bool {
@@ -458,13 +487,19 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     return true;
   };
   auto synced = [&]() { so = sf = 0; };  // after a stream synchronisation
-  // costs of `count` evaluation rows, after one synchronisation
+  // costs of `count` evaluation rows (ilt_eval with device_cost false) from
+  // their partial sums, after one synchronisation
+  const bool has_reg = p->cfg.tv_weight > 0.0 || p->cfg.bin_weight > 0.0;
+  const size_t fid_n = (size_t)p->ncond * (p->n / 32);  // fidelity partials per row
   auto get_costs = [&](int count, double* dst) -> bool {
-    e = hipMemcpyAsync(dl, b.cost_d, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, s);
+    const size_t nf = (size_t)count * fid_n, nr = has_reg ? (size_t)count * kBpi : 0;
+    e = hipMemcpyAsync(dl, b.fid_part, nf * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && nr)
+      e = hipMemcpyAsync(dl + nf, b.reg_part, nr * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     synced();
     if (e != hipSuccess) return false;
-    std::memcpy(dst, dl, (size_t)count * sizeof(double));
+    ilt_host_costs(p, count, dl, nr ? dl + nf : nullptr, dst);
     return true;
   };
   // One synchronisation epoch stages at most ~32 T bytes plus alignment.
@@ -543,7 +578,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       if (!flush()) return hip_fail(e, "staging copy");
       ThetaRows src;
       src.theta = th;
-      huv_status r = ilt_eval(p, T, targets, nullptr, src, g, nullptr, b);
+      huv_status r = ilt_eval(p, T, targets, nullptr, src, g, nullptr, b, false);
       if (r != HUV_OK) return r;
       return get_costs(T, out.data()) ? HUV_OK : hip_fail(e, "cost copy");
     }
@@ -553,7 +588,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     src.theta = th;
     src.rows = idx;
     src.stage = th_c;
-    huv_status r = ilt_eval(p, Tc, targets, idx, src, g, idx, b);
+    huv_status r = ilt_eval(p, Tc, targets, idx, src, g, idx, b, false);
     if (r != HUV_OK) return r;
     if (!get_costs(Tc, cost_c.data())) return hip_fail(e, "cost copy");
     for (int j = 0; j < Tc; ++j) out[which[j]] = cost_c[j];
@@ -677,7 +712,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         trial.dir = dir;
         trial.coef = pcoef_d;
         trial.stage = th_c;
-        st = ilt_eval(p, np, targets, prow_d, trial, nullptr, nullptr, b);
+        st = ilt_eval(p, np, targets, prow_d, trial, nullptr, nullptr, b, false);
         if (st != HUV_OK) return st;
         if (!get_costs(np, cost_c.data())) return fail_e("cost copy");
         acc_src.clear();
