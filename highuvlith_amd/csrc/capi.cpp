@@ -259,11 +259,17 @@ bool Ctx::download_area(size_t bytes, hipError_t& e) {
   e = hipStreamSynchronize(stream());
   if (e != hipSuccess) return false;
   if (dl_h) hipHostFree(dl_h);
-  dl_h = nullptr;
+  dl_h = dl_d = nullptr;
   dl_cap = 0;
-  e = hipHostMalloc((void**)&dl_h, bytes, hipHostMallocDefault);
+  e = hipHostMalloc((void**)&dl_h, bytes, hipHostMallocMapped);
   if (e != hipSuccess) {
     dl_h = nullptr;
+    return false;
+  }
+  e = hipHostGetDevicePointer((void**)&dl_d, dl_h, 0);
+  if (e != hipSuccess) {
+    hipHostFree(dl_h);
+    dl_h = dl_d = nullptr;
     return false;
   }
   dl_cap = bytes;

@@ -461,9 +461,6 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   float* dir = (float*)(vec + 2 * vb);
   const int nblk = 64;
   constexpr int kRed = 3;  // reductions fetched together
-  char* sc = (char*)c->ws.get(Workspace::ILT7, (size_t)kRed * T * nblk * sizeof(double), e);
-  if (!sc) return hip_fail(e, "ILT optimizer workspace");
-  double* part = (double*)sc;
 
   auto fail_e = [&](const char* what) { return hip_fail(e, what); };
   std::vector<double> h_red((size_t)kRed * T), h_a(T), h_b(T);
@@ -472,11 +469,18 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   // them; the area is reused after every stream synchronisation.
   const size_t stage_cap = 96 * (size_t)T + 4096;
   if (!c->staging(stage_cap, e)) return fail_e("staging buffer");
-  // Downloads (reduction partials, costs) land in pinned host memory.
-  const size_t dl_count = std::max((size_t)kRed * T * nblk,
-                                   (size_t)T * (p->ncond * (p->n / 32) + kBpi));
+  // Reduction and cost partials are written by the kernels straight into
+  // pinned, device-mapped host memory (no copy launch): the host reads them
+  // after the synchronisation that ends the launch.  One area serves both;
+  // every use is read before the next launch that writes it.
+  const bool has_reg = p->cfg.tv_weight > 0.0 || p->cfg.bin_weight > 0.0;
+  const size_t fid_n = (size_t)p->ncond * (p->n / 32);  // fidelity partials per row
+  const size_t dl_count = std::max((size_t)kRed * T * nblk, (size_t)T * (fid_n + kBpi));
   if (!c->download_area(dl_count * sizeof(double), e)) return fail_e("download buffer");
   double* dl = (double*)c->dl_h;
+  double* dl_dev = (double*)c->dl_d;
+  b.fid_part = dl_dev;
+  b.reg_part = dl_dev + (size_t)T * fid_n;
   size_t so = 0, sf = 0;
   auto flush = [&]() -> bool {
     if (so > sf) {
@@ -489,17 +493,11 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   auto synced = [&]() { so = sf = 0; };  // after a stream synchronisation
   // costs of `count` evaluation rows (ilt_eval with device_cost false) from
   // their partial sums, after one synchronisation
-  const bool has_reg = p->cfg.tv_weight > 0.0 || p->cfg.bin_weight > 0.0;
-  const size_t fid_n = (size_t)p->ncond * (p->n / 32);  // fidelity partials per row
   auto get_costs = [&](int count, double* dst) -> bool {
-    const size_t nf = (size_t)count * fid_n, nr = has_reg ? (size_t)count * kBpi : 0;
-    e = hipMemcpyAsync(dl, b.fid_part, nf * sizeof(double), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && nr)
-      e = hipMemcpyAsync(dl + nf, b.reg_part, nr * sizeof(double), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    e = hipStreamSynchronize(s);
     synced();
     if (e != hipSuccess) return false;
-    ilt_host_costs(p, count, dl, nr ? dl + nf : nullptr, dst);
+    ilt_host_costs(p, count, dl, has_reg ? dl + (size_t)T * fid_n : nullptr, dst);
     return true;
   };
   // One synchronisation epoch stages at most ~32 T bytes plus alignment.
@@ -531,10 +529,8 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     if (e != hipSuccess || !flush()) return false;
     int code = 0, r = 0;
     for (int op : ops) code |= op << (4 * r++);
-    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, part, nullptr,
-                                s);
-    if (e != hipSuccess) return false;
-    e = hipMemcpyAsync(dl, part, (size_t)r * T * nblk * sizeof(double), hipMemcpyDeviceToHost, s);
+    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, dl_dev,
+                                nullptr, s);
     if (e != hipSuccess) return false;
     e = hipStreamSynchronize(s);
     synced();

@@ -76,9 +76,11 @@ struct Ctx {
   char* pin_d = nullptr;
   size_t pin_cap = 0;
   bool staging(size_t bytes, hipError_t& e);
-  // Pinned host buffer for small per-step downloads (costs, reduction
-  // partials of the ILT driver), grown on demand.
+  // Pinned, device-mapped host buffer for small per-step downloads (the ILT
+  // driver's reduction and cost partials, which its kernels write there
+  // directly through dl_d), grown on demand.
   char* dl_h = nullptr;
+  char* dl_d = nullptr;
   size_t dl_cap = 0;
   bool download_area(size_t bytes, hipError_t& e);
   void drop_graph(GraphEntry& g);
