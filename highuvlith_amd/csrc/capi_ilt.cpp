@@ -235,6 +235,8 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const int* trows,
   }
   c->stage_end(ST_ILT_POINTWISE, ev);
   if (!grad) return HUV_OK;
+  const bool filt = p->Bf > 0;
+  bool chained = false;  // the chain rule ran in the gradient image stage
   if (p->proxy) {
     // local proxy: dC/dm = sum_c w_c G0_c (ilt.rs:719-721)
     c->stage_begin(ST_ILT_OTHER, &ev);
@@ -277,16 +279,30 @@ huv_status ilt_eval(huv_ilt* p, int T, const float* targets, const int* trows,
     if (e != hipSuccess) return hip_fail(e, "ilt hband launch");
     const float2* tw_n = c->twiddles(n, e);
     if (!tw_n) return hip_fail(e, "twiddle table");
-    e = huvk::launch_band_image(n, B, T, b.H, tw_n, b.dcdm, s);
+    if (filt) {
+      e = huvk::launch_band_image(n, B, T, b.H, tw_n, b.dcdm, s);
+    } else {
+      // the chain rule where the image of dC/dm would be stored: the
+      // gradient rows are written directly
+      huvk::ChainEpi ch;
+      ch.m = b.m;
+      ch.reg = reg ? b.reg_grad : nullptr;
+      ch.orows = grows;
+      ch.beta = (float)p->cfg.beta;
+      ch.grad = grad;
+      e = huvk::launch_band_image_chain(n, B, T, b.H, tw_n, ch, s);
+      chained = true;
+    }
     if (e != hipSuccess) return hip_fail(e, "ilt band image launch");
     c->stage_end(ST_ILT_OTHER, ev);
   }
   // chain through the sigmoid, then the (self-adjoint) density filter
   c->stage_begin(ST_ILT_OTHER, &ev);
-  const bool filt = p->Bf > 0;
-  e = huvk::launch_ilt_chain(b.dcdm, reg ? b.reg_grad : nullptr, b.m, (float)p->cfg.beta, N, T,
-                             filt ? nullptr : grows, filt ? b.gm : grad, s);
-  if (e != hipSuccess) return hip_fail(e, "ilt chain launch");
+  if (!chained) {
+    e = huvk::launch_ilt_chain(b.dcdm, reg ? b.reg_grad : nullptr, b.m, (float)p->cfg.beta, N, T,
+                               filt ? nullptr : grows, filt ? b.gm : grad, s);
+    if (e != hipSuccess) return hip_fail(e, "ilt chain launch");
+  }
   if (filt) {
     // the filter writes compact rows; those of a row subset go out through thf
     st = ilt_filter(p, T, b.gm, grows ? b.thf : grad, b);

@@ -820,14 +820,55 @@ __device__ __forceinline__ double fidelity_task(const FidelityEpi& e, float dose
   return acc;
 }
 
-// FID: instead of the image, the ILT fidelity of every pixel against the
-// target (epi): per-work-group FP64 partial sums of r^2 at
-// epi.partial[img][p] and, when epi.G is set, G = dC/dI written where the
-// image would be; image img is (target img / ncond, condition img % ncond).
-template <int TMIN, int TMAX, bool FID>
+// The ILT chain rule on one band_image task's 64 pixels (rows a and b at
+// byte offsets va, vb, column offsets C i): the image value there is dC/dm,
+// and dC/dtheta = (dC/dm + reg) beta m (1 - m) is written in its place, with
+// the arithmetic of the separate chain kernel.
+template <bool REG>
+__device__ __forceinline__ void chain_task(float beta, const c32* d, int C, int va, int vb,
+                                           __amdgpu_buffer_rsrc_t mrc,
+                                           __amdgpu_buffer_rsrc_t rrc,
+                                           __amdgpu_buffer_rsrc_t grc) {
+#pragma unroll
+  for (int i0 = 0; i0 < 32; i0 += 8) {
+    __builtin_amdgcn_sched_barrier(0);
+    float ma[8], mb[8], ra[8], rb[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int so = C * (i0 + k) * (int)sizeof(float);
+      ma[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(mrc, va, so, 0));
+      mb[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(mrc, vb, so, 0));
+      if constexpr (REG) {
+        ra[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rrc, va, so, 0));
+        rb[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rrc, vb, so, 0));
+      } else {
+        ra[k] = rb[k] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int so = C * (i0 + k) * (int)sizeof(float);
+      const float ga = (d[i0 + k].re + ra[k]) * beta * ma[k] * (1.0f - ma[k]);
+      const float gb = (d[i0 + k].im + rb[k]) * beta * mb[k] * (1.0f - mb[k]);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, ga), grc, va, so, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, gb), grc, vb, so, 0);
+    }
+  }
+}
+
+// MODE 0: the image.  MODE 1 (fidelity): instead of the image, the ILT
+// fidelity of every pixel against the target (epi): per-work-group FP64
+// partial sums of r^2 at epi.partial[img][p] and, when epi.G is set,
+// G = dC/dI written where the image would be; image img is (target
+// img / ncond, condition img % ncond).  MODE 2 (chain): the image is dC/dm of
+// ILT row img, and dC/dtheta goes to row (ch.orows ? ch.orows[img] : img) of
+// ch.grad (chain_task).
+template <int TMIN, int TMAX, int MODE>
 __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ Ib, int n, int G,
                                                     int nimg, const float2* __restrict__ tw_n,
-                                                    float* __restrict__ out, FidelityEpi epi) {
+                                                    float* __restrict__ out, FidelityEpi epi,
+                                                    ChainEpi ch) {
+  constexpr bool FID = MODE == 1, CHAIN = MODE == 2;
   constexpr int F = 32, NT = TMAX - TMIN + 1;
   extern __shared__ c32 smem[];
   constexpr int UP = 32 * (-TMIN) + 1;  // U row: g = 0 .. 32 |TMIN|, zero beyond G
@@ -946,9 +987,16 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     __syncthreads();
   }
   // ---- phase 2 ----
-  float* dst = FID ? epi.G : out;
+  float* dst = FID ? epi.G : (CHAIN ? ch.grad : out);
+  const int orow = CHAIN && ch.orows ? ch.orows[img] : img;
   const __amdgpu_buffer_rsrc_t orc = __builtin_amdgcn_make_buffer_rsrc(
-      dst ? dst + (size_t)img * n * n : nullptr, (short)0, n * n * (int)sizeof(float), 0x00020000);
+      dst ? dst + (size_t)orow * n * n : nullptr, (short)0, n * n * (int)sizeof(float), 0x00020000);
+  const __amdgpu_buffer_rsrc_t mrc = __builtin_amdgcn_make_buffer_rsrc(
+      CHAIN ? const_cast<float*>(ch.m) + (size_t)img * n * n : nullptr, (short)0,
+      n * n * (int)sizeof(float), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rrc = __builtin_amdgcn_make_buffer_rsrc(
+      CHAIN && ch.reg ? const_cast<float*>(ch.reg) + (size_t)img * n * n : nullptr, (short)0,
+      n * n * (int)sizeof(float), 0x00020000);
   const int tgt = FID ? img / epi.ncond : 0;
   const float dose = FID ? epi.dose[img % epi.ncond] : 1.0f;
   const __amdgpu_buffer_rsrc_t zrc = __builtin_amdgcn_make_buffer_rsrc(
@@ -1009,7 +1057,10 @@ __global__ __launch_bounds__(256) void k_band_image(const float2* __restrict__ I
     // inputs from L2 and the MALL.
     const int va = ((P * (2 * pr) + p) * n + c) * (int)sizeof(float);
     const int vb = va + P * n * (int)sizeof(float);
-    if constexpr (!FID) {
+    if constexpr (CHAIN) {
+      if (ch.reg) chain_task<true>(ch.beta, d, C, va, vb, mrc, rrc, orc);
+      else chain_task<false>(ch.beta, d, C, va, vb, mrc, rrc, orc);
+    } else if constexpr (!FID) {
 #pragma unroll
       for (int i = 0; i < F; ++i) {
         const int so = C * i * (int)sizeof(float);
@@ -1304,44 +1355,52 @@ hipError_t launch_coarse_cols(int m, int B, int nimg, float flare, const float2*
   }
 }
 
-template <int TMIN, int TMAX, bool FID>
+template <int TMIN, int TMAX, int MODE>
 static hipError_t band_image_t(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
-                               float* out, const FidelityEpi& epi, hipStream_t s) {
+                               float* out, const FidelityEpi& epi, const ChainEpi& ch,
+                               hipStream_t s) {
   constexpr int UP = 32 * (-TMIN) + 1, NT = TMAX - TMIN + 1;
   auto bytes = [](int nn) { return ((size_t)32 * UP + (size_t)(32 + NT) * (nn / 32)) * sizeof(c32); };
   const size_t shmem = bytes(n);
-  auto kern = k_band_image<TMIN, TMAX, FID>;
+  auto kern = k_band_image<TMIN, TMAX, MODE>;
   // Sized for the largest grid (n = 2048), once.
   static const hipError_t attr = hipFuncSetAttribute(
       (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes(2048));
   if (attr != hipSuccess) return attr;
   const int groups = (nimg + kXcds - 1) / kXcds;
   HUV_LAUNCH(kern, dim3(groups * kXcds * (n / 32)), dim3(256), shmem, s, Ib, n, G, nimg, tw_n,
-             out, epi);
+             out, epi, ch);
 }
 
-template <bool FID>
+template <int MODE>
 static hipError_t band_image_g(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
-                               float* out, const FidelityEpi& epi, hipStream_t s) {
+                               float* out, const FidelityEpi& epi, const ChainEpi& ch,
+                               hipStream_t s) {
   if (n < 256 || n > 2048 || (n & (n - 1)) != 0 || G < 0 || G > 255 || 2 * G + 1 > n)
     return hipErrorInvalidValue;
-  if (G <= 31) return band_image_t<-1, 0, FID>(n, G, nimg, Ib, tw_n, out, epi, s);
-  if (G <= 63) return band_image_t<-2, 1, FID>(n, G, nimg, Ib, tw_n, out, epi, s);
-  if (G <= 95) return band_image_t<-3, 2, FID>(n, G, nimg, Ib, tw_n, out, epi, s);
-  if (G <= 127) return band_image_t<-4, 3, FID>(n, G, nimg, Ib, tw_n, out, epi, s);
+  if (G <= 31) return band_image_t<-1, 0, MODE>(n, G, nimg, Ib, tw_n, out, epi, ch, s);
+  if (G <= 63) return band_image_t<-2, 1, MODE>(n, G, nimg, Ib, tw_n, out, epi, ch, s);
+  if (G <= 95) return band_image_t<-3, 2, MODE>(n, G, nimg, Ib, tw_n, out, epi, ch, s);
+  if (G <= 127) return band_image_t<-4, 3, MODE>(n, G, nimg, Ib, tw_n, out, epi, ch, s);
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_band_image(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
                              float* out, hipStream_t s) {
-  return band_image_g<false>(n, G, nimg, Ib, tw_n, out, FidelityEpi{}, s);
+  return band_image_g<0>(n, G, nimg, Ib, tw_n, out, FidelityEpi{}, ChainEpi{}, s);
 }
 
 hipError_t launch_band_image_fidelity(int n, int G, int nimg, const float2* Ib,
                                       const float2* tw_n, const FidelityEpi& epi, hipStream_t s) {
   if (!epi.target || !epi.dose || !epi.partial || epi.ncond < 1 || nimg % epi.ncond != 0)
     return hipErrorInvalidValue;
-  return band_image_g<true>(n, G, nimg, Ib, tw_n, nullptr, epi, s);
+  return band_image_g<1>(n, G, nimg, Ib, tw_n, nullptr, epi, ChainEpi{}, s);
+}
+
+hipError_t launch_band_image_chain(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
+                                   const ChainEpi& ch, hipStream_t s) {
+  if (!ch.m || !ch.grad) return hipErrorInvalidValue;
+  return band_image_g<2>(n, G, nimg, Ib, tw_n, nullptr, FidelityEpi{}, ch, s);
 }
 
 }  // namespace huvk

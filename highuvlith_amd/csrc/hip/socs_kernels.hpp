@@ -89,4 +89,17 @@ struct FidelityEpi {
 hipError_t launch_band_image_fidelity(int n, int G, int nimg, const float2* Ib,
                                       const float2* tw_n, const FidelityEpi& epi, hipStream_t s);
 
+// The ILT chain rule evaluated where band_image would store the image: image
+// j is dC/dm of ILT row j, and row (orows ? orows[j] : j) of grad receives
+// dC/dtheta = (dC/dm + reg) beta m (1 - m) (m, reg: rows j; reg optional).
+struct ChainEpi {
+  const float* m = nullptr;
+  const float* reg = nullptr;
+  const int* orows = nullptr;
+  float beta = 0.f;
+  float* grad = nullptr;
+};
+hipError_t launch_band_image_chain(int n, int G, int nimg, const float2* Ib, const float2* tw_n,
+                                   const ChainEpi& ch, hipStream_t s);
+
 }  // namespace huvk
