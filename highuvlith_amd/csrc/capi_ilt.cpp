@@ -509,7 +509,12 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   auto synced = [&]() { so = sf = 0; };  // after a stream synchronisation
   // costs of `count` evaluation rows (ilt_eval with device_cost false) from
   // their partial sums, after one synchronisation
+  // Rows whose max |d| and d.g partials a fused direction update left in the
+  // download area (valid until the area is written again).
+  std::vector<char> pre_row(T, 0);
+  bool pre_valid = false;
   auto get_costs = [&](int count, double* dst) -> bool {
+    pre_valid = false;
     e = hipStreamSynchronize(s);
     synced();
     if (e != hipSuccess) return false;
@@ -538,16 +543,8 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   // per-block partials and the host adds them in block order (the order the
   // device's own finishing kernel uses), which saves a launch per reduction.
   enum { AA = 0, AB = 1, BB = 2, MAXA = 3 };
-  auto reduce = [&](const float* a, const float* bv, std::initializer_list<int> ops,
-                    const std::vector<int>& rows) -> bool {
-    if (rows.empty()) return true;
-    const int* rd = stage_rows(rows);
-    if (e != hipSuccess || !flush()) return false;
-    int code = 0, r = 0;
-    for (int op : ops) code |= op << (4 * r++);
-    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, dl_dev,
-                                nullptr, s);
-    if (e != hipSuccess) return false;
+  // h_red from the partials in the download area, after one synchronisation
+  auto sum_partials = [&](std::initializer_list<int> ops, const std::vector<int>& rows) -> bool {
     e = hipStreamSynchronize(s);
     synced();
     if (e != hipSuccess) return false;
@@ -563,6 +560,19 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     }
     return true;
   };
+  auto reduce = [&](const float* a, const float* bv, std::initializer_list<int> ops,
+                    const std::vector<int>& rows) -> bool {
+    if (rows.empty()) return true;
+    pre_valid = false;
+    const int* rd = stage_rows(rows);
+    if (e != hipSuccess || !flush()) return false;
+    int code = 0, r = 0;
+    for (int op : ops) code |= op << (4 * r++);
+    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, dl_dev,
+                                nullptr, s);
+    if (e != hipSuccess) return false;
+    return sum_partials(ops, rows);
+  };
   // d = a g + b d on the listed rows (a, b from h_a, h_b)
   auto dir_rows = [&](const float* g, const std::vector<int>& rows) -> bool {
     if (rows.empty()) return true;
@@ -572,6 +582,32 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     if (e != hipSuccess || !ca || !cb || !flush()) return false;
     e = huvk::launch_vec_dir(g, ca, cb, N, (int)rows.size(), rd, dir, s);
     return e == hipSuccess;
+  };
+  // dir_rows followed by what the next iteration first asks of the new d,
+  // max |d| and d.g (reduce(dir, g, {MAXA, AB})), computed in the same pass;
+  // dir_reduce_take then only sums the partials.
+  auto dir_reduce_rows = [&](const float* g, const std::vector<int>& rows) -> bool {
+    pre_valid = false;
+    if (rows.empty()) return true;
+    const double* ca = (const double*)put(h_a.data(), T * sizeof(double));
+    const double* cb = (const double*)put(h_b.data(), T * sizeof(double));
+    const int* rd = stage_rows(rows);
+    if (e != hipSuccess || !ca || !cb || !flush()) return false;
+    e = huvk::launch_vec_dir_reduce(g, ca, cb, N, (int)rows.size(), rd, nblk, T, dir, dl_dev, s);
+    if (e != hipSuccess) return false;
+    std::fill(pre_row.begin(), pre_row.end(), 0);
+    for (int t : rows) pre_row[t] = 1;
+    pre_valid = true;
+    return true;
+  };
+  // reduce(dir, grad, {MAXA, AB}, rows), from the fused update's partials
+  // when it covered every row
+  auto dir_reduce_take = [&](const std::vector<int>& rows) -> bool {
+    bool have = pre_valid;
+    for (int t : rows) have = have && pre_row[t];
+    if (rows.empty() || !have) return reduce(dir, grad, {MAXA, AB}, rows);
+    pre_valid = false;
+    return sum_partials({MAXA, AB}, rows);
   };
   // Targets that are done cost nothing: evaluations read the rows of the
   // targets in `which` in place through a row list and write their gradients
@@ -645,7 +681,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     if (tr.gnorm_hist) tr.gnorm_hist[(size_t)t * H] = gnorm[t];
   }
   for (int t = 0; t < T; ++t) { h_a[t] = -1.0; h_b[t] = 0.0; }
-  if (!dir_rows(grad, all_t)) return fail_e("dir");
+  if (!dir_reduce_rows(grad, all_t)) return fail_e("dir");
   // the local proxy runs steepest descent and accepts on plain decrease (ilt.rs:767-801)
   const bool proxy = p->proxy;
   const bool cg = use_cg != 0 && !proxy;
@@ -668,7 +704,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       lst.clear();
       for (int t = 0; t < T; ++t)
         if (active[t] && !accepted[t] && attempt[t] == att) lst.push_back(t);
-      if (!reduce(dir, grad, {MAXA, AB}, lst)) return fail_e("maxabs / dot");
+      if (!dir_reduce_take(lst)) return fail_e("maxabs / dot");
       rst.clear();
       for (int t : lst) {
         dmax[t] = h_red[t];
@@ -820,7 +856,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       }
       if (active[t] && it[t] >= limit[t]) active[t] = 0;  // MaxIterations
     }
-    if (!dir_rows(gnew, which)) return fail_e("dir");
+    if (!dir_reduce_rows(gnew, which)) return fail_e("dir");
     // The new gradients become the current ones by exchanging the buffers:
     // every target still active next iteration is in `which`, so the rows
     // read later are all current; the other rows are never read again.

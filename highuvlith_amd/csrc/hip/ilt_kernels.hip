@@ -689,6 +689,50 @@ __global__ void k_vec_dir(const float* __restrict__ g, const double* __restrict_
     d[base + i] = (float)(ca * (double)g[base + i] + cb * (double)d[base + i]);
 }
 
+// k_vec_dir fused with the two reductions that follow a direction update,
+// max |d| and d.g of the new d: the grid, pixel order and block tree of
+// k_vec_reduce<2> with ops (3, 1), so part[] holds exactly what that
+// reduction would write (part[(r * T + t) * nblk + block], r = 0 max |d|,
+// r = 1 d.g).
+__global__ __launch_bounds__(PW_THREADS) void k_vec_dir_reduce(
+    const float* __restrict__ g, const double* __restrict__ a, const double* __restrict__ b,
+    int64_t npix, int nblk, int T, const int* __restrict__ rows, float* __restrict__ d,
+    double* __restrict__ part) {
+  __shared__ double sh[2][PW_THREADS];
+  const int t = row_of(rows, blockIdx.y);
+  const size_t base = (size_t)t * npix;
+  const double ca = a[t], cb = b[t];
+  const int64_t chunk = (npix + nblk - 1) / nblk;
+  const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += PW_THREADS) {
+    // The rounding of each step is spelled out: the update as k_vec_dir
+    // compiles it (cb d rounded, then one fused multiply-add with ca g), the
+    // dot product's terms as k_vec_reduce's runtime op switch compiles them
+    // (product and sum rounded separately).
+#pragma clang fp contract(off)
+    const float gv = g[base + i];
+    const double cbd = cb * (double)d[base + i];
+    const float dv = (float)fma(ca, (double)gv, cbd);
+    d[base + i] = dv;
+    acc0 = fmax(acc0, (double)fabsf(dv));
+    const double prod = (double)dv * (double)gv;
+    acc1 = acc1 + prod;
+  }
+  sh[0][threadIdx.x] = acc0;
+  sh[1][threadIdx.x] = acc1;
+  __syncthreads();
+  for (int s = PW_THREADS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + s]);
+      sh[1][threadIdx.x] = sh[1][threadIdx.x] + sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2)
+    part[((size_t)threadIdx.x * T + t) * nblk + blockIdx.x] = sh[threadIdx.x][0];
+}
+
 // Line-search trials: out[j] = x[rows[j]] + coef[j] d[rows[j]] (compact rows).
 __global__ void k_vec_axpy_gather(const float* __restrict__ x, const float* __restrict__ d,
                                   const double* __restrict__ coef, int64_t npix,
@@ -911,6 +955,16 @@ hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int6
                           int count, const int* rows, float* d, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_vec_dir, rows_grid(npix, count), dim3(256), 0, s, g, a, b, npix, rows, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_dir_reduce(const float* g, const double* a, const double* b, int64_t npix,
+                                 int count, const int* rows, int nblk, int T, float* d,
+                                 double* part, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count > T || nblk < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_vec_dir_reduce, dim3(nblk, count), dim3(PW_THREADS), 0, s, g, a, b, npix,
+                     nblk, T, rows, d, part);
   return hipGetLastError();
 }
 

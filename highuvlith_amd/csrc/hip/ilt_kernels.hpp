@@ -64,6 +64,12 @@ hipError_t launch_vec_reduce(const float* a, const float* b, int64_t npix, int c
 // d = a[t] g + b[t] d
 hipError_t launch_vec_dir(const float* g, const double* a, const double* b, int64_t npix,
                           int count, const int* rows, float* d, hipStream_t s);
+// launch_vec_dir, and in the same pass the reductions max |d| (r = 0) and d.g
+// (r = 1) of the new d over the listed rows: part as launch_vec_reduce with
+// ops (3, 1) would write it, bit for bit.
+hipError_t launch_vec_dir_reduce(const float* g, const double* a, const double* b, int64_t npix,
+                                 int count, const int* rows, int nblk, int T, float* d,
+                                 double* part, hipStream_t s);
 // dst[j] = src[idx[j]] (gather) / dst[idx[j]] = src[j] (scatter), rows of npix.
 hipError_t launch_rows_gather(const float* src, const int* idx, int64_t npix, int nrows,
                               float* dst, hipStream_t s);
