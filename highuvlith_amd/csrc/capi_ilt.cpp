@@ -649,7 +649,8 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   };
   std::vector<int> which;
   // line-search trial rows: target, coefficient, step; per-target first row and count
-  std::vector<int> prow, pfirst(T), pcount(T), used(T), acc_src, acc_dst;
+  std::vector<int> prow, pfirst(T), pcount(T), used(T), acc_dst;
+  std::vector<double> acc_coef;
   std::vector<double> pcoef, pstep;
 
   const int H = max_iterations + 1;
@@ -699,6 +700,10 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     for (int t = 0; t < T; ++t) any = any || active[t];
     if (!any) break;
     for (int t = 0; t < T; ++t) { attempt[t] = 0; accepted[t] = 0; searching[t] = 0; }
+    // accepted steps of this iteration, applied to theta in one launch after
+    // the line search (until then only the searching targets' rows are read)
+    acc_dst.clear();
+    acc_coef.clear();
     // Up to two attempts (CG direction, then steepest descent).
     for (int att = 0; att < 2; ++att) {
       lst.clear();
@@ -763,8 +768,6 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
         st = ilt_eval(p, np, targets, prow_d, trial, nullptr, nullptr, b, false);
         if (st != HUV_OK) return st;
         if (!get_costs(np, cost_c.data())) return fail_e("cost copy");
-        acc_src.clear();
-        acc_dst.clear();
         for (int t : which) {
           bool ok = false;
           for (int j = pfirst[t]; j < pfirst[t] + pcount[t]; ++j) {
@@ -774,8 +777,8 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
               searching[t] = 0;
               accepted[t] = 1;
               s_used[t] = pstep[j];
-              acc_src.push_back(j);
               acc_dst.push_back(t);
+              acc_coef.push_back(pcoef[j]);
               ok = true;
               break;
             }
@@ -785,17 +788,6 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
             s_try[t] = pstep[pfirst[t] + pcount[t] - 1] * 0.5;
             if (used[t] >= 25) searching[t] = 0;  // exhausted
           }
-        }
-        if (!acc_src.empty()) {
-          // theta += coef dir in place, rounded exactly as the trial was
-          std::vector<double> acc_coef(acc_src.size());
-          for (size_t k = 0; k < acc_src.size(); ++k) acc_coef[k] = pcoef[acc_src[k]];
-          const int* dr = (const int*)put(acc_dst.data(), acc_dst.size() * sizeof(int));
-          const double* cf =
-              (const double*)put(acc_coef.data(), acc_coef.size() * sizeof(double));
-          if (!dr || !cf || !flush()) return fail_e("accept staging");
-          e = huvk::launch_vec_axpy_rows(theta, dir, cf, N, (int)acc_src.size(), dr, s);
-          if (e != hipSuccess) return fail_e("accept copy");
         }
       }
       // Targets whose search failed: retry once along -grad with CG, else stop.
@@ -823,6 +815,15 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       }
     }
     if (!any_acc) break;
+    {
+      // theta += coef dir in place for every accepted target, rounded exactly
+      // as its trial point was
+      const int* dr = (const int*)put(acc_dst.data(), acc_dst.size() * sizeof(int));
+      const double* cf = (const double*)put(acc_coef.data(), acc_coef.size() * sizeof(double));
+      if (!dr || !cf || !flush()) return fail_e("accept staging");
+      e = huvk::launch_vec_axpy_rows(theta, dir, cf, N, (int)acc_dst.size(), dr, s);
+      if (e != hipSuccess) return fail_e("accept copy");
+    }
     which.clear();
     for (int t = 0; t < T; ++t)
       if (active[t] && accepted[t]) which.push_back(t);
