@@ -491,10 +491,15 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   // every use is read before the next launch that writes it.
   const bool has_reg = p->cfg.tv_weight > 0.0 || p->cfg.bin_weight > 0.0;
   const size_t fid_n = (size_t)p->ncond * (p->n / 32);  // fidelity partials per row
-  const size_t dl_count = std::max((size_t)kRed * T * nblk, (size_t)T * (fid_n + kBpi));
+  // Evaluation partials first, reduction partials after them, so an
+  // evaluation and a reduction can share one synchronisation.
+  const size_t red_off = ((size_t)T * (fid_n + kBpi) + 31) & ~(size_t)31;
+  const size_t dl_count = red_off + (size_t)kRed * T * nblk;
   if (!c->download_area(dl_count * sizeof(double), e)) return fail_e("download buffer");
   double* dl = (double*)c->dl_h;
   double* dl_dev = (double*)c->dl_d;
+  double* dlr = dl + red_off;          // reduction partials (host view)
+  double* dlr_dev = dl_dev + red_off;  // the same, device view
   b.fid_part = dl_dev;
   b.reg_part = dl_dev + (size_t)T * fid_n;
   size_t so = 0, sf = 0;
@@ -513,12 +518,15 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   // download area (valid until the area is written again).
   std::vector<char> pre_row(T, 0);
   bool pre_valid = false;
+  // host_costs reads what the last evaluation left, after a synchronisation
+  auto host_costs = [&](int count, double* dst) {
+    ilt_host_costs(p, count, dl, has_reg ? dl + (size_t)T * fid_n : nullptr, dst);
+  };
   auto get_costs = [&](int count, double* dst) -> bool {
-    pre_valid = false;
     e = hipStreamSynchronize(s);
     synced();
     if (e != hipSuccess) return false;
-    ilt_host_costs(p, count, dl, has_reg ? dl + (size_t)T * fid_n : nullptr, dst);
+    host_costs(count, dst);
     return true;
   };
   // One synchronisation epoch stages at most ~32 T bytes plus alignment.
@@ -551,7 +559,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     int q = 0;
     for (int op : ops) {
       for (int t : rows) {
-        const double* pk = dl + ((size_t)q * T + t) * nblk;
+        const double* pk = dlr + ((size_t)q * T + t) * nblk;
         double acc = 0.0;
         for (int k = 0; k < nblk; ++k) acc = op == MAXA ? std::fmax(acc, pk[k]) : acc + pk[k];
         h_red[(size_t)q * T + t] = acc;
@@ -568,7 +576,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     if (e != hipSuccess || !flush()) return false;
     int code = 0, r = 0;
     for (int op : ops) code |= op << (4 * r++);
-    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, dl_dev,
+    e = huvk::launch_vec_reduce(a, bv, N, (int)rows.size(), rd, nblk, code, r, T, dlr_dev,
                                 nullptr, s);
     if (e != hipSuccess) return false;
     return sum_partials(ops, rows);
@@ -593,7 +601,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     const double* cb = (const double*)put(h_b.data(), T * sizeof(double));
     const int* rd = stage_rows(rows);
     if (e != hipSuccess || !ca || !cb || !flush()) return false;
-    e = huvk::launch_vec_dir_reduce(g, ca, cb, N, (int)rows.size(), rd, nblk, T, dir, dl_dev, s);
+    e = huvk::launch_vec_dir_reduce(g, ca, cb, N, (int)rows.size(), rd, nblk, T, dir, dlr_dev, s);
     if (e != hipSuccess) return false;
     std::fill(pre_row.begin(), pre_row.end(), 0);
     for (int t : rows) pre_row[t] = 1;
@@ -618,8 +626,9 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
   float* th_c = (float*)cmp;
   float* g_c = (float*)(cmp + vb);
   std::vector<double> cost_c(T);
+  // defer: launch only; take_costs fills `out` after a later synchronisation
   auto eval_sel = [&](const float* th, float* g, std::vector<double>& out,
-                      const std::vector<int>& which) -> huv_status {
+                      const std::vector<int>& which, bool defer = false) -> huv_status {
     const int Tc = (int)which.size();
     if (Tc == 0) return HUV_OK;
     if (Tc == T) {
@@ -627,7 +636,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       ThetaRows src;
       src.theta = th;
       huv_status r = ilt_eval(p, T, targets, nullptr, src, g, nullptr, b, false);
-      if (r != HUV_OK) return r;
+      if (r != HUV_OK || defer) return r;
       return get_costs(T, out.data()) ? HUV_OK : hip_fail(e, "cost copy");
     }
     const int* idx = stage_rows(which);
@@ -637,10 +646,20 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     src.rows = idx;
     src.stage = th_c;
     huv_status r = ilt_eval(p, Tc, targets, idx, src, g, idx, b, false);
-    if (r != HUV_OK) return r;
+    if (r != HUV_OK || defer) return r;
     if (!get_costs(Tc, cost_c.data())) return hip_fail(e, "cost copy");
     for (int j = 0; j < Tc; ++j) out[which[j]] = cost_c[j];
     return HUV_OK;
+  };
+  auto take_costs = [&](std::vector<double>& out, const std::vector<int>& which) {
+    const int Tc = (int)which.size();
+    if (Tc == 0) return;
+    if (Tc == T) {
+      host_costs(T, out.data());
+      return;
+    }
+    host_costs(Tc, cost_c.data());
+    for (int j = 0; j < Tc; ++j) out[which[j]] = cost_c[j];
   };
   std::vector<int> all_t(T);
   for (int t = 0; t < T; ++t) all_t[t] = t;
@@ -827,10 +846,12 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     which.clear();
     for (int t = 0; t < T; ++t)
       if (active[t] && accepted[t]) which.push_back(t);
-    st = eval_sel(theta, gnew, cnew, which);
+    // the gradient evaluation's costs come with the CG reductions' synchronisation
+    st = eval_sel(theta, gnew, cnew, which, true);
     if (st != HUV_OK) return st;
     // CG update: beta = max(0, (g'.g' - g'.g) / g.g)
     if (!reduce(grad, gnew, {AA, AB, BB}, which)) return fail_e("dot");
+    take_costs(cnew, which);
     const double* gg = h_red.data();
     const double* gng = gg + T;
     const double* gngn = gng + T;
