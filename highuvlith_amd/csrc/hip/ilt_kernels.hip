@@ -611,14 +611,21 @@ __device__ __forceinline__ int row_of(const int* __restrict__ rows, int j) {
 // reduction keeps the fixed order of a single one: per-thread FP64 sums over
 // the block's strided pixels, then a tree over the block.
 // part[(r * T + t) * nblk + block].
-__device__ __forceinline__ double reduce_term(int op, float av, float bv) {
+// acc after one more pixel of reduction op, each product and sum rounded on
+// its own (no fused multiply-add), whatever the caller's contraction setting.
+__device__ __forceinline__ double reduce_step(int op, double acc, float av, float bv) {
+#pragma clang fp contract(off)
   switch (op) {
-    case 0: return (double)av * (double)av;
-    case 1: return (double)av * (double)bv;
-    case 2: return (double)bv * (double)bv;
-    default: return (double)fabsf(av);
+    case 0: return acc + (double)av * (double)av;
+    case 1: return acc + (double)av * (double)bv;
+    case 2: return acc + (double)bv * (double)bv;
+    default: return fmax(acc, (double)fabsf(av));
   }
 }
+
+// Pixels per thread whose loads are issued together before their terms are
+// added (in pixel order, so the sums are those of one pixel at a time).
+constexpr int kReduceBatch = 8;
 
 template <int NOPS>
 __global__ __launch_bounds__(PW_THREADS) void k_vec_reduce(const float* __restrict__ a,
@@ -638,12 +645,21 @@ __global__ __launch_bounds__(PW_THREADS) void k_vec_reduce(const float* __restri
   double acc[NOPS];
 #pragma unroll
   for (int r = 0; r < NOPS; ++r) acc[r] = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += PW_THREADS) {
-    const float av = aa[i], bv = bb ? bb[i] : 0.0f;
+  constexpr int U = kReduceBatch;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)U * PW_THREADS) {
+    float av[U], bv[U];
 #pragma unroll
-    for (int r = 0; r < NOPS; ++r) {
-      const double v = reduce_term(op[r], av, bv);
-      acc[r] = op[r] == 3 ? fmax(acc[r], v) : acc[r] + v;
+    for (int u = 0; u < U; ++u) {
+      // past the block's end: a valid pixel, loaded and then ignored
+      const int64_t i = min(i0 + (int64_t)u * PW_THREADS, hi - 1);
+      av[u] = aa[i];
+      bv[u] = bb ? bb[i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i0 + (int64_t)u * PW_THREADS >= hi) break;
+#pragma unroll
+      for (int r = 0; r < NOPS; ++r) acc[r] = reduce_step(op[r], acc[r], av[u], bv[u]);
     }
   }
 #pragma unroll
@@ -705,19 +721,29 @@ __global__ __launch_bounds__(PW_THREADS) void k_vec_dir_reduce(
   const int64_t chunk = (npix + nblk - 1) / nblk;
   const int64_t lo = blockIdx.x * chunk, hi = min(npix, lo + chunk);
   double acc0 = 0.0, acc1 = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += PW_THREADS) {
+  constexpr int U = kReduceBatch;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (int64_t)U * PW_THREADS) {
     // The rounding of each step is spelled out: the update as k_vec_dir
     // compiles it (cb d rounded, then one fused multiply-add with ca g), the
-    // dot product's terms as k_vec_reduce's runtime op switch compiles them
-    // (product and sum rounded separately).
+    // reductions as reduce_step.
 #pragma clang fp contract(off)
-    const float gv = g[base + i];
-    const double cbd = cb * (double)d[base + i];
-    const float dv = (float)fma(ca, (double)gv, cbd);
-    d[base + i] = dv;
-    acc0 = fmax(acc0, (double)fabsf(dv));
-    const double prod = (double)dv * (double)gv;
-    acc1 = acc1 + prod;
+    float gv[U], dold[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = min(i0 + (int64_t)u * PW_THREADS, hi - 1);
+      gv[u] = g[base + i];
+      dold[u] = d[base + i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * PW_THREADS;
+      if (i >= hi) break;
+      const double cbd = cb * (double)dold[u];
+      const float dv = (float)fma(ca, (double)gv[u], cbd);
+      d[base + i] = dv;
+      acc0 = reduce_step(3, acc0, dv, gv[u]);
+      acc1 = reduce_step(1, acc1, dv, gv[u]);
+    }
   }
   sh[0][threadIdx.x] = acc0;
   sh[1][threadIdx.x] = acc1;
