@@ -505,6 +505,19 @@ def bench_ilt(args, H, W, np, torch):
            "mean_initial_cost": float(np.mean([h[0] for h in res.cost_history])),
            "mean_final_cost": float(np.mean(res.final_cost)),
            }
+    # Compulsory HBM traffic of a CG iteration: theta, gradient and direction
+    # read once, 12 B per pixel and target.  The iteration is a chain of
+    # dependent launches with host decisions between them, so its bound is
+    # launch and synchronisation latency, not HBM; the fraction says how far.
+    npix = grid.size * grid.size
+    comp = 12.0 * npix * len(targets)
+    ms_it = res.wall_s / args.ilt_iterations * 1e3
+    out["roofline"] = {"bound": "latency (dependent launches and host round trips)",
+                       "compulsory_bytes_per_iteration": comp,
+                       "compulsory_ms_at_peak": round(comp / (HBM_PEAK_GBS * 1e9) * 1e3, 4),
+                       "achieved_GBps": round(comp / (ms_it * 1e-3) / 1e9, 1),
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(comp / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_cpu:
         # the oracle's run_stage (FP64, OpenMP over kernels) on the first target,
         # 3 iterations from the same start
