@@ -696,6 +696,10 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     }
   }
   if (!reduce(grad, nullptr, {AA}, all_t)) return fail_e("dot");
+  // g.g of each target's current gradient: the previous iteration's g'.g'
+  // (same row, same kernel and order), so the CG update reduces only g.g'
+  // and g'.g'
+  std::vector<double> gsq(h_red.begin(), h_red.begin() + T);
   for (int t = 0; t < T; ++t) {
     gnorm[t] = std::sqrt(h_red[t]);
     if (tr.gnorm_hist) tr.gnorm_hist[(size_t)t * H] = gnorm[t];
@@ -850,10 +854,10 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
     st = eval_sel(theta, gnew, cnew, which, true);
     if (st != HUV_OK) return st;
     // CG update: beta = max(0, (g'.g' - g'.g) / g.g)
-    if (!reduce(grad, gnew, {AA, AB, BB}, which)) return fail_e("dot");
+    if (!reduce(grad, gnew, {AB, BB}, which)) return fail_e("dot");
     take_costs(cnew, which);
-    const double* gg = h_red.data();
-    const double* gng = gg + T;
+    const double* gg = gsq.data();
+    const double* gng = h_red.data();
     const double* gngn = gng + T;
     for (int t : which) {
       const double prev = cost[t];
@@ -863,6 +867,7 @@ huv_status ilt_optimize(huv_ilt* p, int T, const float* targets, float* theta, i
       gnorm[t] = std::sqrt(gngn[t]);
       if (tr.gnorm_hist) tr.gnorm_hist[(size_t)t * H + it[t]] = gnorm[t];
       const double beta = cg ? (gg[t] > 0.0 ? std::max(0.0, (gngn[t] - gng[t]) / gg[t]) : 0.0) : 0.0;
+      gsq[t] = gngn[t];
       h_a[t] = -1.0;
       h_b[t] = beta;
       step[t] = std::min(s_used[t] * 1.5, 10.0 * learning_rate);
